@@ -1,0 +1,237 @@
+/*
+ * rt.h — C ABI of the MI355X (gfx950) ray-tracing library.
+ *
+ * The library replaces, for the reference renderer (KaminariOS/rustracer,
+ * gltf_viewer), the Vulkan ray-tracing pipeline: the acceleration-structure build
+ * that vkCmdBuildAccelerationStructuresKHR performs and the raygen / closest-hit /
+ * any-hit / miss shader stages that vkCmdTraceRaysKHR runs.  It consumes the
+ * renderer's own buffers (layouts in rt_layout.h) and writes the same rgba32f
+ * accumulation image.
+ *
+ * Conventions
+ *  - Every call returns RT_OK (0) or a negative rt_status; rt_last_error() gives a
+ *    message.  No call aborts the process.
+ *  - Scene buffers and images are caller-owned HIP device allocations.  BVH memory
+ *    is library-owned, behind rt_blas / rt_tlas handles freed explicitly (the
+ *    reference frees acceleration structures in Drop, crates/libs/vulkan/src/
+ *    ray_tracing/acceleration_structure.rs:270-277).
+ *  - One context per host thread.  Calls enqueue work on the context's HIP stream
+ *    and return; rt_sync() gives the fence semantics the reference gets from
+ *    fence.wait (crates/libs/asset_loader/src/acceleration_structures.rs:237-243).
+ */
+#ifndef RT_H
+#define RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "rt_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INVALID_ARGUMENT = -1,
+    RT_ERR_HIP = -2,
+    RT_ERR_OUT_OF_MEMORY = -3,
+    RT_ERR_STACK_OVERFLOW = -4,
+    RT_ERR_QUEUE_OVERFLOW = -5,
+    RT_ERR_UNSUPPORTED = -6,
+    RT_ERR_INVALID_HANDLE = -7
+} rt_status;
+
+typedef struct rt_ctx rt_ctx;
+typedef uint64_t rt_blas; /* 0 is never a valid handle */
+typedef uint64_t rt_tlas;
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `hip_device`.  `hip_stream` is a hipStream_t to
+ * enqueue on, or NULL for a stream owned by the context. */
+int rt_create(rt_ctx** out, int hip_device, void* hip_stream);
+void rt_destroy(rt_ctx* ctx);
+/* Message of the last failed call on this context ("" if none).  ctx may be NULL
+ * for failures of rt_create. */
+const char* rt_last_error(const rt_ctx* ctx);
+/* Blocks until all work enqueued on the context has finished. */
+int rt_sync(rt_ctx* ctx);
+/* Library version string. */
+const char* rt_version(void);
+
+/* ------------------------------------------------- bottom-level (per mesh) */
+
+enum { RT_GEOMETRY_OPAQUE = 0x1 };
+
+/* One triangle geometry, as primitive_to_vk_geometry fills
+ * VkAccelerationStructureGeometryTrianglesDataKHR + BuildRangeInfo
+ * (crates/libs/asset_loader/src/acceleration_structures.rs:17-76):
+ * vertex i of triangle p is at
+ *   vertex_data + (first_vertex + index_data[primitive_offset/4 + 3p + i]) * vertex_stride
+ * and holds three floats (R32G32B32_SFLOAT). */
+typedef struct rt_blas_input {
+    const void* vertex_data;   /* device address of the vertex buffer */
+    uint32_t vertex_stride;    /* bytes; 128 for rt_vertex */
+    uint32_t max_vertex;       /* highest index value used (bounds check) */
+    const uint32_t* index_data;/* device address of the u32 index buffer */
+    uint32_t first_vertex;     /* PrimInfo.v_offset */
+    uint32_t primitive_offset; /* bytes into index_data: PrimInfo.i_offset * 4 */
+    uint32_t primitive_count;  /* triangles */
+    uint32_t geometry_flags;   /* RT_GEOMETRY_OPAQUE iff the material is OPAQUE */
+} rt_blas_input;
+
+/* Builds one BLAS per input, all in one batched build (one Morton sort over all
+ * triangles), as create_as records one build per primitive into one command
+ * buffer (acceleration_structures.rs:79-129,
+ * crates/libs/vulkan/src/ray_tracing/acceleration_structure.rs:199-221).
+ * `inputs` is a host array; out[i] receives the handle of inputs[i]. */
+int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count, rt_blas* out);
+int rt_build_blas(rt_ctx* ctx, const rt_blas_input* input, rt_blas* out);
+/* 64-bit reference to place in rt_instance.acceleration_structure_reference
+ * (vkGetAccelerationStructureDeviceAddressKHR,
+ * crates/libs/vulkan/src/ray_tracing/acceleration_structure.rs:160-168). */
+int rt_blas_device_address(rt_ctx* ctx, rt_blas blas, uint64_t* out);
+void rt_destroy_blas(rt_ctx* ctx, rt_blas blas);
+
+/* --------------------------------------------------- top-level (instances) */
+
+/* Builds a TLAS over `count` instances held in device memory in the 64-byte
+ * VkAccelerationStructureInstanceKHR layout (create_top_as,
+ * acceleration_structures.rs:136-249).  Every acceleration_structure_reference
+ * must come from rt_blas_device_address() of a live BLAS. */
+int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, rt_tlas* out);
+void rt_destroy_tlas(rt_ctx* ctx, rt_tlas tlas);
+
+/* ----------------------------------------------------------- trace (frame) */
+
+enum { RT_FMT_RGBA8_UNORM = 0, RT_FMT_RGBA8_SRGB = 1, RT_FMT_RGBA32F = 2 };
+enum { RT_WRAP_REPEAT = 0, RT_WRAP_CLAMP_TO_EDGE = 1, RT_WRAP_MIRRORED_REPEAT = 2 };
+enum { RT_FILTER_NEAREST = 0, RT_FILTER_LINEAR = 1 };
+
+/* One combined image sampler of the bindless texture array (binding 6).
+ * Texel (x, y) is at data + (y * width + x) * texel_size.  Sampled at LOD 0. */
+typedef struct rt_texture {
+    const void* data;          /* device pointer */
+    uint32_t width, height;
+    uint32_t format;           /* RT_FMT_* */
+    uint32_t wrap_s, wrap_t;   /* RT_WRAP_* */
+    uint32_t mag_filter;       /* RT_FILTER_* (LOD 0 samples with the mag filter) */
+    uint32_t min_filter;
+} rt_texture;
+
+/* The descriptor bindings of the reference pipeline
+ * (crates/examples/gltf_viewer/shaders/lib/RayTracingCommons.glsl:123-136). */
+typedef struct rt_bindings {
+    const rt_ubo* ubo;                 /* HOST pointer, read when the call enqueues (binding 2) */
+    const rt_vertex* vertices;         /* binding 3 */
+    const uint32_t* indices;           /* binding 4 */
+    const rt_prim_info* prim_infos;    /* binding 5, indexed by instance custom index */
+    const rt_material_raw* materials;  /* binding 9 */
+    const rt_light* dlights;           /* binding 10 (directional, used by miss) */
+    uint32_t num_dlights;
+    const rt_light* plights;           /* binding 11 (point, used by closest-hit) */
+    uint32_t num_plights;
+    const rt_texture* textures;        /* binding 6: HOST array of descriptors */
+    uint32_t num_textures;
+    uint32_t num_vertices;             /* element counts, used for bounds checks */
+    uint32_t num_indices;
+    uint32_t num_prim_infos;
+    uint32_t num_materials;
+} rt_bindings;
+
+enum { RT_MODE_MEGAKERNEL = 0, RT_MODE_WAVEFRONT = 1 };
+
+/* Counters of one trace call (filled when a non-NULL pointer is passed; the call
+ * then runs the counting variant of the kernels and synchronises). */
+typedef struct rt_trace_stats {
+    uint64_t rays;            /* camera + extension + shadow ray segments */
+    uint64_t primary_rays;    /* camera rays */
+    uint64_t extension_rays;  /* bounce rays */
+    uint64_t shadow_rays;
+    uint64_t node_visits;     /* wide BVH nodes fetched (TLAS + BLAS) */
+    uint64_t tri_tests;       /* ray-triangle tests */
+    uint64_t instance_visits; /* TLAS leaf instances entered */
+    uint64_t shade_events;    /* closest-hit shader invocations */
+    uint64_t anyhit_calls;    /* any-hit shader invocations */
+    uint64_t tex_fetches;     /* texture() calls */
+    uint64_t light_candidates;/* RIS candidate light reads */
+    uint64_t stack_overflows; /* rays whose traversal stack overflowed */
+    double kernel_ms;         /* device time of the trace kernels */
+} rt_trace_stats;
+
+/* Runs the whole ray-tracing pipeline for a width x height launch, as
+ * record_raytracing_commands + trace_rays do
+ * (crates/examples/gltf_viewer/src/main.rs:244-270, crates/libs/vulkan/src/command.rs:374-392):
+ * per pixel, ubo->number_of_samples samples of up to ubo->number_of_bounces traces,
+ * accumulated into `accum` (rgba32f, row-major, width*height*4 floats):
+ *   accum = (number_of_samples != total_number_of_samples ? accum : 0) + sum(radiance), alpha = 0.
+ * `out_rgba8` (nullable) receives the tonemapped display image.
+ * `mode` selects the plain megakernel or the wavefront pipeline; both produce the
+ * same image bit for bit. */
+int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
+                  uint8_t* out_rgba8, uint32_t width, uint32_t height, uint32_t mode,
+                  rt_trace_stats* stats);
+
+/* ---------------------------------------------------- ray queries (parity) */
+
+/* Ray flags (gl_RayFlags*EXT values). */
+enum {
+    RT_RAY_FLAG_NONE = 0x00,
+    RT_RAY_FLAG_OPAQUE = 0x01,
+    RT_RAY_FLAG_NO_OPAQUE = 0x02,
+    RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT = 0x04,
+    RT_RAY_FLAG_SKIP_CLOSEST_HIT = 0x08
+};
+
+typedef struct rt_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} rt_ray;
+
+/* Result of one traversal: primitive_id == 0xFFFFFFFF on a miss. */
+typedef struct rt_hit {
+    float t;
+    float u, v;              /* barycentrics of vertices 1 and 2 */
+    uint32_t instance_id;    /* gl_InstanceID */
+    uint32_t primitive_id;   /* gl_PrimitiveID */
+    uint32_t custom_index;   /* gl_InstanceCustomIndexEXT */
+    uint32_t _pad[2];
+} rt_hit;
+
+/* Traverses `count` rays (device arrays) with the closest-hit rule used by
+ * traceRayEXT: nearest accepted hit, ties in t broken by the smaller
+ * (instance_id, primitive_id).  `b` may be NULL when every geometry is opaque or
+ * RT_RAY_FLAG_OPAQUE is set; otherwise it supplies the alpha-test inputs. */
+int rt_trace_ray_queries(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b,
+                         const rt_ray* d_rays, uint32_t count, uint32_t ray_flags,
+                         rt_hit* d_hits);
+
+/* Primary-ray hits of the camera in `b->ubo` (sample 0 of every pixel, jitter as
+ * the raygen shader draws it): one rt_hit per pixel, row-major. */
+int rt_trace_primary(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
+                     uint32_t height, rt_hit* d_hits);
+
+/* ------------------------------------------------------------- inspection */
+
+/* Layout of a built BLAS, for validity checks.  Node records are 128 bytes
+ * (4-wide, fp32 child boxes); triangle records are 48 bytes. */
+typedef struct rt_bvh_info {
+    uint64_t nodes;          /* device pointer to the node array */
+    uint64_t tris;           /* device pointer to the triangle/instance records */
+    uint32_t root;           /* root node index */
+    uint32_t num_nodes;      /* nodes reachable from root (contiguous from root) */
+    uint32_t first_prim;     /* first record index of this structure */
+    uint32_t num_prims;
+    float bounds[6];         /* lo xyz, hi xyz */
+} rt_bvh_info;
+
+int rt_blas_info(rt_ctx* ctx, rt_blas blas, rt_bvh_info* out);
+int rt_tlas_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* RT_H */

@@ -1,0 +1,206 @@
+"""ctypes binding of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg, never by the library.  Builds liboracle.so on first use when it
+is missing (g++ only, no GPU needed).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+SRC = os.path.join(HERE, "oracle.cpp")
+
+
+def build(force=False):
+    """Compile liboracle.so with g++ (FP64 + FP32 instantiations, OpenMP)."""
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    if not force and os.path.exists(LIB_PATH) and os.path.getmtime(LIB_PATH) >= max(
+            os.path.getmtime(SRC), os.path.getmtime(os.path.join(HERE, "oracle.h"))):
+        return LIB_PATH
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off",
+           "-o", LIB_PATH + ".tmp", SRC]
+    subprocess.check_call(cmd)
+    os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    return LIB_PATH
+
+
+class OrcBlas(C.Structure):
+    _fields_ = [("first_vertex", C.c_uint32), ("index_offset", C.c_uint32), ("prim_count", C.c_uint32),
+                ("opaque", C.c_uint32)]
+
+
+class RtTexture(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
+                ("wrap_s", C.c_uint32), ("wrap_t", C.c_uint32), ("mag_filter", C.c_uint32),
+                ("min_filter", C.c_uint32)]
+
+
+class OrcSceneDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint64),
+                ("indices", C.c_void_p), ("num_indices", C.c_uint64),
+                ("prim_infos", C.c_void_p), ("num_prim_infos", C.c_uint32),
+                ("materials", C.c_void_p), ("num_materials", C.c_uint32),
+                ("dlights", C.c_void_p), ("num_dlights", C.c_uint32),
+                ("plights", C.c_void_p), ("num_plights", C.c_uint32),
+                ("textures", C.c_void_p), ("num_textures", C.c_uint32),
+                ("blases", C.c_void_p), ("num_blases", C.c_uint32),
+                ("instances", C.c_void_p), ("num_instances", C.c_uint32)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("primary_rays", "extension_rays", "shadow_rays", "node_visits",
+                                            "tri_tests", "shade_events", "anyhit_calls")]
+
+
+class OrcMatBrdf(C.Structure):
+    _fields_ = [("base_color", C.c_double * 3), ("metallic", C.c_double), ("roughness", C.c_double),
+                ("ior", C.c_double), ("transmission", C.c_double), ("specular_factor", C.c_double),
+                ("specular_color_factor", C.c_double * 3), ("use_spec", C.c_uint32), ("front_face", C.c_uint32),
+                ("volume", C.c_uint32), ("workflow_sg", C.c_uint32), ("sg_specular", C.c_double * 3),
+                ("attenuation_color", C.c_double * 3), ("attenuation_distance", C.c_double),
+                ("F0", C.c_double * 3), ("F90", C.c_double * 3), ("c_diff", C.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        d, u, i, vp = C.c_double, C.c_uint32, C.c_int, C.c_void_p
+        dp = C.POINTER(C.c_double)
+        sig = {
+            "orc_rand_sequence": (u, [vp, u, vp]), "orc_tea16": (u, [u, u]), "orc_pcg4d": (None, [vp, vp]),
+            "orc_lcg_sequence": (u, [u, u, vp]), "orc_stream_b_seed": (u, [u, u]),
+            "orc_mat_build": (None, [vp]), "orc_eval_combined_brdf": (None, [dp, dp, dp, vp, dp]),
+            "orc_brdf_probability": (None, [vp, dp, dp, dp]),
+            "orc_sample_indirect": (i, [dp, dp, dp, dp, vp, u, dp, dp]),
+            "orc_ggx_d": (d, [d, d]), "orc_smith_g2_height_correlated": (d, [d, d, d]),
+            "orc_smith_g1_ggx": (d, [d, d, d, d]), "orc_sample_ggx_vndf": (None, [dp, d, dp, dp]),
+            "orc_sample_hemisphere": (None, [dp, dp]), "orc_rotation_to_z": (None, [dp, dp]),
+            "orc_rotate_point": (None, [dp, dp, dp]), "orc_convert_metallic": (d, [dp, dp, d]),
+            "orc_frostbite_diffuse": (d, [d, d, d, d]), "orc_light_intensity_luminance": (d, [vp, d]),
+            "orc_offset_ray": (None, [vp, vp, vp]), "orc_texture_sample": (None, [vp, d, d, dp]),
+            "orc_intersect_triangle": (i, [dp, dp, dp, dp, dp, d, d, dp]),
+            "orc_scene_create": (vp, [vp]), "orc_scene_destroy": (None, [vp]),
+            "orc_trace_rays": (i, [vp, vp, u, u, vp, vp, i, i]),
+            "orc_trace_primary": (i, [vp, vp, u, u, vp, vp, i]),
+            "orc_render": (i, [vp, vp, u, u, u, u, u, u, i, i, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def vec(*x):
+    return np.ascontiguousarray(np.array(x, np.float64).reshape(-1))
+
+
+def matbrdf(base_color=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.0, transmission=0.0,
+            specular_factor=1.0, specular_color=(1, 1, 1), front_face=True, volume=False, sg_specular=None):
+    m = OrcMatBrdf()
+    m.base_color[:] = base_color
+    m.metallic, m.roughness, m.ior, m.transmission = metallic, roughness, ior, transmission
+    m.specular_factor = specular_factor
+    m.specular_color_factor[:] = specular_color
+    m.front_face, m.volume = int(front_face), int(volume)
+    m.attenuation_color[:] = (1, 1, 1)
+    m.attenuation_distance = 3.4e38
+    if sg_specular is not None:
+        m.workflow_sg = 1
+        m.sg_specular[:] = sg_specular
+    lib().orc_mat_build(C.byref(m))
+    return m
+
+
+class OracleScene:
+    """An oracle scene built from a rustracer_amd.scenes.Scene (host arrays)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        self._keep = []
+        geo = np.ascontiguousarray(scene.geometries)
+        inst = np.ascontiguousarray(scene.instances)
+        self._textures = (RtTexture * max(1, len(scene.textures)))()
+        for k, t in enumerate(scene.textures):
+            data = np.ascontiguousarray(t.data)
+            self._keep.append(data)
+            h, w = data.shape[:2]
+            self._textures[k] = RtTexture(data.ctypes.data, w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter,
+                                          t.min_filter)
+        arrays = dict(vertices=np.ascontiguousarray(scene.vertices), indices=np.ascontiguousarray(scene.indices),
+                      prim_infos=np.ascontiguousarray(scene.prim_infos),
+                      materials=np.ascontiguousarray(scene.materials),
+                      dlights=np.ascontiguousarray(scene.dlights), plights=np.ascontiguousarray(scene.plights),
+                      geo=geo, inst=inst)
+        self._keep.extend(arrays.values())
+        a = arrays
+        self.desc = OrcSceneDesc(a["vertices"].ctypes.data, len(a["vertices"]), a["indices"].ctypes.data,
+                                 len(a["indices"]), a["prim_infos"].ctypes.data, len(a["prim_infos"]),
+                                 a["materials"].ctypes.data, len(a["materials"]), a["dlights"].ctypes.data,
+                                 len(a["dlights"]), a["plights"].ctypes.data, len(a["plights"]),
+                                 C.addressof(self._textures), len(scene.textures), geo.ctypes.data, len(geo),
+                                 inst.ctypes.data, len(inst))
+        self.handle = lib().orc_scene_create(C.byref(self.desc))
+        if not self.handle:
+            raise ValueError("oracle rejected the scene (instance references out of range)")
+
+    def close(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.orc_scene_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace_rays(self, rays, flags=0, brute=False, threads=0):
+        """rays: RAY structured array.  Returns (HIT array, (n,3) float64 t/u/v)."""
+        from rustracer_amd import layouts as L
+        rays = np.ascontiguousarray(rays)
+        hits = np.zeros(len(rays), L.HIT)
+        tuv = np.zeros((len(rays), 3))
+        lib().orc_trace_rays(self.handle, rays.ctypes.data, len(rays), flags, hits.ctypes.data, tuv.ctypes.data,
+                             int(brute), threads)
+        return hits, tuv
+
+    def trace_primary(self, ubo=None, width=None, height=None, threads=0):
+        from rustracer_amd import layouts as L
+        s = self.scene
+        ubo = np.ascontiguousarray(s.ubo if ubo is None else ubo)
+        w, h = width or s.width, height or s.height
+        hits = np.zeros(w * h, L.HIT)
+        tuv = np.zeros((w * h, 3))
+        lib().orc_trace_primary(self.handle, ubo.ctypes.data, w, h, hits.ctypes.data, tuv.ctypes.data, threads)
+        return hits, tuv
+
+    def render(self, ubo=None, width=None, height=None, rect=None, precision=64, threads=0, accum=None):
+        """Returns (accum (h,w,4) float64, stats dict)."""
+        s = self.scene
+        ubo = np.ascontiguousarray(s.ubo if ubo is None else ubo)
+        w, h = width or s.width, height or s.height
+        if accum is None:
+            accum = np.zeros((h, w, 4), np.float64)
+        accum = np.ascontiguousarray(accum, np.float64)
+        x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
+        st = OrcStats()
+        rc = lib().orc_render(self.handle, ubo.ctypes.data, w, h, x0, y0, x1, y1, precision, threads,
+                              accum.ctypes.data, C.byref(st))
+        if rc != 0:
+            raise ValueError("orc_render rejected the arguments")
+        stats = {n: getattr(st, n) for n, _ in OrcStats._fields_}
+        stats["rays"] = stats["primary_rays"] + stats["extension_rays"] + stats["shadow_rays"]
+        return accum, stats

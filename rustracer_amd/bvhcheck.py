@@ -1,0 +1,81 @@
+"""Host-side validity checks of a built BVH (used by tests and the smoke run).
+
+A BLAS is valid when, walking from its root: every child reference is in range,
+no node is reached twice, every triangle record of the BLAS lies in exactly one
+leaf, every child box contains the boxes of its subtree and every leaf box
+contains its triangles (in float, no tolerance)."""
+import numpy as np
+
+from . import rt
+
+TRI = np.dtype([("v0", "<f4", 3), ("prim", "<u4"), ("v1", "<f4", 4), ("v2", "<f4", 4)])
+EMPTY = 0xFFFFFFFF
+
+
+def check_blas(ctx, blas, expect_prims=None):
+    info = blas.info()
+    nodes = rt.read_nodes(ctx, info)
+    n_total = info.first_prim + info.num_prims
+    tris = np.frombuffer(rt.read_device(info.tris, 48 * max(1, n_total)), TRI)[:n_total] if n_total else np.zeros(0, TRI)
+    return _check(nodes, info.root, info.first_prim, info.num_prims, tris, "triangle", expect_prims)
+
+
+def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
+    errors = []
+    seen_nodes = np.zeros(len(nodes), bool)
+    covered = np.zeros(num_prims, np.int32)
+    stack = [(root, None)]
+    while stack:
+        ni, parent_box = stack.pop()
+        if ni >= len(nodes):
+            errors.append(f"node index {ni} out of range {len(nodes)}")
+            continue
+        if seen_nodes[ni]:
+            errors.append(f"node {ni} reached twice")
+            continue
+        seen_nodes[ni] = True
+        nd = nodes[ni]
+        for k in range(4):
+            c, cnt = int(nd["child"][k]), int(nd["count"][k])
+            if c == EMPTY:
+                continue
+            lo = np.array([nd["lo_x"][k], nd["lo_y"][k], nd["lo_z"][k]], np.float32)
+            hi = np.array([nd["hi_x"][k], nd["hi_y"][k], nd["hi_z"][k]], np.float32)
+            if parent_box is not None and (np.any(lo < parent_box[0]) or np.any(hi > parent_box[1])):
+                errors.append(f"node {ni} child {k} box not inside parent box")
+            if cnt == 0:
+                stack.append((c, (lo, hi)))
+                continue
+            if c < first_prim or c + cnt > first_prim + num_prims:
+                errors.append(f"leaf [{c},{c + cnt}) outside [{first_prim},{first_prim + num_prims})")
+                continue
+            covered[c - first_prim:c - first_prim + cnt] += 1
+            if kind == "triangle":
+                t = tris[c:c + cnt]
+                pts = np.concatenate([t["v0"], t["v1"][:, :3], t["v2"][:, :3]])
+                if np.any(pts < lo) or np.any(pts > hi):
+                    errors.append(f"leaf [{c},{c + cnt}) triangles not inside leaf box")
+    if np.any(covered != 1):
+        bad = np.nonzero(covered != 1)[0]
+        errors.append(f"{len(bad)} records not in exactly one leaf (first {bad[:5] + first_prim}, counts {covered[bad[:5]]})")
+    if kind == "triangle" and num_prims:
+        ids = np.sort(tris["prim"][first_prim:first_prim + num_prims])
+        if not np.array_equal(ids, np.arange(num_prims, dtype=np.uint32)):
+            errors.append("primitive ids of the BLAS are not a permutation of 0..n-1")
+    if expect_prims is not None and expect_prims != num_prims:
+        errors.append(f"BLAS has {num_prims} primitives, expected {expect_prims}")
+    return errors
+
+
+INST = np.dtype([("w2o", "<f4", (3, 4)), ("o2w", "<f4", (3, 4)), ("blas", "<u8"), ("custom_index", "<u4"),
+                 ("mask", "<u4"), ("flags", "<u4"), ("instance_id", "<u4"), ("_pad", "<u4", 2)])
+
+
+def check_tlas(ctx, tlas, num_instances):
+    info = tlas.info()
+    nodes = rt.read_nodes(ctx, info)
+    errs = _check(nodes, info.root, 0, info.num_prims, None, "instance", num_instances)
+    inst = np.frombuffer(rt.read_device(info.tris, 128 * max(1, info.num_prims)), INST)[:info.num_prims]
+    if num_instances and not np.array_equal(np.sort(inst["instance_id"]), np.arange(num_instances)):
+        errs.append("TLAS instance ids are not a permutation")
+    return errs

@@ -1,0 +1,632 @@
+// api.hip — the C ABI (include/rt/rt.h): contexts, acceleration-structure handles,
+// trace entry points.
+//
+// Mirrors the reference's operator surface: create_as / create_top_as
+// (crates/libs/asset_loader/src/acceleration_structures.rs:79-249) for builds and
+// record_raytracing_commands -> trace_rays (crates/examples/gltf_viewer/src/main.rs:244-270,
+// crates/libs/vulkan/src/command.rs:374-392) for frames.  Every entry point returns
+// a status code; nothing aborts.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "lbvh.h"
+#include "render.h"
+#include "wavefront.h"
+
+using namespace rt;
+
+namespace {
+
+constexpr uint32_t kBlasLeafMax = 4;
+constexpr uint32_t kTlasLeafMax = 1;
+
+std::string g_create_error;
+
+struct DevFree {
+    void operator()(void* p) const { if (p) (void)hipFree(p); }
+};
+using DevPtr = std::unique_ptr<void, DevFree>;
+
+// Device memory of one batched BLAS build, shared by the BLASes of the batch and
+// kept alive by every TLAS that references one of them.
+struct BlasPool {
+    DevPtr nodes, tris, descs;
+    size_t bytes = 0;
+};
+
+struct BlasEntry {
+    std::shared_ptr<BlasPool> pool;
+    BlasDesc* d_desc;
+    BlasDesc h_desc;
+    uint32_t num_nodes_pool;
+};
+
+struct TlasEntry {
+    DevPtr nodes, inst;
+    uint32_t num_nodes = 0, num_inst = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<std::shared_ptr<BlasPool>> pools;
+    bool any_non_opaque = false;
+};
+
+}  // namespace
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    uint64_t next_handle = 1;
+    std::unordered_map<uint64_t, BlasEntry> blas;
+    std::unordered_map<uint64_t, uint64_t> blas_by_addr;
+    std::unordered_map<uint64_t, TlasEntry> tlas;
+    unsigned long long* d_counters = nullptr;
+    uint32_t* d_err = nullptr;  // kErr* flags raised by kernels, checked at rt_sync
+    float* d_srgb = nullptr;
+    rt_texture* d_textures = nullptr;
+    uint32_t tex_cap = 0;
+    std::vector<rt_texture> tex_cache;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    WavefrontState* wavefront = nullptr;
+};
+
+namespace {
+
+int fail(rt_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+int hip_fail(rt_ctx* ctx, hipError_t e, const char* where) {
+    return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+#define RT_HIP(ctx, call, where)                          \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, where); \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <class T> T* dev_alloc(size_t count, hipError_t* e) {
+    void* p = nullptr;
+    *e = hipMalloc(&p, count * sizeof(T) + 16);
+    return (T*)p;
+}
+
+void invert34(const double m[12], double r[12]) {
+    double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    double inv = det != 0.0 ? 1.0 / det : 0.0;
+    double r3[9] = {A * inv, -(b * i - c * h) * inv, (b * f - c * e) * inv, B * inv, (a * i - c * g) * inv,
+                    -(a * f - c * d) * inv, C * inv, -(a * h - b * g) * inv, (a * e - b * d) * inv};
+    for (int row = 0; row < 3; row++) {
+        for (int col = 0; col < 3; col++) r[row * 4 + col] = r3[row * 3 + col];
+        r[row * 4 + 3] = -(r3[row * 3 + 0] * m[3] + r3[row * 3 + 1] * m[7] + r3[row * 3 + 2] * m[11]);
+    }
+}
+
+int validate_bindings(rt_ctx* ctx, const rt_bindings* b) {
+    if (!b || !b->ubo) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "bindings and bindings->ubo are required");
+    if (!b->vertices || !b->indices || !b->prim_infos || !b->materials)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "vertex, index, prim-info and material buffers are required");
+    if (!b->num_vertices || !b->num_indices || !b->num_prim_infos || !b->num_materials)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "buffer element counts must be non-zero");
+    if (b->num_dlights && !b->dlights) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "dlights is NULL");
+    if (b->num_plights && !b->plights) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "plights is NULL");
+    if (b->num_textures && !b->textures) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "textures is NULL");
+    for (uint32_t i = 0; i < b->num_textures; i++) {
+        const rt_texture& t = b->textures[i];
+        if (!t.data || !t.width || !t.height || t.format > RT_FMT_RGBA32F || t.wrap_s > RT_WRAP_MIRRORED_REPEAT ||
+            t.wrap_t > RT_WRAP_MIRRORED_REPEAT || t.mag_filter > RT_FILTER_LINEAR || t.min_filter > RT_FILTER_LINEAR)
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has an invalid descriptor");
+    }
+    return RT_OK;
+}
+
+// Device copy of the texture descriptors; re-uploaded only when they change.
+int upload_textures(rt_ctx* ctx, const rt_bindings* b) {
+    uint32_t n = b->num_textures;
+    if (n == 0) return RT_OK;
+    bool same = ctx->tex_cache.size() == n &&
+                std::memcmp(ctx->tex_cache.data(), b->textures, sizeof(rt_texture) * n) == 0;
+    if (same) return RT_OK;
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (n > ctx->tex_cap) {
+        if (ctx->d_textures) (void)hipFree(ctx->d_textures);
+        ctx->d_textures = nullptr;
+        hipError_t e;
+        ctx->d_textures = dev_alloc<rt_texture>(n, &e);
+        if (e != hipSuccess) { ctx->tex_cap = 0; return hip_fail(ctx, e, "hipMalloc textures"); }
+        ctx->tex_cap = n;
+    }
+    RT_HIP(ctx, hipMemcpy(ctx->d_textures, b->textures, sizeof(rt_texture) * n, hipMemcpyHostToDevice), "copy textures");
+    ctx->tex_cache.assign(b->textures, b->textures + n);
+    return RT_OK;
+}
+
+int make_frame(rt_ctx* ctx, const TlasEntry& tl, const rt_bindings* b, float* accum, uint32_t w, uint32_t h,
+               FrameArgs* fa) {
+    const rt_ubo& u = *b->ubo;
+    std::memset(fa, 0, sizeof(*fa));
+    fa->tlas = TlasView{(const WideNode*)tl.nodes.get(), (const InstanceRecord*)tl.inst.get(), 0u, tl.num_inst,
+                        ctx->d_err};
+    SceneView& sv = fa->sv;
+    sv.vertices = b->vertices;
+    sv.indices = b->indices;
+    sv.prim_infos = b->prim_infos;
+    sv.materials = b->materials;
+    sv.dlights = b->dlights;
+    sv.plights = b->plights;
+    sv.textures = ctx->d_textures;
+    sv.srgb_lut = ctx->d_srgb;
+    sv.num_dlights = b->num_dlights;
+    sv.num_plights = b->num_plights;
+    sv.num_textures = b->num_textures;
+    sv.num_vertices = b->num_vertices;
+    sv.num_indices = b->num_indices;
+    sv.num_prim_infos = b->num_prim_infos;
+    sv.num_materials = b->num_materials;
+    fa->accum = accum;
+    fa->width = w;
+    fa->height = h;
+    // stream B / LCG seed: replaces clockARB() of RayTracing.rgen:33,40
+    fa->seed_b = tea16(u.random_seed, u.frame_count);
+    fa->reset = u.number_of_samples == u.total_number_of_samples;
+    fa->counters = ctx->d_counters;
+    std::memcpy(fa->mvi, u.model_view_inverse, sizeof(fa->mvi));
+    std::memcpy(fa->pin, u.projection_inverse, sizeof(fa->pin));
+    fa->aperture = u.aperture;
+    fa->focus_distance = u.focus_distance;
+    fa->ortho = u.orthographic_fov_dis;
+    fa->exposure = u.exposure;
+    fa->samples = u.number_of_samples;
+    fa->bounces = u.number_of_bounces;
+    fa->antialiasing = u.antialiasing;
+    fa->mapping = u.mapping;
+    fa->frame_count = u.frame_count;
+    fa->fully_opaque = u.fully_opaque;
+    return RT_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- context
+extern "C" {
+
+const char* rt_version(void) { return "rt-gfx950 0.1.0"; }
+
+const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int rt_create(rt_ctx** out, int hip_device, void* hip_stream) {
+    if (!out) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, RT_ERR_HIP, "no HIP device available");
+    if (hip_device < 0 || hip_device >= n) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "hip_device out of range");
+    auto* ctx = new rt_ctx();
+    ctx->device = hip_device;
+    DeviceGuard g(hip_device);
+    if (hip_stream) {
+        ctx->stream = (hipStream_t)hip_stream;
+    } else {
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { delete ctx; return fail(nullptr, RT_ERR_HIP, "hipStreamCreate failed"); }
+        ctx->own_stream = true;
+    }
+    ctx->d_counters = dev_alloc<unsigned long long>(kNumCounters, &e);
+    if (e == hipSuccess) ctx->d_srgb = dev_alloc<float>(256, &e);
+    if (e == hipSuccess) ctx->d_err = dev_alloc<uint32_t>(1, &e);
+    if (e == hipSuccess) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) {
+        float lut[256];
+        for (int i = 0; i < 256; i++) {  // sRGB EOTF of the Vulkan spec, evaluated in double
+            double c = i / 255.0;
+            lut[i] = float(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+        }
+        e = hipMemcpy(ctx->d_srgb, lut, sizeof(lut), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        std::string msg = std::string("context set-up failed: ") + hipGetErrorString(e);
+        rt_destroy(ctx);
+        return fail(nullptr, RT_ERR_HIP, msg);
+    }
+    *out = ctx;
+    return RT_OK;
+}
+
+void rt_destroy(rt_ctx* ctx) {
+    if (!ctx) return;
+    DeviceGuard g(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->tlas.clear();
+    ctx->blas.clear();
+    if (ctx->wavefront) wavefront_destroy(ctx->wavefront);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->d_textures) (void)hipFree(ctx->d_textures);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int rt_sync(rt_ctx* ctx) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    DeviceGuard g(ctx->device);
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    uint32_t flags = 0;
+    RT_HIP(ctx, hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost), "read error flags");
+    if (flags) {
+        RT_HIP(ctx, hipMemset(ctx->d_err, 0, sizeof(uint32_t)), "reset error flags");
+        if (flags & kErrStackOverflow)
+            return fail(ctx, RT_ERR_STACK_OVERFLOW, "a traversal exceeded its stack restart budget; results are incomplete");
+    }
+    return RT_OK;
+}
+
+// ------------------------------------------------------------------ BLAS
+int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count, rt_blas* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!inputs || !out || count == 0) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "inputs/out must be non-NULL, count > 0");
+    DeviceGuard g(ctx->device);
+    std::vector<TriSegment> segs(count);
+    std::vector<uint32_t> seg_start(count + 1);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const rt_blas_input& in = inputs[i];
+        if (in.primitive_count > 0) {
+            if (!in.vertex_data || !in.index_data)
+                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": NULL vertex or index data");
+            if (in.vertex_stride < 12 || (in.vertex_stride & 3u))
+                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": vertex_stride must be >= 12 and a multiple of 4");
+            if (in.primitive_offset & 3u)
+                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": primitive_offset must be a multiple of 4");
+        }
+        seg_start[i] = uint32_t(total);
+        segs[i] = TriSegment{(const uint8_t*)in.vertex_data,
+                             in.index_data ? in.index_data + in.primitive_offset / 4 : nullptr,
+                             in.vertex_stride, in.max_vertex, in.first_vertex, in.primitive_count, uint32_t(total), 0u};
+        total += in.primitive_count;
+        if (total >= (1ull << kLeafCountShift))
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "batch exceeds 2^27 triangles");
+    }
+    seg_start[count] = uint32_t(total);
+    const uint32_t n = uint32_t(total);
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    DevPtr d_segs(dev_alloc<TriSegment>(count, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_seg_start(dev_alloc<uint32_t>(count + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_tri_in(dev_alloc<float4>(3 * size_t(n) + 3, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_lo(dev_alloc<float4>(n + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_hi(dev_alloc<float4>(n + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_bad(dev_alloc<uint32_t>(1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    RT_HIP(ctx, hipMemcpyAsync(d_segs.get(), segs.data(), sizeof(TriSegment) * count, hipMemcpyHostToDevice, st), "copy segments");
+    RT_HIP(ctx, hipMemcpyAsync(d_seg_start.get(), seg_start.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice, st), "copy segments");
+    RT_HIP(ctx, hipMemsetAsync(d_bad.get(), 0, sizeof(uint32_t), st), "memset");
+    int rc = prepare_triangles(st, (const TriSegment*)d_segs.get(), count, n, (float4*)d_tri_in.get(),
+                               (float4*)d_lo.get(), (float4*)d_hi.get(), (uint32_t*)d_bad.get());
+    if (rc) return hip_fail(ctx, hipError_t(rc), "prepare_triangles");
+    LbvhResult res;
+    std::string err;
+    rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), n, (const uint32_t*)d_seg_start.get(),
+                    seg_start, kBlasLeafMax, &res, &err);
+    if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "BLAS build: " + err);
+    DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
+    auto pool = std::make_shared<BlasPool>();
+    pool->nodes.reset(res.nodes);
+    pool->tris.reset(dev_alloc<float4>(3 * size_t(n) + 3, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    pool->descs.reset(dev_alloc<BlasDesc>(count, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    pool->bytes = sizeof(WideNode) * size_t(res.num_nodes) + 48 * size_t(n) + sizeof(BlasDesc) * count;
+    rc = reorder_triangles(st, (const float4*)d_tri_in.get(), (const uint32_t*)perm.get(), n, (float4*)pool->tris.get());
+    if (rc) return hip_fail(ctx, hipError_t(rc), "reorder_triangles");
+    std::vector<float> hb(6 * size_t(count));
+    uint32_t bad = 0;
+    RT_HIP(ctx, hipMemcpyAsync(hb.data(), seg_bounds.get(), sizeof(float) * hb.size(), hipMemcpyDeviceToHost, st), "copy bounds");
+    RT_HIP(ctx, hipMemcpyAsync(&bad, d_bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, st), "copy flag");
+    RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (bad) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "an index exceeds max_vertex");
+    std::vector<BlasDesc> descs(count);
+    for (uint32_t i = 0; i < count; i++) {
+        BlasDesc& d = descs[i];
+        d.nodes = (const WideNode*)pool->nodes.get();
+        d.tris = (const float4*)pool->tris.get();
+        d.root = i;
+        d.num_prims = inputs[i].primitive_count;
+        d.first_prim = seg_start[i];
+        d.flags = (inputs[i].geometry_flags & RT_GEOMETRY_OPAQUE) ? kBlasOpaque : 0u;
+        std::memcpy(d.bounds, &hb[6 * size_t(i)], sizeof(d.bounds));
+    }
+    RT_HIP(ctx, hipMemcpy(pool->descs.get(), descs.data(), sizeof(BlasDesc) * count, hipMemcpyHostToDevice), "copy descs");
+    for (uint32_t i = 0; i < count; i++) {
+        uint64_t hnd = ctx->next_handle++;
+        BlasEntry be{pool, (BlasDesc*)pool->descs.get() + i, descs[i], res.num_nodes};
+        ctx->blas.emplace(hnd, be);
+        ctx->blas_by_addr[uint64_t(uintptr_t(be.d_desc))] = hnd;
+        out[i] = hnd;
+    }
+    return RT_OK;
+}
+
+int rt_build_blas(rt_ctx* ctx, const rt_blas_input* input, rt_blas* out) {
+    return rt_build_blas_batch(ctx, input, 1, out);
+}
+
+int rt_blas_device_address(rt_ctx* ctx, rt_blas blas, uint64_t* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out is NULL");
+    auto it = ctx->blas.find(blas);
+    if (it == ctx->blas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown BLAS handle");
+    *out = uint64_t(uintptr_t(it->second.d_desc));
+    return RT_OK;
+}
+
+void rt_destroy_blas(rt_ctx* ctx, rt_blas blas) {
+    if (!ctx) return;
+    auto it = ctx->blas.find(blas);
+    if (it == ctx->blas.end()) return;
+    DeviceGuard g(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->blas_by_addr.erase(uint64_t(uintptr_t(it->second.d_desc)));
+    ctx->blas.erase(it);
+}
+
+int rt_blas_info(rt_ctx* ctx, rt_blas blas, rt_bvh_info* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out is NULL");
+    auto it = ctx->blas.find(blas);
+    if (it == ctx->blas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown BLAS handle");
+    const BlasDesc& d = it->second.h_desc;
+    out->nodes = uint64_t(uintptr_t(d.nodes));
+    out->tris = uint64_t(uintptr_t(d.tris));
+    out->root = d.root;
+    out->num_nodes = it->second.num_nodes_pool;
+    out->first_prim = d.first_prim;
+    out->num_prims = d.num_prims;
+    std::memcpy(out->bounds, d.bounds, sizeof(out->bounds));
+    return RT_OK;
+}
+
+// ------------------------------------------------------------------ TLAS
+int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, rt_tlas* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out || (count && !d_instances)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "instances/out must be non-NULL");
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    std::vector<rt_instance> hi(count);
+    if (count) {
+        RT_HIP(ctx, hipMemcpyAsync(hi.data(), d_instances, sizeof(rt_instance) * count, hipMemcpyDeviceToHost, st), "copy instances");
+        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+    TlasEntry te;
+    std::vector<InstanceRecord> recs(count);
+    for (uint32_t i = 0; i < count; i++) {
+        const rt_instance& in = hi[i];
+        auto it = ctx->blas_by_addr.find(in.acceleration_structure_reference);
+        if (it == ctx->blas_by_addr.end())
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": acceleration_structure_reference is not a live BLAS");
+        const BlasEntry& be = ctx->blas.at(it->second);
+        te.pools.push_back(be.pool);
+        double m[12], w[12];
+        for (int k = 0; k < 12; k++) {
+            m[k] = in.transform[k / 4][k % 4];
+            if (!std::isfinite(m[k])) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "instance " + std::to_string(i) + ": non-finite transform");
+        }
+        invert34(m, w);
+        InstanceRecord& r = recs[i];
+        std::memset(&r, 0, sizeof(r));
+        for (int row = 0; row < 3; row++) {
+            r.w2o[row] = make_float4(float(w[row * 4]), float(w[row * 4 + 1]), float(w[row * 4 + 2]), float(w[row * 4 + 3]));
+            r.o2w[row] = make_float4(in.transform[row][0], in.transform[row][1], in.transform[row][2], in.transform[row][3]);
+        }
+        r.blas = be.d_desc;
+        r.custom_index = in.custom_index_and_mask & 0xFFFFFFu;
+        r.mask = in.custom_index_and_mask >> 24;
+        uint32_t f = in.sbt_offset_and_flags >> 24;
+        r.flags = ((be.h_desc.flags & kBlasOpaque) ? kInstOpaqueGeom : 0u) |
+                  ((f & RT_INSTANCE_FORCE_OPAQUE) ? kInstForceOpaque : 0u) |
+                  ((f & RT_INSTANCE_FORCE_NO_OPAQUE) ? kInstForceNoOpaque : 0u) |
+                  (be.h_desc.num_prims == 0 ? kInstEmpty : 0u);
+        r.instance_id = i;
+        if (!(be.h_desc.flags & kBlasOpaque)) te.any_non_opaque = true;
+    }
+    hipError_t e;
+    DevPtr d_rec(dev_alloc<InstanceRecord>(count + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_lo(dev_alloc<float4>(count + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_hi(dev_alloc<float4>(count + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    std::vector<uint32_t> seg_start = {0u, count};
+    DevPtr d_seg(dev_alloc<uint32_t>(2, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    if (count) RT_HIP(ctx, hipMemcpyAsync(d_rec.get(), recs.data(), sizeof(InstanceRecord) * count, hipMemcpyHostToDevice, st), "copy records");
+    RT_HIP(ctx, hipMemcpyAsync(d_seg.get(), seg_start.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, st), "copy segments");
+    int rc = prepare_instances(st, (const InstanceRecord*)d_rec.get(), count, (float4*)d_lo.get(), (float4*)d_hi.get());
+    if (rc) return hip_fail(ctx, hipError_t(rc), "prepare_instances");
+    LbvhResult res;
+    std::string err;
+    rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), count, (const uint32_t*)d_seg.get(),
+                    seg_start, kTlasLeafMax, &res, &err);
+    if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "TLAS build: " + err);
+    DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
+    te.nodes.reset(res.nodes);
+    te.num_nodes = res.num_nodes;
+    te.inst.reset(dev_alloc<InstanceRecord>(count + 1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    te.num_inst = count;
+    rc = reorder_instances(st, (const InstanceRecord*)d_rec.get(), (const uint32_t*)perm.get(), count,
+                           (InstanceRecord*)te.inst.get());
+    if (rc) return hip_fail(ctx, hipError_t(rc), "reorder_instances");
+    RT_HIP(ctx, hipMemcpyAsync(te.bounds, seg_bounds.get(), sizeof(te.bounds), hipMemcpyDeviceToHost, st), "copy bounds");
+    RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    uint64_t hnd = ctx->next_handle++;
+    ctx->tlas.emplace(hnd, std::move(te));
+    *out = hnd;
+    return RT_OK;
+}
+
+void rt_destroy_tlas(rt_ctx* ctx, rt_tlas tlas) {
+    if (!ctx) return;
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return;
+    DeviceGuard g(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->tlas.erase(it);
+}
+
+int rt_tlas_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out is NULL");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    const TlasEntry& t = it->second;
+    out->nodes = uint64_t(uintptr_t(t.nodes.get()));
+    out->tris = uint64_t(uintptr_t(t.inst.get()));
+    out->root = 0;
+    out->num_nodes = t.num_nodes;
+    out->first_prim = 0;
+    out->num_prims = t.num_inst;
+    std::memcpy(out->bounds, t.bounds, sizeof(out->bounds));
+    return RT_OK;
+}
+
+// ----------------------------------------------------------------- trace
+int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum, uint8_t* out_rgba8, uint32_t width,
+                  uint32_t height, uint32_t mode, rt_trace_stats* stats) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    int rc = validate_bindings(ctx, b);
+    if (rc) return rc;
+    if (!accum) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "accum is NULL");
+    if (width == 0 || height == 0 || uint64_t(width) * height > (1ull << 31))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
+    if (mode != RT_MODE_MEGAKERNEL && mode != RT_MODE_WAVEFRONT) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown mode");
+    if (out_rgba8) return fail(ctx, RT_ERR_UNSUPPORTED, "the rgba8 display image is not produced by this build");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    if (b->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, "HasSky (skybox miss) is not supported");
+    DeviceGuard g(ctx->device);
+    rc = upload_textures(ctx, b);
+    if (rc) return rc;
+    FrameArgs fa;
+    make_frame(ctx, it->second, b, accum, width, height, &fa);
+    const bool any_hit = !fa.fully_opaque;
+    hipStream_t st = ctx->stream;
+    if (stats) {
+        RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
+        RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
+    }
+    if (mode == RT_MODE_MEGAKERNEL) {
+        rc = launch_megakernel(fa, st, any_hit, stats != nullptr);
+    } else {
+        if (!ctx->wavefront) {
+            ctx->wavefront = wavefront_create();
+            if (!ctx->wavefront) return fail(ctx, RT_ERR_OUT_OF_MEMORY, "wavefront state");
+        }
+        std::string err;
+        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, &err);
+        if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
+        if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
+    }
+    if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
+    if (stats) {
+        RT_HIP(ctx, hipEventRecord(ctx->ev1, st), "hipEventRecord");
+        unsigned long long c[kNumCounters];
+        RT_HIP(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost, st), "copy counters");
+        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+        float ms = 0.f;
+        RT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+        std::memset(stats, 0, sizeof(*stats));
+        stats->primary_rays = c[kCntPrimary];
+        stats->extension_rays = c[kCntExtension];
+        stats->shadow_rays = c[kCntShadow];
+        stats->rays = c[kCntPrimary] + c[kCntExtension] + c[kCntShadow];
+        stats->node_visits = c[kCntNodes];
+        stats->tri_tests = c[kCntTris];
+        stats->instance_visits = c[kCntInstances];
+        stats->shade_events = c[kCntShade];
+        stats->anyhit_calls = c[kCntAnyHit];
+        stats->tex_fetches = c[kCntTex];
+        stats->light_candidates = c[kCntLights];
+        stats->stack_overflows = c[kCntOverflow];
+        stats->kernel_ms = ms;
+        if (c[kCntBadIndex]) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a hit referenced an out-of-range prim info, material, index or vertex");
+    }
+    return RT_OK;
+}
+
+int rt_trace_ray_queries(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const rt_ray* d_rays, uint32_t count,
+                         uint32_t ray_flags, rt_hit* d_hits) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (count && (!d_rays || !d_hits)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "rays/hits must be non-NULL");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    const bool any_hit = b != nullptr && !(ray_flags & RT_RAY_FLAG_OPAQUE) && it->second.any_non_opaque;
+    if (b) {
+        int rc = validate_bindings(ctx, b);
+        if (rc) return rc;
+    }
+    DeviceGuard g(ctx->device);
+    FrameArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    fa.tlas = TlasView{(const WideNode*)it->second.nodes.get(), (const InstanceRecord*)it->second.inst.get(), 0u,
+                       it->second.num_inst, ctx->d_err};
+    uint32_t seed_b = 0;
+    if (b) {
+        int rc = upload_textures(ctx, b);
+        if (rc) return rc;
+        make_frame(ctx, it->second, b, nullptr, 1, 1, &fa);
+        seed_b = fa.seed_b;
+    }
+    int rc = launch_ray_queries(fa.tlas, fa.sv, d_rays, count, ray_flags | (any_hit ? 0u : RT_RAY_FLAG_OPAQUE), d_hits,
+                                seed_b, ctx->stream, any_hit);
+    if (rc) return hip_fail(ctx, hipError_t(rc), "ray query launch");
+    return RT_OK;
+}
+
+int rt_trace_primary(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height, rt_hit* d_hits) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    int rc = validate_bindings(ctx, b);
+    if (rc) return rc;
+    if (!d_hits || width == 0 || height == 0) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "hits/size invalid");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    DeviceGuard g(ctx->device);
+    rc = upload_textures(ctx, b);
+    if (rc) return rc;
+    FrameArgs fa;
+    make_frame(ctx, it->second, b, nullptr, width, height, &fa);
+    rc = launch_primary(fa, d_hits, ctx->stream, !fa.fully_opaque);
+    if (rc) return hip_fail(ctx, hipError_t(rc), "primary launch");
+    return RT_OK;
+}
+
+}  // extern "C"

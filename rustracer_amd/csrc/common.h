@@ -1,0 +1,111 @@
+// common.h — internal device layouts and small device helpers of the gfx950 library.
+//
+// Acceleration-structure memory (library-owned):
+//   WideNode   128 B: a 4-wide BVH node holding its children's fp32 boxes (SoA per
+//              axis, so one node is one 128-byte HBM transaction pair of dwordx4
+//              loads), child references and leaf counts.
+//   Triangle    48 B: three float4 {v.xyz, w}; v0.w carries the primitive index
+//              (gl_PrimitiveID) as bits.  Leaves reference contiguous runs.
+//   InstanceRecord 128 B: world->object and object->world 3x4 rows, the BLAS
+//              descriptor address, custom index, mask, flags, instance id.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rt/rt.h"
+
+namespace rt {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;   // stack entry / child reference: leaf
+constexpr int kLeafCountShift = 27;          // leaf entry: bits 27..30 = count - 1
+constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
+constexpr int kMaxLeafPrims = 16;
+
+struct alignas(128) WideNode {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    uint32_t child[4];  // internal: node index; leaf: first record; empty: kEmpty
+    uint32_t count[4];  // 0 for internal children, number of records for leaves
+};
+static_assert(sizeof(WideNode) == 128, "WideNode is 128 bytes");
+
+enum : uint32_t { kBlasOpaque = 1u };
+
+struct alignas(64) BlasDesc {
+    const WideNode* nodes;  // node pool base (shared by the BLASes of one batch)
+    const float4* tris;     // triangle pool base, 3 float4 per record
+    uint32_t root;          // root node index in the pool
+    uint32_t num_prims;
+    uint32_t first_prim;    // first triangle record of this BLAS
+    uint32_t flags;         // kBlasOpaque
+    float bounds[6];        // lo xyz, hi xyz (conservative)
+};
+static_assert(sizeof(BlasDesc) == 64, "BlasDesc is 64 bytes");
+
+enum : uint32_t {
+    kInstOpaqueGeom = 1u << 0,   // BLAS geometry flag OPAQUE
+    kInstForceOpaque = 1u << 1,  // VK_GEOMETRY_INSTANCE_FORCE_OPAQUE
+    kInstForceNoOpaque = 1u << 2,
+    kInstEmpty = 1u << 3         // BLAS without triangles
+};
+
+struct alignas(16) InstanceRecord {
+    float4 w2o[3];        // rows of the world-to-object 3x4 matrix
+    float4 o2w[3];        // rows of the object-to-world 3x4 matrix (gl_ObjectToWorldEXT)
+    const BlasDesc* blas;
+    uint32_t custom_index;
+    uint32_t mask;        // 8-bit visibility mask
+    uint32_t flags;       // kInst*
+    uint32_t instance_id; // gl_InstanceID (index in the caller's instance array)
+    uint32_t _pad[2];
+};
+static_assert(sizeof(InstanceRecord) == 128, "InstanceRecord is 128 bytes");
+
+enum : uint32_t { kErrStackOverflow = 1u };
+
+struct TlasView {
+    const WideNode* nodes;
+    const InstanceRecord* inst;  // in TLAS leaf order
+    uint32_t root;
+    uint32_t num_inst;
+    uint32_t* error_flags;       // kErr* bits raised by kernels (may be NULL)
+};
+
+// ------------------------------------------------------------ float3 math
+__host__ __device__ inline float3 mk3(float x, float y, float z) { return make_float3(x, y, z); }
+__host__ __device__ inline float3 operator+(float3 a, float3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline float3 operator-(float3 a, float3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline float3 operator-(float3 a) { return mk3(-a.x, -a.y, -a.z); }
+__host__ __device__ inline float3 operator*(float3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline float3 operator*(float s, float3 a) { return mk3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline float3 operator/(float3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__host__ __device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline float3 cross3(float3 a, float3 b) {
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline float length3(float3 a) { return sqrtf(dot3(a, a)); }
+__device__ inline float3 normalize3(float3 a) { return a / sqrtf(dot3(a, a)); }
+__host__ __device__ inline float3 xyz(float4 a) { return mk3(a.x, a.y, a.z); }
+__device__ inline float3 fmin3(float3 a, float3 b) { return mk3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+__device__ inline float3 fmax3(float3 a, float3 b) { return mk3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+__device__ inline float comp(float3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// Order-preserving float <-> uint for atomic min/max of floats.
+__device__ inline uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__device__ inline uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+
+// Sum of a 64-lane wave, result in every lane.
+template <class T> __device__ inline T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace rt

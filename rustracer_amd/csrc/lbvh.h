@@ -1,0 +1,62 @@
+// lbvh.h — batched LBVH builder (Morton codes, LDS radix sort, Karras hierarchy,
+// bottom-up box fit, collapse to 4-wide 128-byte nodes).
+//
+// It builds the structures the Vulkan driver builds for
+// vkCmdBuildAccelerationStructuresKHR in the reference
+// (crates/libs/vulkan/src/command.rs:354-372): one BVH per geometry (BLAS) and one
+// over the instances (TLAS).  Several BLASes are built in one pass: the segment id
+// sits above the Morton bits of each key, so one sort and one radix tree cover the
+// whole batch and every segment is its own subtree.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+namespace rt {
+
+// One segment (= one BLAS) of a batched triangle build.
+struct TriSegment {
+    const uint8_t* vertex_data;
+    const uint32_t* index_data;  // already offset by the primitive offset
+    uint32_t stride;
+    uint32_t max_vertex;
+    uint32_t first_vertex;
+    uint32_t prim_count;
+    uint32_t prim_start;         // first global primitive of the segment
+    uint32_t _pad;
+};
+
+// Scratch and results of one build; device pointers are owned by the caller
+// (api.hip) so that they can be pooled.
+struct LbvhResult {
+    WideNode* nodes = nullptr;      // final node array (exact size)
+    uint32_t num_nodes = 0;
+    uint32_t* perm = nullptr;       // sorted position -> input primitive index
+    float* seg_bounds = nullptr;    // 6 floats per segment (padded root box)
+};
+
+// Builds the wide BVH of `n` primitives with boxes (lo[i], hi[i]) split into
+// `nseg` segments (d_seg_start has nseg + 1 entries, on device and host).
+// Segment s's root is node s.  Returns a hipError_t-compatible code.
+int lbvh_build(hipStream_t stream, const float4* d_lo, const float4* d_hi, uint32_t n,
+               const uint32_t* d_seg_start, const std::vector<uint32_t>& h_seg_start,
+               uint32_t leaf_max, LbvhResult* out, std::string* err);
+
+// Triangle preparation for a BLAS batch: reads the caller's index and vertex
+// buffers, writes triangle records (input order) and boxes.  Sets *d_bad to 1 if an
+// index exceeds its segment's max_vertex (that triangle is then degenerate).
+int prepare_triangles(hipStream_t stream, const TriSegment* d_segs, uint32_t nseg, uint32_t n,
+                      float4* d_tri, float4* d_lo, float4* d_hi, uint32_t* d_bad);
+
+// Gathers triangle records into BVH leaf order.
+int reorder_triangles(hipStream_t stream, const float4* d_tri_in, const uint32_t* d_perm, uint32_t n,
+                      float4* d_tri_out);
+
+// Instance boxes (world space) for the TLAS build.
+int prepare_instances(hipStream_t stream, const InstanceRecord* d_inst, uint32_t n, float4* d_lo,
+                      float4* d_hi);
+int reorder_instances(hipStream_t stream, const InstanceRecord* d_in, const uint32_t* d_perm, uint32_t n,
+                      InstanceRecord* d_out);
+
+}  // namespace rt

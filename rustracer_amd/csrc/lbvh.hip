@@ -1,0 +1,757 @@
+// lbvh.hip — batched LBVH build for gfx950 (see lbvh.h).
+//
+// Pipeline per build (all on one stream):
+//   1. primitive boxes + per-segment centroid bounds (wave-reduced atomics)
+//   2. 64-bit keys: segment id above an interleaved Morton code
+//   3. LSD radix sort, 8 bits per pass: LDS histogram, three-kernel scan, and an
+//      LDS-staged stable scatter that ranks keys with wave ballots
+//   4. Karras (HPG 2012) radix-tree emission, one thread per internal node
+//   5. bottom-up box fit with acquire/release counters
+//   6. top-down collapse into 4-wide nodes, one kernel launch per level: each
+//      task opens the largest-surface-area child that is too big for a leaf until
+//      four children are gathered
+#include <string>
+
+#include "lbvh.h"
+
+namespace rt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kBlock * kRsItems;
+constexpr int kScanPerBlock = 1024;  // 256 threads x 4 elements
+
+#define RT_CHECK_LAUNCH()                              \
+    do {                                               \
+        hipError_t e_ = hipGetLastError();             \
+        if (e_ != hipSuccess) return int(e_);          \
+    } while (0)
+#define RT_CHECK(x)                                    \
+    do {                                               \
+        hipError_t e_ = (x);                           \
+        if (e_ != hipSuccess) return int(e_);          \
+    } while (0)
+
+inline uint32_t blocks_for(uint64_t n, int per) { return uint32_t((n + per - 1) / per); }
+
+__device__ inline uint32_t find_segment(const uint32_t* seg_start, uint32_t nseg, uint32_t p) {
+    uint32_t lo = 0, hi = nseg;  // seg_start[lo] <= p < seg_start[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (seg_start[mid] <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------ primitives
+__global__ void k_prepare_triangles(const TriSegment* segs, uint32_t nseg, uint32_t n, float4* tri, float4* lo,
+                                    float4* hi, uint32_t* bad) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    // segments: binary search over prim_start
+    uint32_t a = 0, b = nseg;
+    while (b - a > 1) {
+        uint32_t m = (a + b) >> 1;
+        if (segs[m].prim_start <= p) a = m; else b = m;
+    }
+    const TriSegment& s = segs[a];
+    uint32_t local = p - s.prim_start;
+    float3 v[3];
+    bool ok = true;
+    for (int k = 0; k < 3; k++) {
+        uint32_t idx = s.index_data[3 * size_t(local) + k];
+        if (idx > s.max_vertex) { ok = false; idx = 0; }
+        const float* f = (const float*)(s.vertex_data + size_t(s.first_vertex + idx) * s.stride);
+        v[k] = mk3(f[0], f[1], f[2]);
+    }
+    if (!ok) {
+        atomicOr(bad, 1u);
+        v[1] = v[0];
+        v[2] = v[0];
+    }
+    tri[3 * size_t(p) + 0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(local));
+    tri[3 * size_t(p) + 1] = make_float4(v[1].x, v[1].y, v[1].z, 0.f);
+    tri[3 * size_t(p) + 2] = make_float4(v[2].x, v[2].y, v[2].z, 0.f);
+    float3 l = fmin3(fmin3(v[0], v[1]), v[2]);
+    float3 h = fmax3(fmax3(v[0], v[1]), v[2]);
+    lo[p] = make_float4(l.x, l.y, l.z, 0.f);
+    hi[p] = make_float4(h.x, h.y, h.z, 0.f);
+}
+
+__global__ void k_reorder_tris(const float4* in, const uint32_t* perm, uint32_t n, float4* out) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t s = perm[p];
+    out[3 * size_t(p) + 0] = in[3 * size_t(s) + 0];
+    out[3 * size_t(p) + 1] = in[3 * size_t(s) + 1];
+    out[3 * size_t(p) + 2] = in[3 * size_t(s) + 2];
+}
+
+__global__ void k_prepare_instances(const InstanceRecord* inst, uint32_t n, float4* lo, float4* hi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const InstanceRecord& r = inst[i];
+    float3 l = mk3(INFINITY, INFINITY, INFINITY), h = mk3(-INFINITY, -INFINITY, -INFINITY);
+    if (!(r.flags & kInstEmpty)) {
+        const float* bb = r.blas->bounds;
+        for (int c = 0; c < 8; c++) {
+            float3 p = mk3((c & 1) ? bb[3] : bb[0], (c & 2) ? bb[4] : bb[1], (c & 4) ? bb[5] : bb[2]);
+            float3 w = mk3(r.o2w[0].x * p.x + r.o2w[0].y * p.y + r.o2w[0].z * p.z + r.o2w[0].w,
+                           r.o2w[1].x * p.x + r.o2w[1].y * p.y + r.o2w[1].z * p.z + r.o2w[1].w,
+                           r.o2w[2].x * p.x + r.o2w[2].y * p.y + r.o2w[2].z * p.z + r.o2w[2].w);
+            l = fmin3(l, w);
+            h = fmax3(h, w);
+        }
+        // rounding of the transform: pad by a few ulp of the magnitude
+        float3 m = fmax3(mk3(fabsf(l.x), fabsf(l.y), fabsf(l.z)), mk3(fabsf(h.x), fabsf(h.y), fabsf(h.z)));
+        float pad = fmaxf(fmaxf(m.x, m.y), m.z) * 1.0e-6f + 1e-30f;
+        l = l - mk3(pad, pad, pad);
+        h = h + mk3(pad, pad, pad);
+    } else {
+        // empty BLAS: a degenerate box at the origin that no traversal needs to enter
+        l = mk3(0.f, 0.f, 0.f);
+        h = l;
+    }
+    lo[i] = make_float4(l.x, l.y, l.z, 0.f);
+    hi[i] = make_float4(h.x, h.y, h.z, 0.f);
+}
+
+__global__ void k_reorder_instances(const InstanceRecord* in, const uint32_t* perm, uint32_t n, InstanceRecord* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = in[perm[i]];
+}
+
+// ------------------------------------------------------- segment bounds
+// bounds: per segment 12 uints: centroid lo/hi (ordered), box lo/hi (ordered)
+__global__ void k_init_bounds(uint32_t* bounds, uint32_t nseg) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg * 12) return;
+    uint32_t k = i % 12;
+    bool is_lo = (k < 3) || (k >= 6 && k < 9);
+    bounds[i] = is_lo ? f2ord(INFINITY) : f2ord(-INFINITY);
+}
+
+__global__ void k_seg_bounds(const float4* lo, const float4* hi, uint32_t n, const uint32_t* seg_start, uint32_t nseg,
+                             uint32_t* bounds) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = p < n;
+    uint32_t seg = valid ? find_segment(seg_start, nseg, p) : 0;
+    float v[12];
+    if (valid) {
+        float4 l = lo[p], h = hi[p];
+        v[0] = 0.5f * (l.x + h.x); v[1] = 0.5f * (l.y + h.y); v[2] = 0.5f * (l.z + h.z);
+        v[3] = v[0]; v[4] = v[1]; v[5] = v[2];
+        v[6] = l.x; v[7] = l.y; v[8] = l.z;
+        v[9] = h.x; v[10] = h.y; v[11] = h.z;
+    } else {
+        for (int k = 0; k < 12; k++) {
+            bool is_lo = (k < 3) || (k >= 6 && k < 9);
+            v[k] = is_lo ? INFINITY : -INFINITY;
+        }
+    }
+    // wave-uniform segment: reduce in registers, one atomic per wave
+    uint32_t s0 = __shfl(seg, 0, 64);
+    bool uniform = __ballot(seg != s0 && valid) == 0ull;
+    if (uniform) {
+        for (int k = 0; k < 12; k++) {
+            bool is_lo = (k < 3) || (k >= 6 && k < 9);
+            float x = v[k];
+            for (int o = 32; o > 0; o >>= 1) {
+                float y = __shfl_xor(x, o, 64);
+                x = is_lo ? fminf(x, y) : fmaxf(x, y);
+            }
+            v[k] = x;
+        }
+        if (__lane_id() == 0 && __ballot(valid) != 0ull) {
+            for (int k = 0; k < 12; k++) {
+                bool is_lo = (k < 3) || (k >= 6 && k < 9);
+                if (is_lo) atomicMin(&bounds[12 * s0 + k], f2ord(v[k]));
+                else atomicMax(&bounds[12 * s0 + k], f2ord(v[k]));
+            }
+        }
+    } else if (valid) {
+        for (int k = 0; k < 12; k++) {
+            bool is_lo = (k < 3) || (k >= 6 && k < 9);
+            if (is_lo) atomicMin(&bounds[12 * seg + k], f2ord(v[k]));
+            else atomicMax(&bounds[12 * seg + k], f2ord(v[k]));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- Morton
+__device__ inline uint64_t split3(uint32_t a) {
+    uint64_t x = a & 0x1fffffu;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_morton(const float4* lo, const float4* hi, uint32_t n, const uint32_t* seg_start, uint32_t nseg,
+                         const uint32_t* bounds, int bits, uint64_t* keys, uint32_t* vals) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t seg = find_segment(seg_start, nseg, p);
+    const uint32_t* b = bounds + 12 * seg;
+    float4 l = lo[p], h = hi[p];
+    float c[3] = {0.5f * (l.x + h.x), 0.5f * (l.y + h.y), 0.5f * (l.z + h.z)};
+    float scale = float((1u << bits) - 1u);
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float cl = ord2f(b[k]), ch = ord2f(b[3 + k]);
+        float ext = ch - cl;
+        float t = ext > 0.f ? (c[k] - cl) / ext : 0.5f;
+        t = fminf(fmaxf(t, 0.f), 1.f);
+        q[k] = uint32_t(t * scale);
+    }
+    uint64_t code = (split3(q[0]) << 2) | (split3(q[1]) << 1) | split3(q[2]);
+    keys[p] = (uint64_t(seg) << (3 * bits)) | code;
+    vals[p] = p;
+}
+
+// ------------------------------------------------------------ radix sort
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint64_t* keys, uint32_t n, int shift, uint32_t* hist,
+                                                     uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kRsTile;
+    for (int i = threadIdx.x; i < kRsTile; i += kBlock) {
+        uint32_t idx = base + i;
+        if (idx < n) atomicAdd(&h[uint32_t(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_local(uint32_t* data, uint32_t m, uint32_t* sums) {
+    __shared__ uint32_t s[kBlock];
+    uint32_t base = blockIdx.x * kScanPerBlock + threadIdx.x * 4;
+    uint32_t v[4], t = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = (base + k < m) ? data[base + k] : 0u;
+        uint32_t x = v[k];
+        v[k] = t;
+        t += x;
+    }
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        uint32_t y = threadIdx.x >= uint32_t(o) ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += y;
+        __syncthreads();
+    }
+    uint32_t excl = s[threadIdx.x] - t;
+    for (int k = 0; k < 4; k++)
+        if (base + k < m) data[base + k] = v[k] + excl;
+    if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = s[kBlock - 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* data, uint32_t m, const uint32_t* sums) {
+    uint32_t base = blockIdx.x * kScanPerBlock + threadIdx.x * 4;
+    uint32_t add = sums[blockIdx.x];
+    for (int k = 0; k < 4; k++)
+        if (base + k < m) data[base + k] += add;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                                                        uint32_t* vout, uint32_t n, int shift, const uint32_t* offs,
+                                                        uint32_t nblocks) {
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_cnt[kBlock / 64][256];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    s_base[tid] = offs[tid * nblocks + blockIdx.x];
+    const uint32_t base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; r++) {
+        for (int k = tid; k < (kBlock / 64) * 256; k += kBlock) (&s_cnt[0][0])[k] = 0u;
+        __syncthreads();
+        uint32_t idx = base + r * kBlock + tid;
+        bool valid = idx < n;
+        uint64_t key = valid ? kin[idx] : 0ull;
+        uint32_t val = valid ? vin[idx] : 0u;
+        uint32_t digit = uint32_t(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; b++) {
+            bool bit = (digit >> b) & 1u;
+            uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t rank = __popcll(peers & lanemask_lt());
+        if (valid && rank == 0) s_cnt[wave][digit] = __popcll(peers);
+        __syncthreads();
+        uint32_t total;
+        {
+            uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
+            s_cnt[0][tid] = 0;
+            s_cnt[1][tid] = c0;
+            s_cnt[2][tid] = c0 + c1;
+            s_cnt[3][tid] = c0 + c1 + c2;
+            total = c0 + c1 + c2 + c3;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_base[digit] + s_cnt[wave][digit] + rank;
+            if (pos < n) {  // always true for a consistent histogram
+                kout[pos] = key;
+                vout[pos] = val;
+            }
+        }
+        __syncthreads();
+        s_base[tid] += total;
+        __syncthreads();
+    }
+}
+
+int exclusive_scan(hipStream_t st, uint32_t* d, uint32_t m, uint32_t* tmp, uint32_t tmp_cap) {
+    uint32_t nb = blocks_for(m, kScanPerBlock);
+    if (nb > tmp_cap) return int(hipErrorInvalidValue);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kBlock), 0, st, d, m, tmp);
+    RT_CHECK_LAUNCH();
+    if (nb > 1) {
+        int rc = exclusive_scan(st, tmp, nb, tmp + nb, tmp_cap - nb);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kBlock), 0, st, d, m, tmp);
+        RT_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- Karras
+__device__ inline int delta(const uint64_t* k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz(uint32_t(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ void k_karras(const uint64_t* keys, int n, uint32_t* left, uint32_t* right, uint32_t* first,
+                         uint32_t* last, uint32_t* parent_int, uint32_t* parent_leaf, const uint32_t* seg_start,
+                         uint32_t nseg, int seg_shift, uint32_t* seg_root) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t lc = (lo == gamma) ? (kLeafBit | uint32_t(gamma)) : uint32_t(gamma);
+    uint32_t rc = (hi == gamma + 1) ? (kLeafBit | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    first[i] = lo;
+    last[i] = hi;
+    if (lc & kLeafBit) parent_leaf[gamma] = i; else parent_int[gamma] = i;
+    if (rc & kLeafBit) parent_leaf[gamma + 1] = i; else parent_int[gamma + 1] = i;
+    uint32_t seg = uint32_t(keys[lo] >> seg_shift);
+    if (seg < nseg && uint32_t(lo) == seg_start[seg] && uint32_t(hi) + 1 == seg_start[seg + 1]) seg_root[seg] = i;
+}
+
+__global__ void k_single_roots(const uint32_t* seg_start, uint32_t nseg, uint32_t* seg_root) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    uint32_t c = seg_start[s + 1] - seg_start[s];
+    if (c == 1) seg_root[s] = kLeafBit | seg_start[s];
+    else if (c == 0) seg_root[s] = kEmpty;
+}
+
+__global__ void k_mark_roots(const uint32_t* seg_root, uint32_t nseg, uint32_t* is_root) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    uint32_t r = seg_root[s];
+    if (r != kEmpty && !(r & kLeafBit)) is_root[r] = 1u;
+}
+
+// --------------------------------------------------------------- box fit
+__device__ inline void load_box(uint32_t ref, const float4* lo, const float4* hi, const uint32_t* perm,
+                                const float4* nlo, const float4* nhi, float3& l, float3& h) {
+    if (ref & kLeafBit) {
+        uint32_t s = perm[ref & ~kLeafBit];
+        l = xyz(lo[s]);
+        h = xyz(hi[s]);
+    } else {
+        l = xyz(nlo[ref]);
+        h = xyz(nhi[ref]);
+    }
+}
+
+__global__ void k_fit(const float4* lo, const float4* hi, const uint32_t* perm, int n, const uint32_t* left,
+                      const uint32_t* right, const uint32_t* parent_int, const uint32_t* parent_leaf,
+                      const uint32_t* is_root, uint32_t* visits, float4* nlo, float4* nhi) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t node = parent_leaf[p];
+    while (node != kEmpty) {
+        // the second child to arrive computes the union; acq_rel makes the first
+        // child's box visible
+        uint32_t prev = __hip_atomic_fetch_add(&visits[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == 0u) return;
+        float3 l0, h0, l1, h1;
+        load_box(left[node], lo, hi, perm, nlo, nhi, l0, h0);
+        load_box(right[node], lo, hi, perm, nlo, nhi, l1, h1);
+        float3 l = fmin3(l0, l1), h = fmax3(h0, h1);
+        nlo[node] = make_float4(l.x, l.y, l.z, 0.f);
+        nhi[node] = make_float4(h.x, h.y, h.z, 0.f);
+        if (is_root[node]) return;
+        node = parent_int[node];
+    }
+}
+
+// -------------------------------------------------------------- collapse
+struct CollapseArgs {
+    const float4* lo;
+    const float4* hi;
+    const uint32_t* perm;
+    const float4* nlo;
+    const float4* nhi;
+    const uint32_t* left;
+    const uint32_t* right;
+    const uint32_t* first;
+    const uint32_t* last;
+    const uint32_t* seg_start;
+    uint32_t nseg;
+    const float* seg_pad;  // absolute padding per segment
+    uint32_t leaf_max;
+    WideNode* nodes;
+    uint32_t* node_counter;
+    uint32_t max_nodes;     // capacity of nodes and of each task array
+    uint32_t* overflow;     // set when a capacity would be exceeded
+};
+
+__device__ inline uint32_t ref_count(const CollapseArgs& a, uint32_t r) {
+    return (r & kLeafBit) ? 1u : (a.last[r] - a.first[r] + 1u);
+}
+__device__ inline uint32_t ref_first(const CollapseArgs& a, uint32_t r) {
+    return (r & kLeafBit) ? (r & ~kLeafBit) : a.first[r];
+}
+__device__ inline float half_area(float3 l, float3 h) {
+    float3 e = h - l;
+    return e.x * e.y + e.y * e.z + e.z * e.x;
+}
+
+// task = (binary ref, wide node index, segment)
+__global__ void k_collapse(CollapseArgs a, const uint4* tasks_in, const uint32_t* n_in, uint4* tasks_out,
+                           uint32_t* n_out) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *n_in) return;
+    uint4 task = tasks_in[t];
+    uint32_t b = task.x, w = task.y, seg = task.z;
+    uint32_t c[4];
+    int nc;
+    if (b & kLeafBit) { c[0] = b; nc = 1; }
+    else { c[0] = a.left[b]; c[1] = a.right[b]; nc = 2; }
+    float3 cl[4], ch[4];
+    for (int k = 0; k < nc; k++) load_box(c[k], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl[k], ch[k]);
+    while (nc < 4) {
+        int best = -1;
+        float best_a = -1.f;
+        for (int k = 0; k < nc; k++) {
+            if ((c[k] & kLeafBit) || ref_count(a, c[k]) <= a.leaf_max) continue;
+            float ar = half_area(cl[k], ch[k]);
+            if (ar > best_a) { best_a = ar; best = k; }
+        }
+        if (best < 0) break;
+        uint32_t node = c[best];
+        c[best] = a.left[node];
+        c[nc] = a.right[node];
+        load_box(c[best], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl[best], ch[best]);
+        load_box(c[nc], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl[nc], ch[nc]);
+        nc++;
+    }
+    float pad = a.seg_pad[seg];
+    WideNode out;
+    for (int k = 0; k < 4; k++) {
+        if (k >= nc) {
+            out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = 0.f;
+            out.hi_x[k] = out.hi_y[k] = out.hi_z[k] = 0.f;
+            out.child[k] = kEmpty;
+            out.count[k] = 0;
+            continue;
+        }
+        out.lo_x[k] = cl[k].x - pad; out.lo_y[k] = cl[k].y - pad; out.lo_z[k] = cl[k].z - pad;
+        out.hi_x[k] = ch[k].x + pad; out.hi_y[k] = ch[k].y + pad; out.hi_z[k] = ch[k].z + pad;
+        uint32_t cnt = ref_count(a, c[k]);
+        if ((c[k] & kLeafBit) || cnt <= a.leaf_max) {
+            out.child[k] = ref_first(a, c[k]);
+            out.count[k] = cnt;
+        } else {
+            uint32_t idx = atomicAdd(a.node_counter, 1u);
+            uint32_t slot = atomicAdd(n_out, 1u);
+            if (idx >= a.max_nodes || slot >= a.max_nodes) {
+                atomicOr(a.overflow, 1u);
+                out.child[k] = kEmpty;
+                out.count[k] = 0;
+                continue;
+            }
+            tasks_out[slot] = make_uint4(c[k], idx, seg, 0u);
+            out.child[k] = idx;
+            out.count[k] = 0;
+        }
+    }
+    a.nodes[w] = out;
+}
+
+// root wide node of an empty segment: all children empty
+__global__ void k_empty_roots(WideNode* nodes, const uint32_t* seg_root, uint32_t nseg) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg || seg_root[s] != kEmpty) return;
+    WideNode out;
+    for (int k = 0; k < 4; k++) {
+        out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = 0.f;
+        out.hi_x[k] = out.hi_y[k] = out.hi_z[k] = 0.f;
+        out.child[k] = kEmpty;
+        out.count[k] = 0;
+    }
+    nodes[s] = out;
+}
+
+__global__ void k_init_tasks(const uint32_t* seg_root, uint32_t nseg, uint4* tasks, uint32_t* n_tasks) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg || seg_root[s] == kEmpty) return;
+    uint32_t slot = atomicAdd(n_tasks, 1u);
+    tasks[slot] = make_uint4(seg_root[s], s, s, 0u);
+}
+
+// padding and root bounds per segment
+__global__ void k_seg_pad(const uint32_t* bounds, uint32_t nseg, float* pad, float* seg_bounds) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const uint32_t* b = bounds + 12 * s;
+    float m = 0.f;
+    float v[6];
+    for (int k = 0; k < 6; k++) {
+        v[k] = ord2f(b[6 + k]);
+        if (isfinite(v[k])) m = fmaxf(m, fabsf(v[k]));
+    }
+    // about 8 ulp of the largest coordinate magnitude of the segment
+    float p = m * 9.5367431640625e-07f + 1e-30f;  // 2^-20
+    pad[s] = p;
+    bool empty = !(v[0] <= v[3]);
+    for (int k = 0; k < 3; k++) {
+        seg_bounds[6 * s + k] = empty ? 0.f : v[k] - p;
+        seg_bounds[6 * s + 3 + k] = empty ? 0.f : v[3 + k] + p;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+int prepare_triangles(hipStream_t st, const TriSegment* d_segs, uint32_t nseg, uint32_t n, float4* d_tri, float4* d_lo,
+                      float4* d_hi, uint32_t* d_bad) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_prepare_triangles, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_segs, nseg, n, d_tri,
+                       d_lo, d_hi, d_bad);
+    RT_CHECK_LAUNCH();
+    return 0;
+}
+
+int reorder_triangles(hipStream_t st, const float4* in, const uint32_t* perm, uint32_t n, float4* out) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_reorder_tris, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, in, perm, n, out);
+    RT_CHECK_LAUNCH();
+    return 0;
+}
+
+int prepare_instances(hipStream_t st, const InstanceRecord* inst, uint32_t n, float4* lo, float4* hi) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_prepare_instances, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, inst, n, lo, hi);
+    RT_CHECK_LAUNCH();
+    return 0;
+}
+
+int reorder_instances(hipStream_t st, const InstanceRecord* in, const uint32_t* perm, uint32_t n, InstanceRecord* out) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_reorder_instances, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, in, perm, n, out);
+    RT_CHECK_LAUNCH();
+    return 0;
+}
+
+namespace {
+// RAII bundle of temporary device buffers.
+struct Scratch {
+    std::vector<void*> ptrs;
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T> T* alloc(size_t count) {
+        void* p = nullptr;
+        if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+};
+}  // namespace
+
+int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t n, const uint32_t* d_seg_start,
+               const std::vector<uint32_t>& h_seg_start, uint32_t leaf_max, LbvhResult* out, std::string* err) {
+    const uint32_t nseg = uint32_t(h_seg_start.size()) - 1;
+    if (leaf_max < 1 || leaf_max > uint32_t(kMaxLeafPrims)) { *err = "leaf size out of range"; return int(hipErrorInvalidValue); }
+    if (n >= (1u << kLeafCountShift)) { *err = "too many primitives in one build"; return int(hipErrorInvalidValue); }
+    int segbits = 0;
+    while ((1u << segbits) < nseg) segbits++;
+    int bits = (64 - segbits) / 3;
+    if (bits > 21) bits = 21;
+    const int key_bits = 3 * bits + segbits;
+    Scratch s;
+    uint32_t* bounds = s.alloc<uint32_t>(12 * size_t(nseg));
+    float* pad = s.alloc<float>(nseg);
+    uint32_t* seg_root = s.alloc<uint32_t>(nseg);
+    const uint32_t max_nodes = n + nseg;
+    WideNode* nodes_tmp = s.alloc<WideNode>(max_nodes);
+    if (!bounds || !pad || !seg_root || !nodes_tmp) { *err = "out of device memory"; return int(hipErrorOutOfMemory); }
+    float* seg_bounds = nullptr;
+    uint32_t* perm_out = nullptr;
+    if (hipMalloc(&seg_bounds, 6 * sizeof(float) * nseg) != hipSuccess) { *err = "out of device memory"; return int(hipErrorOutOfMemory); }
+    if (hipMalloc(&perm_out, sizeof(uint32_t) * (n + 1)) != hipSuccess) {
+        (void)hipFree(seg_bounds);
+        *err = "out of device memory";
+        return int(hipErrorOutOfMemory);
+    }
+    auto fail = [&](int rc, const char* what) {
+        (void)hipFree(seg_bounds);
+        (void)hipFree(perm_out);
+        *err = what;
+        return rc;
+    };
+    int rc;
+    hipLaunchKernelGGL(k_init_bounds, dim3(blocks_for(12 * size_t(nseg), kBlock)), dim3(kBlock), 0, st, bounds, nseg);
+    if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_init_bounds");
+    if (n > 0) {
+        hipLaunchKernelGGL(k_seg_bounds, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_lo, d_hi, n, d_seg_start,
+                           nseg, bounds);
+        if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_seg_bounds");
+    }
+    hipLaunchKernelGGL(k_seg_pad, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, bounds, nseg, pad, seg_bounds);
+    if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_seg_pad");
+    if (hipMemsetAsync(seg_root, 0xFF, sizeof(uint32_t) * nseg, st) != hipSuccess) return fail(int(hipErrorUnknown), "memset");
+
+    uint32_t* nc = s.alloc<uint32_t>(4);  // node counter, task counters
+    if (!nc) return fail(int(hipErrorOutOfMemory), "out of device memory");
+    if (n > 0) {
+        uint64_t* k0 = s.alloc<uint64_t>(n);
+        uint64_t* k1 = s.alloc<uint64_t>(n);
+        uint32_t* v0 = s.alloc<uint32_t>(n);
+        uint32_t* v1 = s.alloc<uint32_t>(n);
+        const uint32_t nblk = blocks_for(n, kRsTile);
+        const uint32_t m = 256u * nblk;
+        uint32_t* hist = s.alloc<uint32_t>(m);
+        const uint32_t scan_cap = 2 * blocks_for(m, kScanPerBlock) + 64;
+        uint32_t* scan_tmp = s.alloc<uint32_t>(scan_cap);
+        if (!k0 || !k1 || !v0 || !v1 || !hist || !scan_tmp) return fail(int(hipErrorOutOfMemory), "out of device memory");
+        hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_lo, d_hi, n, d_seg_start, nseg,
+                           bounds, bits, k0, v0);
+        if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_morton");
+        for (int shift = 0; shift < key_bits; shift += 8) {
+            hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(kBlock), 0, st, k0, n, shift, hist, nblk);
+            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_rs_hist");
+            rc = exclusive_scan(st, hist, m, scan_tmp, scan_cap);
+            if (rc) return fail(rc, "scan");
+            hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(kBlock), 0, st, k0, v0, k1, v1, n, shift, hist, nblk);
+            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_rs_scatter");
+            std::swap(k0, k1);
+            std::swap(v0, v1);
+        }
+        // v0 = perm (sorted position -> primitive), k0 = sorted keys
+        if (hipMemcpyAsync(perm_out, v0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return fail(int(hipErrorUnknown), "copy perm");
+        const int ni = int(n) - 1;
+        uint32_t* left = s.alloc<uint32_t>(n);
+        uint32_t* right = s.alloc<uint32_t>(n);
+        uint32_t* first = s.alloc<uint32_t>(n);
+        uint32_t* last = s.alloc<uint32_t>(n);
+        uint32_t* parent_int = s.alloc<uint32_t>(n);
+        uint32_t* parent_leaf = s.alloc<uint32_t>(n);
+        uint32_t* is_root = s.alloc<uint32_t>(n);
+        uint32_t* visits = s.alloc<uint32_t>(n);
+        float4* nlo = s.alloc<float4>(n);
+        float4* nhi = s.alloc<float4>(n);
+        if (!left || !right || !first || !last || !parent_int || !parent_leaf || !is_root || !visits || !nlo || !nhi)
+            return fail(int(hipErrorOutOfMemory), "out of device memory");
+        if (hipMemsetAsync(parent_int, 0xFF, sizeof(uint32_t) * n, st) != hipSuccess ||
+            hipMemsetAsync(parent_leaf, 0xFF, sizeof(uint32_t) * n, st) != hipSuccess ||
+            hipMemsetAsync(is_root, 0, sizeof(uint32_t) * n, st) != hipSuccess ||
+            hipMemsetAsync(visits, 0, sizeof(uint32_t) * n, st) != hipSuccess)
+            return fail(int(hipErrorUnknown), "memset");
+        if (ni > 0) {
+            hipLaunchKernelGGL(k_karras, dim3(blocks_for(ni, kBlock)), dim3(kBlock), 0, st, k0, int(n), left, right,
+                               first, last, parent_int, parent_leaf, d_seg_start, nseg, 3 * bits, seg_root);
+            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_karras");
+        }
+        hipLaunchKernelGGL(k_single_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, d_seg_start, nseg,
+                           seg_root);
+        hipLaunchKernelGGL(k_mark_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, seg_root, nseg, is_root);
+        hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_lo, d_hi, v0, int(n), left, right,
+                           parent_int, parent_leaf, is_root, visits, nlo, nhi);
+        if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_fit");
+
+        CollapseArgs a{d_lo, d_hi, v0, nlo, nhi, left, right, first, last, d_seg_start, nseg, pad, leaf_max,
+                       nodes_tmp, nc, max_nodes, nc + 3};
+        uint4* ta = s.alloc<uint4>(max_nodes);
+        uint4* tb = s.alloc<uint4>(max_nodes);
+        if (!ta || !tb) return fail(int(hipErrorOutOfMemory), "out of device memory");
+        uint32_t h_init[4] = {nseg, 0, 0, 0};
+        if (hipMemcpyAsync(nc, h_init, sizeof(h_init), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(int(hipErrorUnknown), "copy counters");
+        hipLaunchKernelGGL(k_empty_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, nodes_tmp, seg_root, nseg);
+        hipLaunchKernelGGL(k_init_tasks, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, seg_root, nseg, ta, nc + 1);
+        if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_init_tasks");
+        uint32_t count = 0;
+        if (hipMemcpyAsync(&count, nc + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return fail(int(hipErrorUnknown), "read task count");
+        int in_slot = 1;
+        for (int level = 0; count > 0; level++) {
+            if (level > 200) return fail(int(hipErrorUnknown), "collapse did not terminate");
+            int out_slot = in_slot == 1 ? 2 : 1;
+            if (hipMemsetAsync(nc + out_slot, 0, sizeof(uint32_t), st) != hipSuccess) return fail(int(hipErrorUnknown), "memset");
+            hipLaunchKernelGGL(k_collapse, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, st, a, ta, nc + in_slot, tb,
+                               nc + out_slot);
+            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_collapse");
+            if (hipMemcpyAsync(&count, nc + out_slot, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return fail(int(hipErrorUnknown), "read task count");
+            std::swap(ta, tb);
+            in_slot = out_slot;
+        }
+    } else {
+        uint32_t h_init[4] = {nseg, 0, 0, 0};
+        if (hipMemcpyAsync(nc, h_init, sizeof(h_init), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(int(hipErrorUnknown), "copy counters");
+        hipLaunchKernelGGL(k_empty_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, nodes_tmp, seg_root, nseg);
+    }
+    uint32_t hc[4] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(hc, nc, sizeof(hc), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return fail(int(hipErrorUnknown), "read node count");
+    if (hc[3] != 0u || hc[0] > max_nodes) return fail(int(hipErrorUnknown), "node capacity exceeded during collapse");
+    uint32_t num_nodes = hc[0];
+    WideNode* nodes = nullptr;
+    if (hipMalloc(&nodes, sizeof(WideNode) * size_t(num_nodes)) != hipSuccess) return fail(int(hipErrorOutOfMemory), "out of device memory");
+    if (hipMemcpyAsync(nodes, nodes_tmp, sizeof(WideNode) * size_t(num_nodes), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipFree(nodes);
+        return fail(int(hipErrorUnknown), "copy nodes");
+    }
+    out->nodes = nodes;
+    out->num_nodes = num_nodes;
+    out->perm = perm_out;
+    out->seg_bounds = seg_bounds;
+    return 0;
+}
+
+}  // namespace rt

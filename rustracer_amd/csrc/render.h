@@ -1,0 +1,34 @@
+// render.h — launch interface of the trace kernels (render.hip, wavefront.hip).
+#pragma once
+
+#include "shade.h"
+#include "traverse.h"
+
+namespace rt {
+
+// Per-launch arguments: the UBO fields the pipeline reads, the scene bindings and
+// the acceleration structure.  Passed by value (kernel arguments).
+struct FrameArgs {
+    TlasView tlas;
+    SceneView sv;
+    float* accum;
+    uint32_t width, height;
+    uint32_t seed_b;       // stream-B seed (replaces clockARB(), RayTracing.rgen:33,40)
+    uint32_t reset;        // number_of_samples == total_number_of_samples
+    unsigned long long* counters;  // kNumCounters, used by the counting variants
+    float mvi[16], pin[16];        // ModelViewInverse, ProjectionInverse (column-major)
+    float aperture, focus_distance, ortho, exposure;
+    uint32_t samples, bounces, antialiasing, mapping, frame_count, fully_opaque;
+};
+
+enum Counter : int {
+    kCntPrimary = 0, kCntExtension, kCntShadow, kCntNodes, kCntTris, kCntInstances, kCntShade, kCntAnyHit,
+    kCntTex, kCntLights, kCntOverflow, kCntBadIndex, kNumCounters = 16
+};
+
+int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats);
+int launch_ray_queries(const TlasView& tl, const SceneView& sv, const rt_ray* rays, uint32_t n, uint32_t flags,
+                       rt_hit* hits, uint32_t seed_b, hipStream_t st, bool any_hit);
+int launch_primary(const FrameArgs& fa, rt_hit* hits, hipStream_t st, bool any_hit);
+
+}  // namespace rt

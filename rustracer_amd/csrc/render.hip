@@ -1,0 +1,146 @@
+// render.hip — the plain megakernel (one lane per pixel; raygen loop with the
+// closest-hit, any-hit and miss shaders inlined) plus the ray-query and
+// primary-hit kernels used for traversal parity.
+//
+// Mapping: 128-lane workgroups, each 64-lane wave owns an 8x8 pixel tile; the
+// workgroup order is remapped so each of the 8 XCDs (which receive workgroups
+// round-robin) renders one horizontal band of the image and its L2 sees the part
+// of the scene that band looks at.
+#include "path.h"
+
+namespace rt {
+
+namespace {
+
+constexpr int kMegaBlock = 128;
+
+__device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+    const uint32_t nx = 8;
+    uint32_t full = (nblocks / nx) * nx;
+    if (bid >= full) return bid;
+    uint32_t per = full / nx;
+    return (bid % nx) * per + bid / nx;
+}
+
+template <bool kAnyHit, bool kStats>
+__global__ __launch_bounds__(kMegaBlock) void k_megakernel(FrameArgs fa) {
+    __shared__ uint32_t s_stack[kStackDepth * kMegaBlock];
+    const uint32_t tiles_x = (fa.width + 15) / 16;
+    const uint32_t tiles_y = (fa.height + 7) / 8;
+    const uint32_t bid = xcd_remap(blockIdx.x, tiles_x * tiles_y);
+    const uint32_t tx = bid % tiles_x, ty = bid / tiles_x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t x = tx * 16 + wave * 8 + (lane & 7);
+    const uint32_t y = ty * 8 + (lane >> 3);
+    PathCounters pc;
+    if (x < fa.width && y < fa.height) {
+        float3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, kMegaBlock, pc);
+        float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
+        float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
+        *acc = make_float4(prev.x + radiance.x, prev.y + radiance.y, prev.z + radiance.z, 0.f);
+    }
+    if (kStats) flush_counters(fa.counters, pc);
+}
+
+template <bool kAnyHit>
+__global__ __launch_bounds__(kMegaBlock) void k_ray_queries(TlasView tl, SceneView sv, const rt_ray* rays, uint32_t n,
+                                                             uint32_t flags, rt_hit* hits, uint32_t seed_b) {
+    __shared__ uint32_t s_stack[kStackDepth * kMegaBlock];
+    uint32_t i = blockIdx.x * kMegaBlock + threadIdx.x;
+    if (i >= n) return;
+    rt_ray r = rays[i];
+    Rng4 rng{i, 0u, seed_b, 0u};
+    AlphaAnyHit ah{&sv, &tl, &rng};
+    HitInfo h;
+    TraceStats st;
+    bool found = trace_ray<kAnyHit, false>(tl, mk3(r.origin[0], r.origin[1], r.origin[2]),
+                                          mk3(r.direction[0], r.direction[1], r.direction[2]), r.tmin, r.tmax, flags,
+                                          ah, s_stack + threadIdx.x, kMegaBlock, h, st);
+    rt_hit out;
+    out._pad[0] = out._pad[1] = 0;
+    if (found) {
+        const InstanceRecord& ir = tl.inst[h.inst];
+        out.t = h.t;
+        out.u = h.u;
+        out.v = h.v;
+        out.instance_id = ir.instance_id;
+        out.primitive_id = h.prim;
+        out.custom_index = ir.custom_index;
+    } else {
+        out.t = -1.f;
+        out.u = out.v = 0.f;
+        out.instance_id = out.primitive_id = out.custom_index = kEmpty;
+    }
+    hits[i] = out;
+}
+
+template <bool kAnyHit>
+__global__ __launch_bounds__(kMegaBlock) void k_primary(FrameArgs fa, rt_hit* hits) {
+    __shared__ uint32_t s_stack[kStackDepth * kMegaBlock];
+    uint32_t i = blockIdx.x * kMegaBlock + threadIdx.x;
+    if (i >= fa.width * fa.height) return;
+    uint32_t x = i % fa.width, y = i / fa.width;
+    Rng4 rng_a{x, y, fa.frame_count, 0u};
+    Rng4 rng_b{x, y, fa.seed_b, 0u};
+    uint32_t lcg = tea16(tea16(x, y), fa.seed_b);
+    float3 o, d;
+    float tfar;
+    camera_ray(fa, x, y, rng_a, lcg, o, d, tfar);
+    AlphaAnyHit ah{&fa.sv, &fa.tlas, &rng_b};
+    HitInfo h;
+    TraceStats st;
+    uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
+    bool found = trace_ray<kAnyHit, false>(fa.tlas, o, d, kTMin, tfar, flags, ah, s_stack + threadIdx.x, kMegaBlock,
+                                          h, st);
+    rt_hit out;
+    out._pad[0] = out._pad[1] = 0;
+    if (found) {
+        const InstanceRecord& ir = fa.tlas.inst[h.inst];
+        out.t = h.t;
+        out.u = h.u;
+        out.v = h.v;
+        out.instance_id = ir.instance_id;
+        out.primitive_id = h.prim;
+        out.custom_index = ir.custom_index;
+    } else {
+        out.t = -1.f;
+        out.u = out.v = 0.f;
+        out.instance_id = out.primitive_id = out.custom_index = kEmpty;
+    }
+    hits[i] = out;
+}
+
+}  // namespace
+
+int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats) {
+    uint32_t tiles = ((fa.width + 15) / 16) * ((fa.height + 7) / 8);
+    if (tiles == 0) return 0;
+    if (any_hit) {
+        if (stats) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
+        else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
+    } else {
+        if (stats) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
+        else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
+    }
+    return int(hipGetLastError());
+}
+
+int launch_ray_queries(const TlasView& tl, const SceneView& sv, const rt_ray* rays, uint32_t n, uint32_t flags,
+                       rt_hit* hits, uint32_t seed_b, hipStream_t st, bool any_hit) {
+    if (n == 0) return 0;
+    uint32_t nb = (n + kMegaBlock - 1) / kMegaBlock;
+    if (any_hit) hipLaunchKernelGGL((k_ray_queries<true>), dim3(nb), dim3(kMegaBlock), 0, st, tl, sv, rays, n, flags, hits, seed_b);
+    else hipLaunchKernelGGL((k_ray_queries<false>), dim3(nb), dim3(kMegaBlock), 0, st, tl, sv, rays, n, flags, hits, seed_b);
+    return int(hipGetLastError());
+}
+
+int launch_primary(const FrameArgs& fa, rt_hit* hits, hipStream_t st, bool any_hit) {
+    uint32_t n = fa.width * fa.height;
+    if (n == 0) return 0;
+    uint32_t nb = (n + kMegaBlock - 1) / kMegaBlock;
+    if (any_hit) hipLaunchKernelGGL((k_primary<true>), dim3(nb), dim3(kMegaBlock), 0, st, fa, hits);
+    else hipLaunchKernelGGL((k_primary<false>), dim3(nb), dim3(kMegaBlock), 0, st, fa, hits);
+    return int(hipGetLastError());
+}
+
+}  // namespace rt

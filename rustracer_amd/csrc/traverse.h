@@ -1,0 +1,271 @@
+// traverse.h — device-side traceRayEXT: two-level 4-wide BVH traversal and the
+// watertight ray/triangle test.
+//
+// Semantics follow what the reference pipeline asks of the driver
+// (crates/examples/gltf_viewer/shaders/RayTracing.rgen:89-92,
+//  RayTracing.rchit:51-55): closest accepted hit in [tmin, tmax], any-hit on
+// non-opaque geometry unless the ray or instance forces opacity, optional
+// termination on the first accepted hit, no face culling.  Ties in t are broken
+// by the smaller (instance, primitive) so results do not depend on BVH shape.
+//
+// Stack: a per-lane ring of kStackDepth entries in LDS (one column per lane, so a
+// wave's pushes hit 64 distinct banks).  When the ring overwrites an entry the
+// traversal restarts from the TLAS root with the current closest t once the ring
+// drains: closest-hit and shadow results stay exact; only the number of any-hit
+// invocations can change.  After kMaxRestarts restarts the ray gives up and raises
+// kErrStackOverflow, so every traversal ends.
+#pragma once
+
+#include "common.h"
+
+namespace rt {
+
+constexpr int kStackDepth = 32;  // power of two
+constexpr uint32_t kMaxRestarts = 64;
+
+struct TraceStats {
+    uint32_t nodes = 0, tris = 0, instances = 0, anyhits = 0, overflows = 0;
+};
+
+struct HitInfo {
+    float t, u, v;
+    uint32_t inst;  // index into TlasView::inst
+    uint32_t prim;  // gl_PrimitiveID
+};
+
+// Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2013).
+struct WoopRay {
+    float3 o;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+__device__ inline WoopRay woop_setup(float3 o, float3 d) {
+    WoopRay w;
+    w.o = o;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    w.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    w.kx = w.kz == 2 ? 0 : w.kz + 1;
+    w.ky = w.kx == 2 ? 0 : w.kx + 1;
+    float dz = comp(d, w.kz);
+    if (dz < 0.f) { int t = w.kx; w.kx = w.ky; w.ky = t; }
+    w.sx = comp(d, w.kx) / dz;
+    w.sy = comp(d, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+
+// Returns true and (t, u, v) for a hit with tmin <= t <= tmax.
+__device__ inline bool woop_hit(const WoopRay& w, float3 v0, float3 v1, float3 v2, float tmin, float tmax, float& t,
+                                float& bu, float& bv) {
+    float3 A = v0 - w.o, B = v1 - w.o, C = v2 - w.o;
+    float Akz = comp(A, w.kz), Bkz = comp(B, w.kz), Ckz = comp(C, w.kz);
+    float Ax = comp(A, w.kx) - w.sx * Akz, Ay = comp(A, w.ky) - w.sy * Akz;
+    float Bx = comp(B, w.kx) - w.sx * Bkz, By = comp(B, w.ky) - w.sy * Bkz;
+    float Cx = comp(C, w.kx) - w.sx * Ckz, Cy = comp(C, w.ky) - w.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    float det = U + V + W;
+    if (det == 0.f) return false;
+    float T = U * (w.sz * Akz) + V * (w.sz * Bkz) + W * (w.sz * Ckz);
+    float tt = T / det;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt;
+    bu = V / det;
+    bv = W / det;
+    return true;
+}
+
+struct TravRay {
+    float3 o, d, inv;
+};
+
+__device__ inline float safe_inv(float x) {
+    return 1.0f / x;  // +-inf for +-0, the slab test handles it
+}
+
+// Slab test of the four children of a node; returns a 4-bit hit mask and the
+// entry distances.  The exit distance is widened by a few ulp (Ize 2013) so a box
+// the exact ray touches is never rejected.
+__device__ inline uint32_t node_hits(const WideNode& n, const TravRay& r, float tmin, float tmax, float tn[4]) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float tx0 = (n.lo_x[k] - r.o.x) * r.inv.x, tx1 = (n.hi_x[k] - r.o.x) * r.inv.x;
+        float ty0 = (n.lo_y[k] - r.o.y) * r.inv.y, ty1 = (n.hi_y[k] - r.o.y) * r.inv.y;
+        float tz0 = (n.lo_z[k] - r.o.z) * r.inv.z, tz1 = (n.hi_z[k] - r.o.z) * r.inv.z;
+        float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax * 1.0000002f));
+        t1 = t1 * 1.0000004f;
+        tn[k] = t0;
+        if (t0 <= t1 && n.child[k] != kEmpty) mask |= 1u << k;
+    }
+    return mask;
+}
+
+__device__ inline WideNode load_node(const WideNode* nodes, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    WideNode n;
+    float4* q = reinterpret_cast<float4*>(&n);
+#pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = p[k];
+    return n;
+}
+
+struct NoAnyHit {
+    __device__ bool ignore(uint32_t, uint32_t, float, float) { return false; }
+};
+
+// Closest-hit traversal.  `ah.ignore(inst_index, prim, u, v)` is the any-hit
+// shader (true = ignoreIntersectionEXT).  Returns true on a hit.
+template <bool kAnyHit, bool kStats, class AnyHit>
+__device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, float tmax, uint32_t ray_flags,
+                          AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit, TraceStats& st) {
+    const bool terminate = (ray_flags & RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT) != 0;
+    float best_t = tmax;
+    bool found = false;
+    hit.inst = kEmpty;
+    hit.prim = kEmpty;
+    TravRay world{o, d, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))};
+    TravRay cur = world;
+    WoopRay wr;
+    const WideNode* nodes = tl.nodes;
+    const float4* tris = nullptr;
+    uint32_t inst_idx = 0;
+    bool inst_opaque = true;
+    bool in_blas = false;
+    uint32_t blas_base = 0;
+    uint32_t sp = 0, base = 0;
+    bool lost = false;
+    uint32_t restarts = 0;
+    auto push = [&](uint32_t e) {
+        lds_stack[(sp & (kStackDepth - 1)) * stride] = e;
+        sp++;
+        if (sp - base > uint32_t(kStackDepth)) {
+            base++;
+            lost = true;
+        }
+    };
+    push(tl.root);
+    for (;;) {
+        if (sp == base) {
+            if (!lost) break;
+            lost = false;
+            if (kStats) st.overflows++;
+            if (++restarts > kMaxRestarts) {
+                if (tl.error_flags) atomicOr(tl.error_flags, kErrStackOverflow);
+                break;
+            }
+            if (in_blas) {
+                in_blas = false;
+                cur = world;
+                nodes = tl.nodes;
+            }
+            push(tl.root);
+            continue;
+        }
+        sp--;
+        uint32_t e = lds_stack[(sp & (kStackDepth - 1)) * stride];
+        if (in_blas && sp < blas_base) {
+            in_blas = false;
+            cur = world;
+            nodes = tl.nodes;
+        }
+        if (e & kLeafBit) {
+            uint32_t first = e & kLeafFirstMask;
+            uint32_t count = ((e >> kLeafCountShift) & 15u) + 1u;
+            if (!in_blas) {
+                // TLAS leaf: one instance
+                const InstanceRecord& ir = tl.inst[first];
+                if (kStats) st.instances++;
+                uint32_t mf = ir.mask;
+                uint32_t fl = ir.flags;
+                if ((mf & 0xFFu) == 0u || (fl & kInstEmpty)) continue;
+                const BlasDesc* bd = ir.blas;
+                float4 r0 = ir.w2o[0], r1 = ir.w2o[1], r2 = ir.w2o[2];
+                float3 oo = mk3(r0.x * o.x + r0.y * o.y + r0.z * o.z + r0.w, r1.x * o.x + r1.y * o.y + r1.z * o.z + r1.w,
+                                r2.x * o.x + r2.y * o.y + r2.z * o.z + r2.w);
+                float3 od = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+                                r2.x * d.x + r2.y * d.y + r2.z * d.z);
+                cur.o = oo;
+                cur.d = od;
+                cur.inv = mk3(safe_inv(od.x), safe_inv(od.y), safe_inv(od.z));
+                wr = woop_setup(oo, od);
+                nodes = bd->nodes;
+                tris = bd->tris;
+                inst_idx = first;
+                bool opq = (fl & kInstOpaqueGeom) != 0;
+                if (fl & kInstForceOpaque) opq = true;
+                if (fl & kInstForceNoOpaque) opq = false;
+                if (ray_flags & RT_RAY_FLAG_OPAQUE) opq = true;
+                if (ray_flags & RT_RAY_FLAG_NO_OPAQUE) opq = false;
+                inst_opaque = opq;
+                in_blas = true;
+                blas_base = sp;
+                push(bd->root);
+                continue;
+            }
+            for (uint32_t k = 0; k < count; k++) {
+                const float4* tp = tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                if (kStats) st.tris++;
+                float t, bu, bv;
+                if (!woop_hit(wr, xyz(a), xyz(b), xyz(c), tmin, best_t, t, bu, bv)) continue;
+                uint32_t prim = __float_as_uint(a.w);
+                if (found && t == best_t) {
+                    uint32_t hi = tl.inst[inst_idx].instance_id, ho = tl.inst[hit.inst].instance_id;
+                    if (hi > ho || (hi == ho && prim > hit.prim)) continue;
+                }
+                if (kAnyHit && !inst_opaque) {
+                    if (kStats) st.anyhits++;
+                    if (ah.ignore(inst_idx, prim, bu, bv)) continue;
+                }
+                found = true;
+                best_t = t;
+                hit.t = t;
+                hit.u = bu;
+                hit.v = bv;
+                hit.inst = inst_idx;
+                hit.prim = prim;
+                if (terminate) return true;
+            }
+            continue;
+        }
+        // internal node
+        if (kStats) st.nodes++;
+        WideNode n = load_node(nodes, e);
+        float tn[4];
+        uint32_t mask = node_hits(n, cur, tmin, best_t, tn);
+        if (!mask) continue;
+        // order the hit children near-to-far; push far first
+        float key[4];
+        uint32_t ent[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            bool h = (mask >> k) & 1u;
+            key[k] = h ? tn[k] : INFINITY;
+            uint32_t cnt = n.count[k];
+            ent[k] = h ? (cnt ? (kLeafBit | ((cnt - 1u) << kLeafCountShift) | n.child[k]) : n.child[k]) : kEmpty;
+        }
+#define RT_CSWAP(i, j)                                                         \
+    if (key[j] < key[i]) {                                                     \
+        float tk = key[i]; key[i] = key[j]; key[j] = tk;                       \
+        uint32_t te = ent[i]; ent[i] = ent[j]; ent[j] = te;                    \
+    }
+        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+#pragma unroll
+        for (int k = 3; k >= 0; k--)
+            if (ent[k] != kEmpty) push(ent[k]);
+    }
+    return found;
+}
+
+}  // namespace rt

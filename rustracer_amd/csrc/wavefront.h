@@ -1,0 +1,20 @@
+// wavefront.h — the wavefront pipeline (per-bounce ray queues) behind RT_MODE_WAVEFRONT.
+#pragma once
+
+#include <string>
+
+#include "render.h"
+
+namespace rt {
+
+struct WavefrontState;
+
+WavefrontState* wavefront_create();
+void wavefront_destroy(WavefrontState* s);
+// Renders the frame described by `fa` with queue kernels; same image as the
+// megakernel.  Returns 0 or a hipError_t code (kWavefrontUnsupported when the
+// wavefront path is not available for this frame).
+constexpr int kWavefrontUnsupported = -1000;
+int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, std::string* err);
+
+}  // namespace rt

@@ -1,0 +1,327 @@
+"""Host-side mirror of the reference's ray-tracing operator surface, over the C ABI.
+
+The reference renderer enters its ray-tracing path at two points
+(crates/libs/asset_loader/src/acceleration_structures.rs, crates/examples/gltf_viewer/src/main.rs):
+  create_as(context, doc, buffers, flags) -> (Vec<AccelerationStructure>, TopAS)   :79-129
+  create_top_as(context, doc, blases, flags, cmd)   -> TopAS                         :136-249
+  record_raytracing_commands -> trace_rays(&sbt, W, H)                          main.rs:244-270
+This module offers the same three operations (same argument meaning: one BLAS per
+geometry id with its OPAQUE flag, one instance per scene instance with custom index =
+geometry id, a trace writing the rgba32f accumulation image) on top of
+librt_gfx950.so.  Device memory comes from PyTorch (HIP allocations); the library
+enqueues on PyTorch's current stream so torch copies and traces are ordered.
+
+There is no CPU fallback: if librt_gfx950.so is missing or no GPU is visible,
+loading raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import layouts as L
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librt_gfx950.so")
+
+RT_OK = 0
+STATUS = {0: "RT_OK", -1: "RT_ERR_INVALID_ARGUMENT", -2: "RT_ERR_HIP", -3: "RT_ERR_OUT_OF_MEMORY",
+          -4: "RT_ERR_STACK_OVERFLOW", -5: "RT_ERR_QUEUE_OVERFLOW", -6: "RT_ERR_UNSUPPORTED",
+          -7: "RT_ERR_INVALID_HANDLE"}
+MODE_MEGAKERNEL, MODE_WAVEFRONT = 0, 1
+GEOMETRY_OPAQUE = 0x1
+RAY_FLAG_NONE, RAY_FLAG_OPAQUE, RAY_FLAG_NO_OPAQUE = 0x0, 0x1, 0x2
+RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT = 0x4, 0x8
+
+# every symbol include/rt/rt.h declares
+EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", "rt_build_blas_batch",
+           "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
+           "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info"]
+
+
+class RtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class BlasInput(C.Structure):
+    _fields_ = [("vertex_data", C.c_void_p), ("vertex_stride", C.c_uint32), ("max_vertex", C.c_uint32),
+                ("index_data", C.c_void_p), ("first_vertex", C.c_uint32), ("primitive_offset", C.c_uint32),
+                ("primitive_count", C.c_uint32), ("geometry_flags", C.c_uint32)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
+                ("wrap_s", C.c_uint32), ("wrap_t", C.c_uint32), ("mag_filter", C.c_uint32),
+                ("min_filter", C.c_uint32)]
+
+
+class Bindings(C.Structure):
+    _fields_ = [("ubo", C.c_void_p), ("vertices", C.c_void_p), ("indices", C.c_void_p),
+                ("prim_infos", C.c_void_p), ("materials", C.c_void_p), ("dlights", C.c_void_p),
+                ("num_dlights", C.c_uint32), ("plights", C.c_void_p), ("num_plights", C.c_uint32),
+                ("textures", C.c_void_p), ("num_textures", C.c_uint32), ("num_vertices", C.c_uint32),
+                ("num_indices", C.c_uint32), ("num_prim_infos", C.c_uint32), ("num_materials", C.c_uint32)]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("rays", "primary_rays", "extension_rays", "shadow_rays", "node_visits",
+                                            "tri_tests", "instance_visits", "shade_events", "anyhit_calls",
+                                            "tex_fetches", "light_candidates", "stack_overflows")] + \
+               [("kernel_ms", C.c_double)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_uint64), ("tris", C.c_uint64), ("root", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("first_prim", C.c_uint32), ("num_prims", C.c_uint32), ("bounds", C.c_float * 6)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads librt_gfx950.so and declares the C signatures.  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with `python -m rustracer_amd.build`")
+    lib = C.CDLL(path)
+    i, u, vp, u64 = C.c_int, C.c_uint32, C.c_void_p, C.c_uint64
+    sig = {
+        "rt_create": (i, [C.POINTER(vp), i, vp]), "rt_destroy": (None, [vp]), "rt_last_error": (C.c_char_p, [vp]),
+        "rt_sync": (i, [vp]), "rt_version": (C.c_char_p, []),
+        "rt_build_blas_batch": (i, [vp, vp, u, C.POINTER(u64)]), "rt_build_blas": (i, [vp, vp, C.POINTER(u64)]),
+        "rt_blas_device_address": (i, [vp, u64, C.POINTER(u64)]), "rt_destroy_blas": (None, [vp, u64]),
+        "rt_build_tlas": (i, [vp, vp, u, C.POINTER(u64)]), "rt_destroy_tlas": (None, [vp, u64]),
+        "rt_trace_rays": (i, [vp, u64, vp, vp, vp, u, u, u, vp]),
+        "rt_trace_ray_queries": (i, [vp, u64, vp, vp, u, u, vp]),
+        "rt_trace_primary": (i, [vp, u64, vp, u, u, vp]),
+        "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def to_device(arr, device="cuda"):
+    """Copies a numpy array (any dtype) into a HIP allocation; returns the torch tensor."""
+    torch = _torch()
+    a = np.ascontiguousarray(arr)
+    t = torch.from_numpy(a.view(np.uint8).reshape(-1).copy()).to(device)
+    return t
+
+
+class Context:
+    """rt_create / rt_destroy.  Enqueues on PyTorch's current stream of `device`."""
+
+    def __init__(self, device=0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("no HIP device visible; the gfx950 library has no CPU path")
+        self.lib = load_library()
+        self.device = device
+        with torch.cuda.device(device):
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = C.c_void_p()
+        rc = self.lib.rt_create(C.byref(h), device, C.c_void_p(stream))
+        if rc != RT_OK:
+            raise RtError(rc, self.lib.rt_last_error(None).decode())
+        self.handle = h
+
+    def check(self, rc):
+        if rc != RT_OK:
+            raise RtError(rc, self.lib.rt_last_error(self.handle).decode())
+
+    def sync(self):
+        self.check(self.lib.rt_sync(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.rt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Buffers:
+    """The renderer's device buffers (globals.rs Buffers::new + textures): vertex,
+    index, geometry-info, material, directional/point light buffers, textures."""
+
+    def __init__(self, scene, device="cuda"):
+        self.scene = scene
+        self.vertices = to_device(scene.vertices, device)
+        self.indices = to_device(scene.indices, device)
+        self.prim_infos = to_device(scene.prim_infos, device)
+        self.materials = to_device(scene.materials, device)
+        self.dlights = to_device(scene.dlights, device) if len(scene.dlights) else None
+        self.plights = to_device(scene.plights, device) if len(scene.plights) else None
+        self.textures = [to_device(t.data, device) for t in scene.textures]
+        self._tex = (Texture * max(1, len(scene.textures)))()
+        for k, t in enumerate(scene.textures):
+            h, w = t.data.shape[:2]
+            self._tex[k] = Texture(self.textures[k].data_ptr(), w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter,
+                                   t.min_filter)
+
+    def bindings(self, ubo):
+        """rt_bindings for one trace; `ubo` (UBO record) must stay alive during the call."""
+        s = self.scene
+        self._ubo = np.ascontiguousarray(ubo)
+        return Bindings(self._ubo.ctypes.data, self.vertices.data_ptr(), self.indices.data_ptr(),
+                        self.prim_infos.data_ptr(), self.materials.data_ptr(),
+                        self.dlights.data_ptr() if self.dlights is not None else None, len(s.dlights),
+                        self.plights.data_ptr() if self.plights is not None else None, len(s.plights),
+                        C.addressof(self._tex) if s.textures else None, len(s.textures), len(s.vertices),
+                        len(s.indices), len(s.prim_infos), len(s.materials))
+
+
+class TopAS:
+    def __init__(self, ctx, handle, instances):
+        self.ctx, self.handle, self.instances = ctx, handle, instances
+
+    def info(self):
+        i = BvhInfo()
+        self.ctx.check(self.ctx.lib.rt_tlas_info(self.ctx.handle, self.handle, C.byref(i)))
+        return i
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.ctx.lib.rt_destroy_tlas(self.ctx.handle, self.handle)
+        except Exception:
+            pass
+
+
+class Blas:
+    def __init__(self, ctx, handle):
+        self.ctx, self.handle = ctx, handle
+
+    def address(self):
+        a = C.c_uint64()
+        self.ctx.check(self.ctx.lib.rt_blas_device_address(self.ctx.handle, self.handle, C.byref(a)))
+        return a.value
+
+    def info(self):
+        i = BvhInfo()
+        self.ctx.check(self.ctx.lib.rt_blas_info(self.ctx.handle, self.handle, C.byref(i)))
+        return i
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.ctx.lib.rt_destroy_blas(self.ctx.handle, self.handle)
+        except Exception:
+            pass
+
+
+def blas_inputs(scene, buffers):
+    """primitive_to_vk_geometry (acceleration_structures.rs:17-76) for every geometry id."""
+    geo = scene.geometries
+    ins = (BlasInput * len(geo))()
+    vb, ib = buffers.vertices.data_ptr(), buffers.indices.data_ptr()
+    for g in range(len(geo)):
+        first_vertex, index_offset, count, opaque = (int(x) for x in geo[g])
+        idx = scene.indices[index_offset:index_offset + 3 * count]
+        max_vertex = int(idx.max()) if count else 0
+        ins[g] = BlasInput(vb, L.VERTEX.itemsize, max_vertex, ib, first_vertex, 4 * index_offset, count,
+                           GEOMETRY_OPAQUE if opaque else 0)
+    return ins
+
+
+def create_as(ctx, scene, buffers):
+    """create_as (acceleration_structures.rs:79-129): one BLAS per geometry id, built in
+    one batch, then create_top_as.  Returns (blases, TopAS)."""
+    ins = blas_inputs(scene, buffers)
+    handles = (C.c_uint64 * len(ins))()
+    ctx.check(ctx.lib.rt_build_blas_batch(ctx.handle, ins, len(ins), handles))
+    blases = [Blas(ctx, h) for h in handles]
+    return blases, create_top_as(ctx, scene, blases)
+
+
+def create_top_as(ctx, scene, blases):
+    """create_top_as (acceleration_structures.rs:136-249): the scene's instance records
+    with the BLAS reference replaced by the BLAS device address, uploaded and built."""
+    inst = np.array(scene.instances, copy=True)
+    addr = np.array([b.address() for b in blases], np.uint64)
+    inst["acceleration_structure_reference"] = addr[scene.instances["acceleration_structure_reference"].astype(np.int64)]
+    d_inst = to_device(inst)
+    out = C.c_uint64()
+    ctx.check(ctx.lib.rt_build_tlas(ctx.handle, C.c_void_p(d_inst.data_ptr()), len(inst), C.byref(out)))
+    return TopAS(ctx, out.value, d_inst)
+
+
+def new_accumulation(width, height, device="cuda"):
+    """The rgba32f accumulation image (crates/libs/app/src/lib.rs:316-325)."""
+    torch = _torch()
+    return torch.zeros((height, width, 4), dtype=torch.float32, device=device)
+
+
+def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERNEL, stats=False):
+    """record_raytracing_commands + trace_rays (main.rs:244-270): runs the pipeline for
+    a width x height launch into `accum` (torch float32 (h, w, 4) on the device).
+    Returns a dict of counters when stats=True (the counting kernel variant)."""
+    b = buffers.bindings(ubo)
+    st = TraceStats() if stats else None
+    ctx.check(ctx.lib.rt_trace_rays(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(accum.data_ptr()), None,
+                                    width, height, mode, C.byref(st) if st is not None else None))
+    if st is None:
+        return None
+    return {n: getattr(st, n) for n, _ in TraceStats._fields_}
+
+
+def trace_ray_queries(ctx, tlas, rays, flags=0, buffers=None, ubo=None):
+    """Traverses rays (numpy RAY records or a device tensor) -> numpy HIT records."""
+    torch = _torch()
+    d_rays = to_device(rays) if isinstance(rays, np.ndarray) else rays
+    n = d_rays.numel() // L.RAY.itemsize
+    d_hits = torch.empty(n * L.HIT.itemsize, dtype=torch.uint8, device=d_rays.device)
+    b = buffers.bindings(ubo) if buffers is not None else None
+    ctx.check(ctx.lib.rt_trace_ray_queries(ctx.handle, tlas.handle, C.byref(b) if b is not None else None,
+                                           C.c_void_p(d_rays.data_ptr()), n, flags, C.c_void_p(d_hits.data_ptr())))
+    ctx.sync()
+    return d_hits.cpu().numpy().view(L.HIT)
+
+
+def trace_primary(ctx, tlas, buffers, ubo, width, height):
+    torch = _torch()
+    d_hits = torch.empty(width * height * L.HIT.itemsize, dtype=torch.uint8, device="cuda")
+    b = buffers.bindings(ubo)
+    ctx.check(ctx.lib.rt_trace_primary(ctx.handle, tlas.handle, C.byref(b), width, height,
+                                       C.c_void_p(d_hits.data_ptr())))
+    ctx.sync()
+    return d_hits.cpu().numpy().view(L.HIT)
+
+
+def read_nodes(ctx, info):
+    """Copies a BVH's node array (128-byte nodes) to the host as structured records."""
+    ctx.sync()
+    return np.frombuffer(read_device(info.nodes, 128 * info.num_nodes), NODE)
+
+
+NODE = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<f4", 4), ("hi_y", "<f4", 4),
+                 ("lo_z", "<f4", 4), ("hi_z", "<f4", 4), ("child", "<u4", 4), ("count", "<u4", 4)])
+
+
+def read_device(ptr, nbytes):
+    """Copies nbytes from a device pointer to a host bytes object (hipMemcpy D2H)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    buf = (C.c_uint8 * nbytes)()
+    rc = hip.hipMemcpy(C.addressof(buf), C.c_void_p(ptr), nbytes, 2)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    return bytes(buf)
