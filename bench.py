@@ -1,0 +1,174 @@
+"""Benchmark: Mrays/s of one frame of the path-tracing hot path on one MI355X.
+
+Workload (one step = one frame): the displaced icosphere, 1,997,120 triangles in 1951
+geometries of 1024 triangles (one BLAS each, one TLAS over them), three point lights,
+1920x1080, 64 samples per pixel, depth 8, seed 1.  Every traced segment counts as a
+ray (camera + extension + shadow), as counted by the kernels' counting variant.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build measures
+one GPU.  The CPU baseline is the project's FP32 CPU restatement (oracle/) on a
+bounded stratified pixel sample of the same frame, on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# bytes per unit of work (reference layouts): a 4-wide node, a triangle record, an
+# instance record, the closest-hit gather (3 indices + 3 Vertex + PrimInfo +
+# MaterialRaw = 668 B), a bilinear texture fetch, a light record, and the
+# accumulation read+write per pixel.
+BYTES = {"node": 128, "tri": 48, "instance": 128, "shade": 668, "anyhit": 668, "tex": 16, "light": 48, "pixel": 32}
+
+
+def algorithmic_bytes(st, pixels):
+    return (st["node_visits"] * BYTES["node"] + st["tri_tests"] * BYTES["tri"] +
+            st["instance_visits"] * BYTES["instance"] + st["shade_events"] * BYTES["shade"] +
+            st["anyhit_calls"] * BYTES["anyhit"] + st["tex_fetches"] * BYTES["tex"] +
+            st["light_candidates"] * BYTES["light"] + pixels * BYTES["pixel"])
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024.0
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(scene, target_s, threads):
+    """FP32 oracle on a stratified pixel sample, timed on the host cores."""
+    import numpy as np
+    from oracle import binding as OB
+    o = OB.OracleScene(scene)
+    W, H = scene.width, scene.height
+    rng = np.random.default_rng(0)
+    order = rng.permutation(W * H)
+    n = 64
+    t0 = time.perf_counter()
+    _, st = o.render_pixels(np.stack([order[:n] % W, order[:n] // W], 1), precision=32, threads=threads)
+    dt = time.perf_counter() - t0
+    n2 = int(min(W * H, max(n, n * target_s / max(dt, 1e-3))))
+    xy = np.stack([order[n:n + n2] % W, order[n:n + n2] // W], 1)
+    t0 = time.perf_counter()
+    _, st = o.render_pixels(xy, precision=32, threads=threads)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": st["rays"] / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{n2} random pixels of the same 1920x1080 frame at 64 spp, depth 8 "
+                      f"({st['rays']} rays in {dt:.1f} s), FP32 C++ restatement, OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["megakernel", "wavefront", "best"], default="best")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--spp", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the path-tracing frame shards by image tiles, but this build runs "
+              "and reports a single MI355X only (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from rustracer_amd import rt, scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.displaced_sphere(spp=args.spp)
+    gen_s = time.perf_counter() - t0
+    ctx = rt.Context(0)
+    buf = rt.Buffers(sc)
+    torch.cuda.synchronize()
+    # acceleration-structure build, timed on its own
+    build_ms = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        blases, tlas = rt.create_as(ctx, sc, buf)
+        ctx.sync()
+        build_ms.append((time.perf_counter() - t0) * 1e3)
+    W, H = sc.width, sc.height
+    acc = rt.new_accumulation(W, H)
+
+    modes = {"megakernel": rt.MODE_MEGAKERNEL, "wavefront": rt.MODE_WAVEFRONT}
+    wanted = list(modes) if args.mode == "best" else [args.mode]
+    results = {}
+    stream = torch.cuda.current_stream()
+    for name in wanted:
+        try:
+            st = rt.trace_rays(ctx, tlas, buf, sc.ubo, acc, W, H, mode=modes[name], stats=True)
+        except rt.RtError as e:
+            if e.code == -6:  # RT_ERR_UNSUPPORTED
+                continue
+            raise
+        for _ in range(args.warmup):
+            rt.trace_rays(ctx, tlas, buf, sc.ubo, acc, W, H, mode=modes[name])
+        ctx.sync()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ev[k][0].record(stream)
+            rt.trace_rays(ctx, tlas, buf, sc.ubo, acc, W, H, mode=modes[name])
+            ev[k][1].record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        ms_per_step = wall * 1e3 / args.steps
+        results[name] = {"st": st, "ms_per_step": ms_per_step, "kernel_ms": kern_ms,
+                         "mrays": st["rays"] / (ms_per_step / 1e3) / 1e6}
+    best = max(results, key=lambda k: results[k]["mrays"])
+    r = results[best]
+    st = r["st"]
+    bytes_per_launch = algorithmic_bytes(st, W * H)
+    achieved = bytes_per_launch / (r["kernel_ms"] / 1e3) / 1e9
+    kernel_name = "k_megakernel" if best == "megakernel" else "wavefront"
+    traffic = measured_traffic(kernel_name + "_displaced_sphere")
+    line = {
+        "metric": "Mrays/s at 1080p/64spp (wavefront vs megakernel) at RMSE<=1e-3; adjoint ms/iter",
+        "value": round(r["mrays"], 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, generated in-process)",
+        "config": {"workload": f"displaced icosphere {sc.num_triangles} tris in {len(sc.geometries)} BLAS, "
+                               f"{W}x{H}, {int(sc.ubo['number_of_samples'])} spp, depth "
+                               f"{int(sc.ubo['number_of_bounces'])}, 3 point lights",
+                   "mode": best, "rays_per_frame": st["rays"],
+                   "modes": {k: round(v["mrays"], 2) for k, v in results.items()},
+                   "build_ms": round(min(build_ms), 2), "scene_gen_s": round(gen_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": kernel_name, "kernel_ms": round(r["kernel_ms"], 3),
+                     "algorithmic_bytes": int(bytes_per_launch)},
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -88,7 +88,8 @@ def lib():
             "orc_offset_ray": (None, [vp, vp, vp]), "orc_texture_sample": (None, [vp, d, d, dp]),
             "orc_intersect_triangle": (i, [dp, dp, dp, dp, dp, d, d, dp]),
             "orc_scene_create": (vp, [vp]), "orc_scene_destroy": (None, [vp]),
-            "orc_trace_rays": (i, [vp, vp, u, u, vp, vp, i, i]),
+            "orc_trace_rays": (i, [vp, vp, u, u, u, vp, vp, i, i]),
+            "orc_render_pixels": (i, [vp, vp, u, u, vp, u, i, i, vp, vp]),
             "orc_trace_primary": (i, [vp, vp, u, u, vp, vp, i]),
             "orc_render": (i, [vp, vp, u, u, u, u, u, u, i, i, vp, vp]),
         }
@@ -167,15 +168,32 @@ class OracleScene:
         except Exception:
             pass
 
-    def trace_rays(self, rays, flags=0, brute=False, threads=0):
-        """rays: RAY structured array.  Returns (HIT array, (n,3) float64 t/u/v)."""
+    def trace_rays(self, rays, flags=1, brute=False, threads=0, seed=0):
+        """rays: RAY structured array; flags default RT_RAY_FLAG_OPAQUE.
+        Returns (HIT array, (n,3) float64 t/u/v)."""
         from rustracer_amd import layouts as L
         rays = np.ascontiguousarray(rays)
         hits = np.zeros(len(rays), L.HIT)
         tuv = np.zeros((len(rays), 3))
-        lib().orc_trace_rays(self.handle, rays.ctypes.data, len(rays), flags, hits.ctypes.data, tuv.ctypes.data,
-                             int(brute), threads)
+        lib().orc_trace_rays(self.handle, rays.ctypes.data, len(rays), flags, seed, hits.ctypes.data,
+                             tuv.ctypes.data, int(brute), threads)
         return hits, tuv
+
+    def render_pixels(self, xy, ubo=None, width=None, height=None, precision=64, threads=0):
+        """Summed radiance of the listed pixels (xy: (n, 2) ints).  Returns ((n, 4), stats)."""
+        s = self.scene
+        ubo = np.ascontiguousarray(s.ubo if ubo is None else ubo)
+        w, h = width or s.width, height or s.height
+        xy = np.ascontiguousarray(xy, np.uint32)
+        out = np.zeros((len(xy), 4))
+        st = OrcStats()
+        rc = lib().orc_render_pixels(self.handle, ubo.ctypes.data, w, h, xy.ctypes.data, len(xy), precision, threads,
+                                     out.ctypes.data, C.byref(st))
+        if rc != 0:
+            raise ValueError("orc_render_pixels rejected the pixel list")
+        stats = {n: getattr(st, n) for n, _ in OrcStats._fields_}
+        stats["rays"] = stats["primary_rays"] + stats["extension_rays"] + stats["shadow_rays"]
+        return out, stats
 
     def trace_primary(self, ubo=None, width=None, height=None, threads=0):
         from rustracer_amd import layouts as L

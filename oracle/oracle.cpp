@@ -1302,16 +1302,61 @@ static void fill_hit(const HitRec& h, const Scene& s, rt_hit* out, double* tuv) 
     if (tuv) { tuv[0] = h.t; tuv[1] = h.u; tuv[2] = h.v; }
 }
 
-int orc_trace_rays(void* sp, const float* rays, uint32_t n, uint32_t flags, rt_hit* hits, double* tuv, int brute, int threads) {
+int orc_trace_rays(void* sp, const float* rays, uint32_t n, uint32_t flags, uint32_t seed, rt_hit* hits, double* tuv,
+                   int brute, int threads) {
     const Scene& s = *(Scene*)sp;
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < int64_t(n); i++) {
         const float* r = rays + 8 * size_t(i);
         double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
-        HitRec h = trace<double>(s, o, d, r[3], r[7], flags | RT_RAY_FLAG_OPAQUE, nullptr, brute != 0, nullptr);
+        U4 rng{uint32_t(i), 0u, seed, 0u};  // stream of a ray query: (index, 0, seed)
+        AnyHitCtx ah{&s, &rng, nullptr};
+        HitRec h = trace<double>(s, o, d, r[3], r[7], flags, &ah, brute != 0, nullptr);
         fill_hit(h, s, hits + i, tuv ? tuv + 3 * size_t(i) : nullptr);
     }
+    return 0;
+}
+
+int orc_render_pixels(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, const uint32_t* xy, uint32_t n,
+                      int precision, int threads, double* out, orc_stats* stats) {
+    const Scene& s = *(Scene*)sp;
+    if (threads > 0) omp_set_num_threads(threads);
+    for (uint32_t i = 0; i < n; i++)
+        if (xy[2 * i] >= W || xy[2 * i + 1] >= H) return -1;
+    orc_stats total{};
+#pragma omp parallel
+    {
+        orc_stats local{};
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < int64_t(n); i++) {
+            uint32_t x = xy[2 * i], y = xy[2 * i + 1];
+            double* o = out + 4 * size_t(i);
+            if (precision == 32) {
+                Tracer<float> tr{s, *ubo, stats ? &local : nullptr};
+                V3<float> r = tr.pixel(x, y, W, H);
+                o[0] = r.x; o[1] = r.y; o[2] = r.z;
+            } else {
+                Tracer<double> tr{s, *ubo, stats ? &local : nullptr};
+                V3<double> r = tr.pixel(x, y, W, H);
+                o[0] = r.x; o[1] = r.y; o[2] = r.z;
+            }
+            o[3] = 0.0;
+        }
+        if (stats) {
+#pragma omp critical
+            {
+                total.primary_rays += local.primary_rays;
+                total.extension_rays += local.extension_rays;
+                total.shadow_rays += local.shadow_rays;
+                total.node_visits += local.node_visits;
+                total.tri_tests += local.tri_tests;
+                total.shade_events += local.shade_events;
+                total.anyhit_calls += local.anyhit_calls;
+            }
+        }
+    }
+    if (stats) *stats = total;
     return 0;
 }
 

@@ -107,9 +107,16 @@ void* orc_scene_create(const orc_scene_desc* desc);
 void orc_scene_destroy(void* scene);
 /* rays: n x {ox,oy,oz,tmin,dx,dy,dz,tmax} (float).  hits: n x rt_hit (t,u,v in
  * float for layout compatibility) plus n x 3 doubles of (t,u,v) when hits_tuv is
- * non-NULL.  brute != 0 tests every triangle of every instance. */
-int orc_trace_rays(void* scene, const float* rays, uint32_t n, uint32_t ray_flags,
+ * non-NULL.  brute != 0 tests every triangle of every instance.  Without
+ * RT_RAY_FLAG_OPAQUE, non-opaque geometry runs the any-hit alpha test; ray i draws
+ * from the stream (i, 0, seed, 0). */
+int orc_trace_rays(void* scene, const float* rays, uint32_t n, uint32_t ray_flags, uint32_t seed,
                    rt_hit* hits, double* hits_tuv, int brute, int threads);
+/* Renders the n pixels xy[2i], xy[2i+1] of a width x height launch; out receives
+ * n x 4 doubles (summed radiance, alpha 0). */
+int orc_render_pixels(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
+                      const uint32_t* xy, uint32_t n, int precision, int threads, double* out,
+                      orc_stats* stats);
 /* Primary hits of sample 0 (camera ray and jitter as RayTracing.rgen computes them). */
 int orc_trace_primary(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
                       rt_hit* hits, double* hits_tuv, int threads);

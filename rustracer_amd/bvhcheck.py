@@ -68,14 +68,15 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
 
 
 INST = np.dtype([("w2o", "<f4", (3, 4)), ("o2w", "<f4", (3, 4)), ("blas", "<u8"), ("custom_index", "<u4"),
-                 ("mask", "<u4"), ("flags", "<u4"), ("instance_id", "<u4"), ("_pad", "<u4", 2)])
+                 ("mask", "<u4"), ("flags", "<u4"), ("instance_id", "<u4"), ("_pad", "<u4", 2),
+                 ("w2o_d", "<f8", 12), ("_pad2", "<u4", 8)])
 
 
 def check_tlas(ctx, tlas, num_instances):
     info = tlas.info()
     nodes = rt.read_nodes(ctx, info)
     errs = _check(nodes, info.root, 0, info.num_prims, None, "instance", num_instances)
-    inst = np.frombuffer(rt.read_device(info.tris, 128 * max(1, info.num_prims)), INST)[:info.num_prims]
+    inst = np.frombuffer(rt.read_device(info.tris, INST.itemsize * max(1, info.num_prims)), INST)[:info.num_prims]
     if num_instances and not np.array_equal(np.sort(inst["instance_id"]), np.arange(num_instances)):
         errs.append("TLAS instance ids are not a permutation")
     return errs

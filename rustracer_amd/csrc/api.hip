@@ -447,6 +447,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
             r.w2o[row] = make_float4(float(w[row * 4]), float(w[row * 4 + 1]), float(w[row * 4 + 2]), float(w[row * 4 + 3]));
             r.o2w[row] = make_float4(in.transform[row][0], in.transform[row][1], in.transform[row][2], in.transform[row][3]);
         }
+        for (int k = 0; k < 12; k++) r.w2o_d[k] = w[k];
         r.blas = be.d_desc;
         r.custom_index = in.custom_index_and_mask & 0xFFFFFFu;
         r.mask = in.custom_index_and_mask >> 24;

@@ -6,8 +6,9 @@
 //              loads), child references and leaf counts.
 //   Triangle    48 B: three float4 {v.xyz, w}; v0.w carries the primitive index
 //              (gl_PrimitiveID) as bits.  Leaves reference contiguous runs.
-//   InstanceRecord 128 B: world->object and object->world 3x4 rows, the BLAS
-//              descriptor address, custom index, mask, flags, instance id.
+//   InstanceRecord 256 B: world->object and object->world 3x4 rows (fp32), the
+//              BLAS descriptor address, custom index, mask, flags, instance id,
+//              and world->object in fp64 for hit refinement.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,7 +52,7 @@ enum : uint32_t {
 };
 
 struct alignas(16) InstanceRecord {
-    float4 w2o[3];        // rows of the world-to-object 3x4 matrix
+    float4 w2o[3];        // rows of the world-to-object 3x4 matrix (traversal)
     float4 o2w[3];        // rows of the object-to-world 3x4 matrix (gl_ObjectToWorldEXT)
     const BlasDesc* blas;
     uint32_t custom_index;
@@ -59,8 +60,10 @@ struct alignas(16) InstanceRecord {
     uint32_t flags;       // kInst*
     uint32_t instance_id; // gl_InstanceID (index in the caller's instance array)
     uint32_t _pad[2];
+    double w2o_d[12];     // world-to-object in fp64 (hit refinement)
+    uint32_t _pad2[8];
 };
-static_assert(sizeof(InstanceRecord) == 128, "InstanceRecord is 128 bytes");
+static_assert(sizeof(InstanceRecord) == 256, "InstanceRecord is 256 bytes");
 
 enum : uint32_t { kErrStackOverflow = 1u };
 
@@ -90,6 +93,31 @@ __host__ __device__ inline float3 xyz(float4 a) { return mk3(a.x, a.y, a.z); }
 __device__ inline float3 fmin3(float3 a, float3 b) { return mk3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
 __device__ inline float3 fmax3(float3 a, float3 b) { return mk3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
 __device__ inline float comp(float3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ------------------------------------------------------------ double3 math
+// Shading runs in fp64 (see path.h); a plain struct avoids the operator set of
+// HIP's vector types.
+struct D3 {
+    double x, y, z;
+};
+__host__ __device__ inline D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__host__ __device__ inline D3 d3(double s) { return D3{s, s, s}; }
+__host__ __device__ inline D3 d3(float3 v) { return D3{v.x, v.y, v.z}; }
+__host__ __device__ inline D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline D3 operator-(D3 a) { return D3{-a.x, -a.y, -a.z}; }
+__host__ __device__ inline D3 operator*(D3 a, D3 b) { return D3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__host__ __device__ inline D3 operator*(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline D3 operator*(double s, D3 a) { return D3{a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline D3 operator/(D3 a, double s) { return D3{a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline D3 dcross(D3 a, D3 b) {
+    return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ inline double dlength(D3 a) { return sqrt(ddot(a, a)); }
+__device__ inline D3 dnormalize(D3 a) { return a / sqrt(ddot(a, a)); }
+__host__ __device__ inline float3 to_f3(D3 a) { return make_float3(float(a.x), float(a.y), float(a.z)); }
+__host__ __device__ inline double dcomp(D3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 
 // Order-preserving float <-> uint for atomic min/max of floats.
 __device__ inline uint32_t f2ord(float f) {

@@ -445,18 +445,16 @@ __device__ inline float half_area(float3 l, float3 h) {
     return e.x * e.y + e.y * e.z + e.z * e.x;
 }
 
-// task = (binary ref, wide node index, segment)
-__global__ void k_collapse(CollapseArgs a, const uint4* tasks_in, const uint32_t* n_in, uint4* tasks_out,
-                           uint32_t* n_out) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *n_in) return;
-    uint4 task = tasks_in[t];
-    uint32_t b = task.x, w = task.y, seg = task.z;
-    uint32_t c[4];
+// Collapse of one level, in two passes so node numbering is deterministic:
+// gather picks the (up to 4) children of every task and counts the ones that become
+// internal wide nodes; after an exclusive scan of the counts, emit writes the nodes
+// and numbers the new internal nodes level by level in task order.
+// task = (binary ref or kEmpty, wide node index, segment)
+__device__ inline int gather_children(const CollapseArgs& a, uint32_t b, uint32_t c[4], float3 cl[4], float3 ch[4]) {
     int nc;
+    if (b == kEmpty) return 0;
     if (b & kLeafBit) { c[0] = b; nc = 1; }
     else { c[0] = a.left[b]; c[1] = a.right[b]; nc = 2; }
-    float3 cl[4], ch[4];
     for (int k = 0; k < nc; k++) load_box(c[k], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl[k], ch[k]);
     while (nc < 4) {
         int best = -1;
@@ -474,43 +472,73 @@ __global__ void k_collapse(CollapseArgs a, const uint4* tasks_in, const uint32_t
         load_box(c[nc], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl[nc], ch[nc]);
         nc++;
     }
+    return nc;
+}
+
+__device__ inline bool becomes_internal(const CollapseArgs& a, uint32_t r) {
+    return !(r & kLeafBit) && ref_count(a, r) > a.leaf_max;
+}
+
+__global__ void k_collapse_gather(CollapseArgs a, const uint4* tasks, uint32_t n, uint4* refs, uint32_t* counts,
+                                  uint32_t* offsets) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t c[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+    float3 cl[4], ch[4];
+    int nc = gather_children(a, tasks[t].x, c, cl, ch);
+    uint32_t internal = 0;
+    for (int k = 0; k < nc; k++) internal += becomes_internal(a, c[k]) ? 1u : 0u;
+    refs[t] = make_uint4(c[0], c[1], c[2], c[3]);
+    counts[t] = internal;
+    offsets[t] = internal;
+}
+
+__global__ void k_collapse_emit(CollapseArgs a, const uint4* tasks, uint32_t n, const uint4* refs,
+                                const uint32_t* offsets, uint32_t node_base, uint4* tasks_out) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint4 task = tasks[t];
+    uint32_t w = task.y, seg = task.z;
+    uint4 r4 = refs[t];
+    uint32_t c[4] = {r4.x, r4.y, r4.z, r4.w};
     float pad = a.seg_pad[seg];
+    uint32_t next = offsets[t];
     WideNode out;
     for (int k = 0; k < 4; k++) {
-        if (k >= nc) {
+        if (c[k] == kEmpty) {
             out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = 0.f;
             out.hi_x[k] = out.hi_y[k] = out.hi_z[k] = 0.f;
             out.child[k] = kEmpty;
             out.count[k] = 0;
             continue;
         }
-        out.lo_x[k] = cl[k].x - pad; out.lo_y[k] = cl[k].y - pad; out.lo_z[k] = cl[k].z - pad;
-        out.hi_x[k] = ch[k].x + pad; out.hi_y[k] = ch[k].y + pad; out.hi_z[k] = ch[k].z + pad;
-        uint32_t cnt = ref_count(a, c[k]);
-        if ((c[k] & kLeafBit) || cnt <= a.leaf_max) {
+        float3 cl, ch;
+        load_box(c[k], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl, ch);
+        out.lo_x[k] = cl.x - pad; out.lo_y[k] = cl.y - pad; out.lo_z[k] = cl.z - pad;
+        out.hi_x[k] = ch.x + pad; out.hi_y[k] = ch.y + pad; out.hi_z[k] = ch.z + pad;
+        if (!becomes_internal(a, c[k])) {
             out.child[k] = ref_first(a, c[k]);
-            out.count[k] = cnt;
+            out.count[k] = ref_count(a, c[k]);
         } else {
-            uint32_t idx = atomicAdd(a.node_counter, 1u);
-            uint32_t slot = atomicAdd(n_out, 1u);
-            if (idx >= a.max_nodes || slot >= a.max_nodes) {
+            uint32_t idx = node_base + next;
+            if (idx >= a.max_nodes || next >= a.max_nodes) {
                 atomicOr(a.overflow, 1u);
                 out.child[k] = kEmpty;
                 out.count[k] = 0;
                 continue;
             }
-            tasks_out[slot] = make_uint4(c[k], idx, seg, 0u);
+            tasks_out[next] = make_uint4(c[k], idx, seg, 0u);
             out.child[k] = idx;
             out.count[k] = 0;
+            next++;
         }
     }
     a.nodes[w] = out;
 }
 
-// root wide node of an empty segment: all children empty
-__global__ void k_empty_roots(WideNode* nodes, const uint32_t* seg_root, uint32_t nseg) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg || seg_root[s] != kEmpty) return;
+__global__ void k_empty_nodes(WideNode* nodes, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     WideNode out;
     for (int k = 0; k < 4; k++) {
         out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = 0.f;
@@ -518,14 +546,14 @@ __global__ void k_empty_roots(WideNode* nodes, const uint32_t* seg_root, uint32_
         out.child[k] = kEmpty;
         out.count[k] = 0;
     }
-    nodes[s] = out;
+    nodes[i] = out;
 }
 
-__global__ void k_init_tasks(const uint32_t* seg_root, uint32_t nseg, uint4* tasks, uint32_t* n_tasks) {
+// level 0: one task per segment, its root is node s (kEmpty for an empty segment)
+__global__ void k_init_tasks(const uint32_t* seg_root, uint32_t nseg, uint4* tasks) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg || seg_root[s] == kEmpty) return;
-    uint32_t slot = atomicAdd(n_tasks, 1u);
-    tasks[slot] = make_uint4(seg_root[s], s, s, 0u);
+    if (s >= nseg) return;
+    tasks[s] = make_uint4(seg_root[s], s, s, 0u);
 }
 
 // padding and root bounds per segment
@@ -704,36 +732,45 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
                        nodes_tmp, nc, max_nodes, nc + 3};
         uint4* ta = s.alloc<uint4>(max_nodes);
         uint4* tb = s.alloc<uint4>(max_nodes);
-        if (!ta || !tb) return fail(int(hipErrorOutOfMemory), "out of device memory");
-        uint32_t h_init[4] = {nseg, 0, 0, 0};
-        if (hipMemcpyAsync(nc, h_init, sizeof(h_init), hipMemcpyHostToDevice, st) != hipSuccess)
-            return fail(int(hipErrorUnknown), "copy counters");
-        hipLaunchKernelGGL(k_empty_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, nodes_tmp, seg_root, nseg);
-        hipLaunchKernelGGL(k_init_tasks, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, seg_root, nseg, ta, nc + 1);
+        uint4* refs = s.alloc<uint4>(max_nodes);
+        uint32_t* counts = s.alloc<uint32_t>(max_nodes);
+        uint32_t* offs = s.alloc<uint32_t>(max_nodes);
+        const uint32_t ccap = 2 * blocks_for(max_nodes, kScanPerBlock) + 64;
+        uint32_t* ctmp = s.alloc<uint32_t>(ccap);
+        if (!ta || !tb || !refs || !counts || !offs || !ctmp) return fail(int(hipErrorOutOfMemory), "out of device memory");
+        if (hipMemsetAsync(nc, 0, 4 * sizeof(uint32_t), st) != hipSuccess) return fail(int(hipErrorUnknown), "memset");
+        hipLaunchKernelGGL(k_init_tasks, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, seg_root, nseg, ta);
         if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_init_tasks");
-        uint32_t count = 0;
-        if (hipMemcpyAsync(&count, nc + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            return fail(int(hipErrorUnknown), "read task count");
-        int in_slot = 1;
+        uint32_t count = nseg, node_base = nseg;
         for (int level = 0; count > 0; level++) {
             if (level > 200) return fail(int(hipErrorUnknown), "collapse did not terminate");
-            int out_slot = in_slot == 1 ? 2 : 1;
-            if (hipMemsetAsync(nc + out_slot, 0, sizeof(uint32_t), st) != hipSuccess) return fail(int(hipErrorUnknown), "memset");
-            hipLaunchKernelGGL(k_collapse, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, st, a, ta, nc + in_slot, tb,
-                               nc + out_slot);
-            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_collapse");
-            if (hipMemcpyAsync(&count, nc + out_slot, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipLaunchKernelGGL(k_collapse_gather, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, st, a, ta, count,
+                               refs, counts, offs);
+            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_collapse_gather");
+            rc = exclusive_scan(st, offs, count, ctmp, ccap);
+            if (rc) return fail(rc, "scan");
+            hipLaunchKernelGGL(k_collapse_emit, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, st, a, ta, count,
+                               refs, offs, node_base, tb);
+            if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_collapse_emit");
+            uint32_t last[2] = {0, 0};
+            if (hipMemcpyAsync(&last[0], offs + count - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(&last[1], counts + count - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess)
-                return fail(int(hipErrorUnknown), "read task count");
+                return fail(int(hipErrorUnknown), "read level size");
+            count = last[0] + last[1];
+            node_base += count;
+            if (node_base > max_nodes) return fail(int(hipErrorUnknown), "node capacity exceeded during collapse");
             std::swap(ta, tb);
-            in_slot = out_slot;
         }
+        if (hipMemcpyAsync(nc, &node_base, sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(int(hipErrorUnknown), "copy node count");
     } else {
+        // no primitives at all: every segment root is an empty node
         uint32_t h_init[4] = {nseg, 0, 0, 0};
         if (hipMemcpyAsync(nc, h_init, sizeof(h_init), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(int(hipErrorUnknown), "copy counters");
-        hipLaunchKernelGGL(k_empty_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, nodes_tmp, seg_root, nseg);
+        hipLaunchKernelGGL(k_empty_nodes, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, nodes_tmp, nseg);
+        if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_empty_nodes");
     }
     uint32_t hc[4] = {0, 0, 0, 0};
     if (hipMemcpyAsync(hc, nc, sizeof(hc), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)

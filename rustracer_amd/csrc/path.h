@@ -6,15 +6,20 @@
 // bounce loops, Russian roulette, accumulation), RayTracing.rchit:136-477
 // (closest hit), RayTracing.rmiss:14-45 (miss) and RayTracing.rchit:35-59,
 // RayTracing.shadow.rahit, RayTracing.shadow.rmiss (shadow ray).
+//
+// Precision: the path state (ray origin and direction, throughput, radiance) and
+// all shading are fp64; traversal receives the rays rounded to fp32 and returns the
+// closest triangle, whose t and barycentrics are then recomputed in fp64 from the
+// fp64 ray (refine_hit).
 #pragma once
 
 #include "render.h"
 
 namespace rt {
 
-constexpr float kTMin = 0.001f;     // lib/Camera.glsl:2
-constexpr float kTMax = 10000.0f;   // lib/Camera.glsl:3
-constexpr float kShadowTMin = 0.1f; // RayTracing.rchit:43
+constexpr double kTMin = 0.001;     // lib/Camera.glsl:2
+constexpr double kTMax = 10000.0;   // lib/Camera.glsl:3
+constexpr double kShadowTMin = 0.1; // RayTracing.rchit:43
 constexpr uint32_t kMinBounces = 3; // RayTracing.rgen:15
 
 struct PathCounters {
@@ -42,19 +47,25 @@ struct AlphaAnyHit {
     }
 };
 
+// column-major mat4 * (a, b, c, w), xyz part
+__device__ inline D3 mat4_mul(const float* m, double a, double b, double c, double w) {
+    return d3(m[0] * a + m[4] * b + m[8] * c + m[12] * w, m[1] * a + m[5] * b + m[9] * c + m[13] * w,
+              m[2] * a + m[6] * b + m[10] * c + m[14] * w);
+}
+
 // Camera ray of one sample, RayTracing.rgen:52-75.
-__device__ inline void camera_ray(const FrameArgs& fa, uint32_t x, uint32_t y, Rng4& rng_a, uint32_t& lcg,
-                                  float3& o, float3& d, float& tfar) {
-    float px = float(x) + 0.5f, py = float(y) + 0.5f;
+__device__ inline void camera_ray(const FrameArgs& fa, uint32_t x, uint32_t y, Rng4& rng_a, uint32_t& lcg, D3& o,
+                                  D3& d, double& tfar) {
+    double px = double(x) + 0.5, py = double(y) + 0.5;
     if (fa.antialiasing) {
-        float jx = rng_next(rng_a);
-        float jy = rng_next(rng_a);
-        px += jx - 0.5f;
-        py += jy - 0.5f;
+        double jx = rng_next(rng_a);
+        double jy = rng_next(rng_a);
+        px += jx - 0.5;
+        py += jy - 0.5;
     }
-    float uvx = (px / float(fa.width)) * 2.f - 1.f;
-    float uvy = (py / float(fa.height)) * 2.f - 1.f;
-    float offx = 0.f, offy = 0.f;
+    double uvx = (px / double(fa.width)) * 2.0 - 1.0;
+    double uvy = (py / double(fa.height)) * 2.0 - 1.0;
+    double offx = 0.0, offy = 0.0;
     if (fa.aperture != 0.f) {  // RandomInUnitDisk (lib/Random.glsl:44-54); with aperture 0 the offset is 0
         float dx, dy;
         for (;;) {
@@ -63,178 +74,208 @@ __device__ inline void camera_ray(const FrameArgs& fa, uint32_t x, uint32_t y, R
             dy = 2.f * b - 1.f;
             if (dx * dx + dy * dy < 1.f) break;
         }
-        offx = fa.aperture / 2.f * dx;
-        offy = fa.aperture / 2.f * dy;
+        offx = double(fa.aperture) / 2.0 * dx;
+        offy = double(fa.aperture) / 2.0 * dy;
     }
-    const float* M = fa.mvi;
-    const float* P = fa.pin;
-    o = mk3(M[0] * offx + M[4] * offy + M[12], M[1] * offx + M[5] * offy + M[13], M[2] * offx + M[6] * offy + M[14]);
-    float3 tg = mk3(P[0] * uvx + P[4] * uvy + P[8] + P[12], P[1] * uvx + P[5] * uvy + P[9] + P[13],
-                    P[2] * uvx + P[6] * uvy + P[10] + P[14]);
-    float3 td = normalize3(tg * fa.focus_distance - mk3(offx, offy, 0.f));
-    d = mk3(M[0] * td.x + M[4] * td.y + M[8] * td.z, M[1] * td.x + M[5] * td.y + M[9] * td.z,
-            M[2] * td.x + M[6] * td.y + M[10] * td.z);
+    o = mat4_mul(fa.mvi, offx, offy, 0.0, 1.0);
+    D3 tg = mat4_mul(fa.pin, uvx, uvy, 1.0, 1.0);
+    D3 td = dnormalize(tg * double(fa.focus_distance) - d3(offx, offy, 0.0));
+    d = mat4_mul(fa.mvi, td.x, td.y, td.z, 0.0);
     tfar = kTMax;
     if (fa.ortho > 0.f) {  // rgen:67-75
-        float k = 1.f + fa.ortho;
-        float ax = k * uvx, ay = -(k * uvy);
-        o = mk3(M[0] * ax + M[4] * ay + M[12], M[1] * ax + M[5] * ay + M[13], M[2] * ax + M[6] * ay + M[14]);
-        d = mk3(-M[8], -M[9], -M[10]);
-        tfar = 10.f * kTMax;
+        double k = 1.0 + fa.ortho;
+        o = mat4_mul(fa.mvi, k * uvx, -(k * uvy), 0.0, 1.0);
+        d = mat4_mul(fa.mvi, 0.0, 0.0, -1.0, 0.0);
+        tfar = 10.0 * kTMax;
     }
+}
+
+// t and barycentrics of the hit triangle recomputed in fp64 from the fp64 ray
+// (Woop, Benthin & Wald 2013 without the inside test: the triangle is known).
+__device__ inline void refine_hit(const TlasView& tl, const HitInfo& h, D3 o, D3 d, double& t, double& u, double& v) {
+    const InstanceRecord& ir = tl.inst[h.inst];
+    const double* m = ir.w2o_d;
+    D3 oo = d3(m[0] * o.x + m[1] * o.y + m[2] * o.z + m[3], m[4] * o.x + m[5] * o.y + m[6] * o.z + m[7],
+               m[8] * o.x + m[9] * o.y + m[10] * o.z + m[11]);
+    D3 od = d3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+               m[8] * d.x + m[9] * d.y + m[10] * d.z);
+    const float4* tp = ir.blas->tris + 3 * size_t(h.rec);
+    float4 fa = tp[0], fb = tp[1], fc = tp[2];
+    double ax = fabs(od.x), ay = fabs(od.y), az = fabs(od.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (dcomp(od, kz) < 0.0) { int tt = kx; kx = ky; ky = tt; }
+    double dz = dcomp(od, kz);
+    double Sx = dcomp(od, kx) / dz, Sy = dcomp(od, ky) / dz, Sz = 1.0 / dz;
+    D3 A = d3(fa.x, fa.y, fa.z) - oo, B = d3(fb.x, fb.y, fb.z) - oo, C = d3(fc.x, fc.y, fc.z) - oo;
+    double Ax = dcomp(A, kx) - Sx * dcomp(A, kz), Ay = dcomp(A, ky) - Sy * dcomp(A, kz);
+    double Bx = dcomp(B, kx) - Sx * dcomp(B, kz), By = dcomp(B, ky) - Sy * dcomp(B, kz);
+    double Cx = dcomp(C, kx) - Sx * dcomp(C, kz), Cy = dcomp(C, ky) - Sy * dcomp(C, kz);
+    double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    double det = U + V + W;
+    if (det == 0.0) {
+        t = h.t;
+        u = h.u;
+        v = h.v;
+        return;
+    }
+    double T = U * (Sz * dcomp(A, kz)) + V * (Sz * dcomp(B, kz)) + W * (Sz * dcomp(C, kz));
+    double inv = 1.0 / det;
+    t = T * inv;
+    u = V * inv;
+    v = W * inv;
 }
 
 // RayPayload, lib/RayTracingCommons.glsl:11-24
 struct Payload {
-    float3 hit_value, hit_point, scatter_dir, emittance;
-    float t;
+    D3 hit_value, hit_point, scatter_dir, emittance;
+    double t;
     bool need_scatter;
     Rng4 rng;
-    float volume_dis;
+    double volume_dis;
 };
 
 // RayTracing.rmiss:14-45 (skybox off: HasSky is not supported)
-__device__ inline void miss_shader(const SceneView& sv, Payload& ray, float3 dir) {
-    float3 acc = mk3(0.f, 0.f, 0.f);
-    float3 rd = normalize3(dir);
-    if (ray.t != 0.f) {
+__device__ inline void miss_shader(const SceneView& sv, Payload& ray, D3 dir) {
+    D3 acc = d3(0.0);
+    D3 rd = dnormalize(dir);
+    if (ray.t != 0.0) {
         for (uint32_t i = 0; i < sv.num_dlights; i++) {
             const rt_light& li = sv.dlights[i];
-            float c = dot3(normalize3(mk3(li.transform[0], li.transform[1], li.transform[2])), rd);
-            if (c < 0.f) acc = acc + mk3(li.color[0], li.color[1], li.color[2]) * (-c * li.intensity);
+            double c = ddot(dnormalize(ld3(li.transform)), rd);
+            if (c < 0.0) acc = acc + ld3(li.color) * (-c * double(li.intensity));
         }
     }
-    acc = acc + mk3(0.01f, 0.01f, 0.01f);
-    ray.hit_value = mk3(0.f, 0.f, 0.f);
+    acc = acc + d3(0.01);
+    ray.hit_value = d3(0.0);
     ray.need_scatter = false;
-    ray.emittance = sv.num_dlights == 0 ? mk3(0.f, 0.f, 0.f) : acc;
-    ray.t = -1.f;
+    ray.emittance = sv.num_dlights == 0 ? d3(0.0) : acc;
+    ray.t = -1.0;
 }
 
-__device__ inline float3 o2w_dir(const InstanceRecord& ir, float3 v) {
+__device__ inline D3 o2w_dir(const InstanceRecord& ir, D3 v) {
     float4 r0 = ir.o2w[0], r1 = ir.o2w[1], r2 = ir.o2w[2];
-    return mk3(r0.x * v.x + r0.y * v.y + r0.z * v.z, r1.x * v.x + r1.y * v.y + r1.z * v.z,
-               r2.x * v.x + r2.y * v.y + r2.z * v.z);
+    return d3(r0.x * v.x + r0.y * v.y + r0.z * v.z, r1.x * v.x + r1.y * v.y + r1.z * v.z,
+              r2.x * v.x + r2.y * v.y + r2.z * v.z);
 }
-__device__ inline float3 o2w_point(const InstanceRecord& ir, float3 v) {
+__device__ inline D3 o2w_point(const InstanceRecord& ir, D3 v) {
     float4 r0 = ir.o2w[0], r1 = ir.o2w[1], r2 = ir.o2w[2];
-    return mk3(r0.x * v.x + r0.y * v.y + r0.z * v.z + r0.w, r1.x * v.x + r1.y * v.y + r1.z * v.z + r1.w,
-               r2.x * v.x + r2.y * v.y + r2.z * v.z + r2.w);
+    return d3(r0.x * v.x + r0.y * v.y + r0.z * v.z + r0.w, r1.x * v.x + r1.y * v.y + r1.z * v.z + r1.w,
+              r2.x * v.x + r2.y * v.y + r2.z * v.z + r2.w);
 }
 
 // Result of the closest-hit shader up to the point where it casts its shadow ray.
 struct ShadeState {
-    float3 origin;        // world hit point
-    float3 geo_normal;    // geometric normal facing V
-    float3 outward;       // shading normal on V's side of the geometric normal
-    float3 V;
+    D3 origin;        // world hit point
+    D3 geo_normal;    // geometric normal facing V
+    D3 outward;       // shading normal on V's side of the geometric normal
+    D3 V;
     MatBrdf m;
     bool front;
     bool has_light;
-    float3 light_dir;
-    float light_dist;
-    float3 light_scale;   // light_weight * intensity * color
-    Rng4 rng;             // local stream after the RIS draws
-    float displacement;   // |origin - previous hit point|, for volume bookkeeping
+    D3 light_dir;
+    double light_dist;
+    D3 light_scale;   // light_weight * intensity * color
+    Rng4 rng;         // local stream after the RIS draws
+    double displacement;
 };
+
+__device__ inline D4 mix4(D4 a, D4 b, D4 c, double b0, double bu, double bv) {
+    return D4{a.x * b0 + b.x * bu + c.x * bv, a.y * b0 + b.y * bu + c.y * bv, a.z * b0 + b.z * bu + c.z * bv,
+              a.w * b0 + b.w * bu + c.w * bv};
+}
 
 // First half of RayTracing.rchit main (:136-340): material resolution and the RIS
 // light choice.  Returns false when a debug mapping or an invalid reference ended
 // the shader (ray already updated).
 template <bool kStats>
-__device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitInfo& h, float3 world_dir,
-                                   ShadeState& s, PathCounters& pc) {
+__device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitInfo& h, double ht, double bu,
+                                   double bv, D3 world_dir, ShadeState& s, PathCounters& pc) {
     const SceneView& sv = fa.sv;
     const InstanceRecord& ir = fa.tlas.inst[h.inst];
     if (kStats) pc.shade++;
     uint32_t ci = ir.custom_index;
-    ray.t = h.t;
+    ray.t = ht;
     ray.need_scatter = false;
-    if (ci >= sv.num_prim_infos) { pc.bad++; ray.emittance = mk3(0.f, 0.f, 0.f); return false; }
+    if (ci >= sv.num_prim_infos) { pc.bad++; ray.emittance = d3(0.0); return false; }
     const rt_prim_info pi = sv.prim_infos[ci];
-    if (pi.material_id >= sv.num_materials) { pc.bad++; ray.emittance = mk3(0.f, 0.f, 0.f); return false; }
+    if (pi.material_id >= sv.num_materials) { pc.bad++; ray.emittance = d3(0.0); return false; }
     const rt_material_raw& mat = sv.materials[pi.material_id];
-    float3 last_hit = ray.hit_point;
+    D3 last_hit = ray.hit_point;
     uint32_t io = pi.i_offset + 3u * h.prim;
-    if (io + 2u >= sv.num_indices) { pc.bad++; ray.emittance = mk3(0.f, 0.f, 0.f); return false; }
+    if (io + 2u >= sv.num_indices) { pc.bad++; ray.emittance = d3(0.0); return false; }
     uint32_t i0 = pi.v_offset + sv.indices[io], i1 = pi.v_offset + sv.indices[io + 1], i2 = pi.v_offset + sv.indices[io + 2];
     if (i0 >= sv.num_vertices || i1 >= sv.num_vertices || i2 >= sv.num_vertices) {
         pc.bad++;
-        ray.emittance = mk3(0.f, 0.f, 0.f);
+        ray.emittance = d3(0.0);
         return false;
     }
-    const float4* va = reinterpret_cast<const float4*>(sv.vertices + i0);
-    const float4* vb = reinterpret_cast<const float4*>(sv.vertices + i1);
-    const float4* vc = reinterpret_cast<const float4*>(sv.vertices + i2);
+    const float* va = reinterpret_cast<const float*>(sv.vertices + i0);
+    const float* vb = reinterpret_cast<const float*>(sv.vertices + i1);
+    const float* vc = reinterpret_cast<const float*>(sv.vertices + i2);
     // getMixVertexAndGeoNormal, lib/RayTracingCommons.glsl:86-101
-    const float bu = h.u, bv = h.v, b0 = 1.f - bu - bv;
-    float3 p0 = xyz(va[0]), p1 = xyz(vb[0]), p2 = xyz(vc[0]);
-    float4 ua = va[6], ub = vb[6], uc = vc[6];
-    float4 uv = make_float4(ua.x * b0 + ub.x * bu + uc.x * bv, ua.y * b0 + ub.y * bu + uc.y * bv,
-                            ua.z * b0 + ub.z * bu + uc.z * bv, ua.w * b0 + ub.w * bu + uc.w * bv);
-    float3 pos = p0 * b0 + p1 * bu + p2 * bv;
-    float4 ca = va[3], cb = vb[3], cc = vc[3];
-    float4 vcol = make_float4(ca.x * b0 + cb.x * bu + cc.x * bv, ca.y * b0 + cb.y * bu + cc.y * bv,
-                              ca.z * b0 + cb.z * bu + cc.z * bv, ca.w * b0 + cb.w * bu + cc.w * bv);
-    float3 normal = normalize3(normalize3(xyz(va[1])) * b0 + normalize3(xyz(vb[1])) * bu + normalize3(xyz(vc[1])) * bv);
-    float3 geo_normal = normalize3(o2w_dir(ir, cross3(p1 - p0, p2 - p0)));
-    float3 origin = o2w_point(ir, pos);
-    float4 color4 = mul4(vcol, ld4(mat.base_color));
+    const double b0 = 1.0 - bu - bv;
+    D3 p0 = ld3(va), p1 = ld3(vb), p2 = ld3(vc);
+    D4 uv = mix4(ld4(va + 24), ld4(vb + 24), ld4(vc + 24), b0, bu, bv);
+    D3 pos = p0 * b0 + p1 * bu + p2 * bv;
+    D4 vcol = mix4(ld4(va + 12), ld4(vb + 12), ld4(vc + 12), b0, bu, bv);
+    D3 normal = dnormalize(dnormalize(ld3(va + 4)) * b0 + dnormalize(ld3(vb + 4)) * bu + dnormalize(ld3(vc + 4)) * bv);
+    D3 geo_normal = dnormalize(o2w_dir(ir, dcross(p1 - p0, p2 - p0)));
+    D3 origin = o2w_point(ir, pos);
+    D4 color4 = mul4(vcol, ld4(mat.base_color));
     if (mat.base_color_texture.index >= 0) {
         color4 = mul4(color4, tex_uv(sv, mat.base_color_texture, uv));
         if (kStats) pc.tex++;
     }
-    float3 color = mk3(color4.x, color4.y, color4.z);
+    D3 color = xyz(color4);
     ray.hit_point = pos;
     if (mat.normal_texture.index >= 0) {  // rchit:178-185, getNormal :124-129
-        float4 tx = tex_uv(sv, mat.normal_texture, uv);
+        D4 tx = tex_uv(sv, mat.normal_texture, uv);
         if (kStats) pc.tex++;
-        float3 nt = normalize3(mk3(tx.x * 2.f - 1.f, tx.y * 2.f - 1.f, tx.z * 2.f - 1.f));
-        float4 ta = va[2], tb = vb[2], tc = vc[2];
-        float4 t4 = make_float4(ta.x * b0 + tb.x * bu + tc.x * bv, ta.y * b0 + tb.y * bu + tc.y * bv,
-                                ta.z * b0 + tb.z * bu + tc.z * bv, ta.w * b0 + tb.w * bu + tc.w * bv);
-        float tl = sqrtf(t4.x * t4.x + t4.y * t4.y + t4.z * t4.z + t4.w * t4.w);
-        t4 = make_float4(t4.x / tl, t4.y / tl, t4.z / tl, t4.w / tl);
-        float3 tm = xyz(t4);
-        float3 tangent = normalize3(tm - normal * dot3(tm, normal));
-        float3 bit = normalize3(cross3(normal, tangent) * t4.w);
+        D3 nt = dnormalize(d3(tx.x * 2.0 - 1.0, tx.y * 2.0 - 1.0, tx.z * 2.0 - 1.0));
+        D4 t4 = mix4(ld4(va + 8), ld4(vb + 8), ld4(vc + 8), b0, bu, bv);
+        double tl = sqrt(t4.x * t4.x + t4.y * t4.y + t4.z * t4.z + t4.w * t4.w);
+        t4 = D4{t4.x / tl, t4.y / tl, t4.z / tl, t4.w / tl};
+        D3 tm = xyz(t4);
+        D3 tangent = dnormalize(tm - normal * ddot(tm, normal));
+        D3 bit = dnormalize(dcross(normal, tangent) * t4.w);
         normal = tangent * nt.x + bit * nt.y + normal * nt.z;
     }
-    normal = normalize3(o2w_dir(ir, normal));
-    float3 V = -normalize3(world_dir);
-    bool front = dot3(V, geo_normal) >= 0.f;
+    normal = dnormalize(o2w_dir(ir, normal));
+    D3 V = -dnormalize(world_dir);
+    bool front = ddot(V, geo_normal) >= 0.0;
     geo_normal = front ? geo_normal : -geo_normal;
-    float3 outward = dot3(geo_normal, normal) < 0.f ? -normal : normal;
-    float3 emit = xyz(ld4(mat.emissive_factor));
+    D3 outward = ddot(geo_normal, normal) < 0.0 ? -normal : normal;
+    D3 emit = ld3(mat.emissive_factor);
     if (mat.emissive_texture.index >= 0) {
-        float4 tx = tex_uv(sv, mat.emissive_texture, uv);
+        emit = emit * xyz(tex_uv(sv, mat.emissive_texture, uv));
         if (kStats) pc.tex++;
-        emit = emit * xyz(tx);
     }
-    float metallic = mat.metallic_factor, roughness = mat.roughness_factor;
+    double metallic = mat.metallic_factor, roughness = mat.roughness_factor;
     if (mat.metallic_roughness_texture.index >= 0) {
-        float4 tx = tex_uv(sv, mat.metallic_roughness_texture, uv);
+        D4 tx = tex_uv(sv, mat.metallic_roughness_texture, uv);
         if (kStats) pc.tex++;
         roughness *= tx.y;
         metallic *= tx.z;
     }
-    float3 spec_wf = mk3(1.f, 1.f, 1.f);
+    D3 spec_wf = d3(1.0);
     const bool sg = mat.workflow == RT_WORKFLOW_SPECULAR_GLOSSINESS;
     if (sg) {  // rchit:215-239
-        float4 df = ld4(mat.sg_diffuse_factor);
-        float4 sgf = ld4(mat.sg_specular_glossiness_factor);
+        D4 df = ld4(mat.sg_diffuse_factor);
+        D4 sgf = ld4(mat.sg_specular_glossiness_factor);
         if (mat.sg_diffuse_texture.index >= 0) { df = mul4(df, tex_uv(sv, mat.sg_diffuse_texture, uv)); if (kStats) pc.tex++; }
         if (mat.sg_specular_glossiness_texture.index >= 0) {
             sgf = mul4(sgf, tex_uv(sv, mat.sg_specular_glossiness_texture, uv));
             if (kStats) pc.tex++;
         }
         spec_wf = xyz(sgf);
-        roughness = 1.f - sgf.w;
-        float mx = fmaxf(fmaxf(spec_wf.x, spec_wf.y), spec_wf.z);
+        roughness = 1.0 - sgf.w;
+        double mx = fmax(fmax(spec_wf.x, spec_wf.y), spec_wf.z);
         color = xyz(vcol) * xyz(df);
         metallic = convert_metallic(color, spec_wf, mx);
     }
-    float trans = 0.f;
+    double trans = 0.0;
     if (mat.transmission_exist) {
         trans = mat.transmission_factor;
         if (mat.transmission_texture.index >= 0) {
@@ -245,27 +286,25 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
     uint32_t mapping = mat.unlit ? uint32_t(RT_MAPPING_ALBEDO) : fa.mapping;
     if (mapping != RT_MAPPING_RENDER && mapping != RT_MAPPING_HEAT && mapping != RT_MAPPING_DISTANCE) {
         // debug mappings, rchit:258-286 (hitValue keeps its previous value, needScatter is false)
-        float3 e = mk3(0.f, 0.f, 0.f);
+        D3 e = d3(0.0);
         switch (mapping) {
             case RT_MAPPING_ALBEDO: e = color; break;
-            case RT_MAPPING_TRIANGLE: e = mk3(b0, bu, bv); break;
-            case RT_MAPPING_METALLIC: e = mk3(metallic, metallic, metallic); break;
-            case RT_MAPPING_ROUGHNESS: e = mk3(roughness, roughness, roughness); break;
-            case RT_MAPPING_NORMAL: e = (outward + mk3(1.f, 1.f, 1.f)) / 2.f; break;
-            case RT_MAPPING_TRANSMISSION: e = mk3(trans, trans, trans); break;
+            case RT_MAPPING_TRIANGLE: e = d3(b0, bu, bv); break;
+            case RT_MAPPING_METALLIC: e = d3(metallic); break;
+            case RT_MAPPING_ROUGHNESS: e = d3(roughness); break;
+            case RT_MAPPING_NORMAL: e = (outward + d3(1.0)) / 2.0; break;
+            case RT_MAPPING_TRANSMISSION: e = d3(trans); break;
             case RT_MAPPING_INSTANCE:
             case RT_MAPPING_GEO_ID: {
                 uint32_t k = mapping == RT_MAPPING_INSTANCE ? ir.instance_id : ci;
                 k += k << 10; k ^= k >> 6; k += k << 3; k ^= k >> 11; k += k << 15;  // jenkinsHash
-                e = mk3(float(k & 255u) / 255.f, float((k >> 8) & 255u) / 255.f, float((k >> 16) & 255u) / 255.f);
+                e = d3((k & 255u) / 255.0, ((k >> 8) & 255u) / 255.0, ((k >> 16) & 255u) / 255.0);
                 break;
             }
             case RT_MAPPING_TANGENT: {
-                float4 ta = va[2], tb = vb[2], tc = vc[2];
-                float4 t4 = make_float4(ta.x * b0 + tb.x * bu + tc.x * bv, ta.y * b0 + tb.y * bu + tc.y * bv,
-                                        ta.z * b0 + tb.z * bu + tc.z * bv, ta.w * b0 + tb.w * bu + tc.w * bv);
-                float tl = sqrtf(t4.x * t4.x + t4.y * t4.y + t4.z * t4.z + t4.w * t4.w);
-                e = (normalize3(o2w_dir(ir, xyz(t4) / tl)) + mk3(1.f, 1.f, 1.f)) / 2.f;
+                D4 t4 = mix4(ld4(va + 8), ld4(vb + 8), ld4(vc + 8), b0, bu, bv);
+                double tl = sqrt(t4.x * t4.x + t4.y * t4.y + t4.z * t4.z + t4.w * t4.w);
+                e = (dnormalize(o2w_dir(ir, xyz(t4) / tl)) + d3(1.0)) / 2.0;
                 break;
             }
             default: break;
@@ -273,34 +312,34 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
         ray.emittance = e;
         return false;
     }
-    float spec_factor = mat.specular_factor;
-    float3 spec_color = xyz(ld4(mat.specular_color_factor));
+    double spec_factor = mat.specular_factor;
+    D3 spec_color = ld3(mat.specular_color_factor);
     if (mat.specular_texture.index >= 0) { spec_factor *= tex_uv(sv, mat.specular_texture, uv).w; if (kStats) pc.tex++; }
     if (mat.specular_color_texture.index >= 0) {
         spec_color = spec_color * xyz(tex_uv(sv, mat.specular_color_texture, uv));
         if (kStats) pc.tex++;
     }
     ray.hit_point = origin;
-    ray.emittance = emit * fa.exposure;
+    ray.emittance = emit * double(fa.exposure);
     MatBrdf& m = s.m;
     m.base_color = color;
     m.metallic = metallic;
     m.roughness = roughness;
-    m.ior = mat.volume_exists ? mat.ior : 1.f;
+    m.ior = mat.volume_exists ? double(mat.ior) : 1.0;
     m.transmission = trans;
     m.specular_factor = spec_factor;
     m.specular_color_factor = spec_color;
     m.front_face = front;
     mat_build(m);
     if (sg) {
-        float mx = fmaxf(fmaxf(spec_wf.x, spec_wf.y), spec_wf.z);
-        m.c_diff = color * (1.f - mx);
+        double mx = fmax(fmax(spec_wf.x, spec_wf.y), spec_wf.z);
+        m.c_diff = color * (1.0 - mx);
         m.F0 = spec_wf;
     }
-    m.attenuation_color = mk3(mat.attenuation_color[0], mat.attenuation_color[1], mat.attenuation_color[2]);
+    m.attenuation_color = ld3(mat.attenuation_color);
     m.attenuation_distance = mat.attenuation_distance;
     m.volume = mat.volume_exists != 0;
-    float displacement = length3(origin - last_hit);
+    s.displacement = dlength(origin - last_hit);
     s.origin = origin;
     s.geo_normal = geo_normal;
     s.outward = outward;
@@ -311,40 +350,39 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
     s.has_light = false;
     uint32_t num = sv.num_plights;
     if (num > 0) {
-        float total = 0.f, sel_pdf = 0.f;
+        double total = 0.0, sel_pdf = 0.0;
         uint32_t sel = 0;
         uint32_t cand = num < 3u ? num : 3u;
         for (uint32_t i = 0; i < cand; i++) {
             const rt_light& li = sv.plights[i];
             if (kStats) pc.lights++;
-            if (luminance(mk3(li.color[0], li.color[1], li.color[2]) * li.intensity) < 0.1f) continue;
-            uint32_t idx = min(num - 1u, uint32_t(rng_next(rng) * float(num)));
+            if (luminance(ld3(li.color) * double(li.intensity)) < 0.1) continue;
+            uint32_t idx = min(num - 1u, uint32_t(double(rng_next(rng)) * double(num)));
             const rt_light& cl = sv.plights[idx];
             if (kStats) pc.lights++;
-            float3 lv = mk3(cl.transform[0], cl.transform[1], cl.transform[2]) - origin;
-            float ld = length3(lv);
-            float3 L = lv / ld;
-            if (dot3(geo_normal, L) < 0.00001f) continue;
-            float pdf = luminance(light_intensity_at(cl, ld));
-            float ris = pdf * float(num);
+            D3 lv = ld3(cl.transform) - origin;
+            double ld = dlength(lv);
+            D3 L = dnormalize(lv);
+            if (ddot(geo_normal, L) < 0.00001) continue;
+            double pdf = luminance(light_intensity_at(cl, ld));
+            double ris = pdf * double(num);
             total += ris;
-            if (rng_next(rng) < ris / total) {
+            if (double(rng_next(rng)) < ris / total) {
                 sel = idx;
                 sel_pdf = pdf;
             }
         }
-        if (total != 0.f) {
+        if (total != 0.0) {
             const rt_light& l = sv.plights[sel];
-            float w = (total / 3.f) / sel_pdf;
-            float3 lv = mk3(l.transform[0], l.transform[1], l.transform[2]) - origin;
-            s.light_dist = length3(lv);
-            s.light_dir = normalize3(lv);
-            s.light_scale = mk3(l.color[0], l.color[1], l.color[2]) * (w * l.intensity);
+            double w = (total / 3.0) / sel_pdf;
+            D3 lv = ld3(l.transform) - origin;
+            s.light_dist = dlength(lv);
+            s.light_dir = dnormalize(lv);
+            s.light_scale = ld3(l.color) * (w * double(l.intensity));
             s.has_light = true;
         }
     }
     s.rng = rng;
-    s.displacement = displacement;
     return true;
 }
 
@@ -356,34 +394,34 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s, bool unoccluded) {
         ray.emittance = ray.emittance + eval_combined_brdf(s.outward, s.light_dir, s.V, m) * s.light_scale;
     Rng4 rng = s.rng;
     uint32_t type;
-    float3 thr = mk3(1.f, 1.f, 1.f);
-    if (m.metallic == 1.f && m.roughness == 0.f) {
+    D3 thr = d3(1.0);
+    if (m.metallic == 1.0 && m.roughness == 0.0) {
         type = kSpecular;
     } else {
-        float3 p = brdf_probability(m, s.V, s.outward);
-        float r = rng_next(rng);
+        D3 p = brdf_probability(m, s.V, s.outward);
+        double r = rng_next(rng);
         if (r < p.x) {
             type = kSpecular;
             thr = thr / p.x;
-            if (ray.volume_dis >= 0.f) ray.volume_dis += s.displacement;
+            if (ray.volume_dis >= 0.0) ray.volume_dis += s.displacement;
         } else if (r >= p.x && r <= p.x + p.y) {
             type = kDiffuse;
             thr = thr / p.y;
-            if (ray.volume_dis >= 0.f) ray.volume_dis += s.displacement;
+            if (ray.volume_dis >= 0.0) ray.volume_dis += s.displacement;
         } else {
             type = kTransmission;
             if (m.volume) {
-                if (ray.volume_dis >= 0.f) ray.volume_dis += s.displacement;
-                else if (s.front) ray.volume_dis = 0.f;
+                if (ray.volume_dis >= 0.0) ray.volume_dis += s.displacement;
+                else if (s.front) ray.volume_dis = 0.0;
             }
             thr = thr / p.z;
         }
     }
-    float3 origin = s.origin;
+    D3 origin = s.origin;
     if (type == kTransmission) origin = offset_ray(origin, -s.geo_normal);
-    float ux = rng_next(rng);
-    float uy = rng_next(rng);
-    float3 dir = mk3(0.f, 0.f, 0.f), w = mk3(0.f, 0.f, 0.f);
+    double ux = rng_next(rng);
+    double uy = rng_next(rng);
+    D3 dir = d3(0.0), w = d3(0.0);
     ray.need_scatter = sample_indirect(ux, uy, s.outward, s.geo_normal, s.V, m, type, dir, w, ray.volume_dis);
     ray.hit_point = origin;
     ray.scatter_dir = dir;
@@ -391,65 +429,72 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s, bool unoccluded) {
     ray.rng = rng;
 }
 
+// The closest-hit shader for a found hit, including its shadow ray.
+template <bool kAnyHit, bool kStats>
+__device__ inline void closest_hit(const FrameArgs& fa, Payload& ray, const HitInfo& h, D3 o, D3 d, uint32_t* stack,
+                                   uint32_t stride, PathCounters& pc) {
+    double ht, bu, bv;
+    refine_hit(fa.tlas, h, o, d, ht, bu, bv);
+    ShadeState ss;
+    if (!shade_begin<kStats>(fa, ray, h, ht, bu, bv, d, ss, pc)) return;
+    bool unocc = false;
+    if (ss.has_light) {
+        if (kStats) pc.shadow++;
+        const uint32_t shadow_flags = RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RT_RAY_FLAG_SKIP_CLOSEST_HIT |
+                                      (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
+        Rng4 shadow_rng = ray.rng;  // copy of Ray.rngState, discarded at rchit:476
+        AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
+        HitInfo sh;
+        unocc = !trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(ss.origin), to_f3(ss.light_dir), float(kShadowTMin),
+                                           float(ss.light_dist), shadow_flags, sah, stack, stride, sh, pc.tr);
+    }
+    shade_end(ray, ss, unocc);
+}
+
 // RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance.
 template <bool kAnyHit, bool kStats>
-__device__ inline float3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, uint32_t stride,
-                                      PathCounters& pc) {
+__device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, uint32_t stride,
+                                  PathCounters& pc) {
     Rng4 rng_a{x, y, fa.frame_count, 0u};
     Payload ray;
     ray.rng = Rng4{x, y, fa.seed_b, 0u};
-    ray.hit_point = mk3(0.f, 0.f, 0.f);
-    ray.hit_value = mk3(0.f, 0.f, 0.f);
-    ray.emittance = mk3(0.f, 0.f, 0.f);
+    ray.hit_point = d3(0.0);
+    ray.hit_value = d3(0.0);
+    ray.emittance = d3(0.0);
     ray.need_scatter = false;
-    ray.volume_dis = -1.f;
+    ray.volume_dis = -1.0;
     uint32_t lcg = tea16(tea16(x, y), fa.seed_b);
-    float3 radiance = mk3(0.f, 0.f, 0.f);
+    D3 radiance = d3(0.0);
     const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-    const uint32_t shadow_flags = RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RT_RAY_FLAG_SKIP_CLOSEST_HIT | flags;
     for (uint32_t s = 0; s < fa.samples; s++) {
-        float3 o, d;
-        float tfar;
+        D3 o, d;
+        double tfar;
         camera_ray(fa, x, y, rng_a, lcg, o, d, tfar);
-        float3 thr = mk3(1.f, 1.f, 1.f);
-        ray.t = 0.f;
-        ray.volume_dis = -1.f;
+        D3 thr = d3(1.0);
+        ray.t = 0.0;
+        ray.volume_dis = -1.0;
         for (uint32_t b = 0; b < fa.bounces; b++) {
             if (kStats) { if (b == 0) pc.primary++; else pc.extension++; }
             AlphaAnyHit ah{&fa.sv, &fa.tlas, &ray.rng};
             HitInfo h;
-            bool hit = trace_ray<kAnyHit, kStats>(fa.tlas, o, d, kTMin, tfar, flags, ah, stack, stride, h, pc.tr);
-            if (hit) {
-                ShadeState ss;
-                if (shade_begin<kStats>(fa, ray, h, d, ss, pc)) {
-                    bool unocc = false;
-                    if (ss.has_light) {
-                        if (kStats) pc.shadow++;
-                        Rng4 shadow_rng = ray.rng;  // copy of Ray.rngState, discarded at rchit:476
-                        AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
-                        HitInfo sh;
-                        unocc = !trace_ray<kAnyHit, kStats>(fa.tlas, ss.origin, ss.light_dir, kShadowTMin, ss.light_dist,
-                                                           shadow_flags, sah, stack, stride, sh, pc.tr);
-                    }
-                    shade_end(ray, ss, unocc);
-                }
-            } else {
-                miss_shader(fa.sv, ray, d);
-            }
-            float3 hit_color = ray.hit_value;
-            float t = ray.t;
+            bool hit = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(o), to_f3(d), float(kTMin), float(tfar), flags, ah,
+                                                  stack, stride, h, pc.tr);
+            if (hit) closest_hit<kAnyHit, kStats>(fa, ray, h, o, d, stack, stride, pc);
+            else miss_shader(fa.sv, ray, d);
+            D3 hit_color = ray.hit_value;
+            double t = ray.t;
             bool scattered = ray.need_scatter;
             radiance = radiance + thr * ray.emittance;
-            ray.emittance = mk3(0.f, 0.f, 0.f);
+            ray.emittance = d3(0.0);
             if (b + 1 == fa.bounces) break;
             if (b > kMinBounces) {  // Russian roulette, rgen:109-118
-                float rr = clampf(luminance(thr), 0.01f, 0.95f);
-                float prop = rng_next(rng_a);
+                double rr = clampd(luminance(thr), 0.01, 0.95);
+                double prop = rng_next(rng_a);
                 if (rr < prop) break;
                 thr = thr / rr;
             }
             thr = thr * hit_color;
-            if (!scattered || t < 0.f) break;
+            if (!scattered || t < 0.0) break;
             o = ray.hit_point;
             d = ray.scatter_dir;
         }

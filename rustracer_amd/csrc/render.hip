@@ -34,10 +34,10 @@ __global__ __launch_bounds__(kMegaBlock) void k_megakernel(FrameArgs fa) {
     const uint32_t y = ty * 8 + (lane >> 3);
     PathCounters pc;
     if (x < fa.width && y < fa.height) {
-        float3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, kMegaBlock, pc);
+        D3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, kMegaBlock, pc);
         float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
         float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
-        *acc = make_float4(prev.x + radiance.x, prev.y + radiance.y, prev.z + radiance.z, 0.f);
+        *acc = make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
     }
     if (kStats) flush_counters(fa.counters, pc);
 }
@@ -83,15 +83,15 @@ __global__ __launch_bounds__(kMegaBlock) void k_primary(FrameArgs fa, rt_hit* hi
     Rng4 rng_a{x, y, fa.frame_count, 0u};
     Rng4 rng_b{x, y, fa.seed_b, 0u};
     uint32_t lcg = tea16(tea16(x, y), fa.seed_b);
-    float3 o, d;
-    float tfar;
+    D3 o, d;
+    double tfar;
     camera_ray(fa, x, y, rng_a, lcg, o, d, tfar);
     AlphaAnyHit ah{&fa.sv, &fa.tlas, &rng_b};
     HitInfo h;
     TraceStats st;
     uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-    bool found = trace_ray<kAnyHit, false>(fa.tlas, o, d, kTMin, tfar, flags, ah, s_stack + threadIdx.x, kMegaBlock,
-                                          h, st);
+    bool found = trace_ray<kAnyHit, false>(fa.tlas, to_f3(o), to_f3(d), float(kTMin), float(tfar), flags, ah,
+                                          s_stack + threadIdx.x, kMegaBlock, h, st);
     rt_hit out;
     out._pad[0] = out._pad[1] = 0;
     if (found) {

@@ -31,6 +31,7 @@ struct HitInfo {
     float t, u, v;
     uint32_t inst;  // index into TlasView::inst
     uint32_t prim;  // gl_PrimitiveID
+    uint32_t rec;   // triangle record index in the BLAS pool
 };
 
 // Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2013).
@@ -55,7 +56,11 @@ __device__ inline WoopRay woop_setup(float3 o, float3 d) {
     return w;
 }
 
-// Returns true and (t, u, v) for a hit with tmin <= t <= tmax.
+// Returns true and (t, u, v) for a hit with tmin <= t <= tmax.  The edge functions
+// are evaluated without FMA contraction: the test is watertight only if a shared
+// edge's function evaluates to exactly the negated value in the neighbouring
+// triangle, which a fused a*b - c*d does not guarantee.  (The sheared vertex
+// coordinates may be fused: a shared vertex gets the same value in both triangles.)
 __device__ inline bool woop_hit(const WoopRay& w, float3 v0, float3 v1, float3 v2, float tmin, float tmax, float& t,
                                 float& bu, float& bv) {
     float3 A = v0 - w.o, B = v1 - w.o, C = v2 - w.o;
@@ -63,9 +68,13 @@ __device__ inline bool woop_hit(const WoopRay& w, float3 v0, float3 v1, float3 v
     float Ax = comp(A, w.kx) - w.sx * Akz, Ay = comp(A, w.ky) - w.sy * Akz;
     float Bx = comp(B, w.kx) - w.sx * Bkz, By = comp(B, w.ky) - w.sy * Bkz;
     float Cx = comp(C, w.kx) - w.sx * Ckz, Cy = comp(C, w.ky) - w.sy * Ckz;
-    float U = Cx * By - Cy * Bx;
-    float V = Ax * Cy - Ay * Cx;
-    float W = Bx * Ay - By * Ax;
+    float U, V, W;
+    {
+#pragma clang fp contract(off)
+        U = Cx * By - Cy * Bx;
+        V = Ax * Cy - Ay * Cx;
+        W = Bx * Ay - By * Ax;
+    }
     if (U == 0.f || V == 0.f || W == 0.f) {
         U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
         V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
@@ -234,6 +243,7 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
                 hit.v = bv;
                 hit.inst = inst_idx;
                 hit.prim = prim;
+                hit.rec = first + k;
                 if (terminate) return true;
             }
             continue;

@@ -1,0 +1,144 @@
+"""Traversal parity: the gfx950 traversal (fp32 watertight test) against the FP64
+oracle's brute force on the same fp32 rays.  Closest-hit (instance, primitive) ids
+must be identical; a mismatch is accepted only when it is a documented sub-ulp tie
+(the ray passes within 2e-5 barycentric units of an edge of both candidates, or both
+t values agree to 1e-5).  Restates what traceRayEXT returns to
+crates/examples/gltf_viewer/shaders/RayTracing.rchit (gl_InstanceID,
+gl_PrimitiveID, gl_HitTEXT, barycentrics)."""
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from rustracer_amd import layouts as L, rt
+from tests import gpu_kit as G
+from tests import scene_kit as K
+
+pytestmark = pytest.mark.gpu
+MISS = G.MISS
+
+
+def _queries(ctx, ds, rays, flags=rt.RAY_FLAG_OPAQUE, bindings=False):
+    if bindings:
+        return rt.trace_ray_queries(ctx, ds.tlas, rays, flags, buffers=ds.buffers, ubo=ds.scene.ubo)
+    return rt.trace_ray_queries(ctx, ds.tlas, rays, flags)
+
+
+def _assert_parity(scene, rays, gh, oh, otuv, max_tie_frac=1e-3):
+    same = (gh["instance_id"] == oh["instance_id"]) & (gh["primitive_id"] == oh["primitive_id"])
+    bad = G.explain_mismatches(scene, rays["origin"], rays["direction"], gh, oh, otuv[:, 0])
+    assert len(bad) == 0, f"{len(bad)} unexplained id mismatches, e.g. ray {bad[:3]}"
+    assert (~same).mean() <= max_tie_frac
+    hit = same & (oh["primitive_id"] != MISS)
+    # t and barycentrics agree to fp32 accuracy (grazing hits are ill-conditioned)
+    rel = np.abs(gh["t"][hit] - otuv[hit, 0]) / np.maximum(otuv[hit, 0], 1e-3)
+    assert np.percentile(rel, 99.9) < 1e-5 and rel.max() < 1e-3
+    assert np.abs(gh["u"][hit] - otuv[hit, 1]).max() < 1e-3 and np.abs(gh["v"][hit] - otuv[hit, 2]).max() < 1e-3
+
+
+def test_random_rays_cornell(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    rays = K.random_rays(50000, [-0.95, 0.05, -0.95], [0.95, 1.95, 3.0], seed=11)
+    gh = _queries(gpu_ctx, ds, rays)
+    oh, otuv = B.OracleScene(ds.scene).trace_rays(rays, brute=True)
+    _assert_parity(ds.scene, rays, gh, oh, otuv)
+
+
+def test_random_rays_instanced_affine(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "instanced", lambda: K.instanced_spheres(n_inst=60, seed=3, level=2))
+    rays = K.random_rays(50000, [-8, -8, -8], [8, 8, 8], seed=12, toward=(0, 0, 0))
+    gh = _queries(gpu_ctx, ds, rays)
+    oh, otuv = B.OracleScene(ds.scene).trace_rays(rays, brute=True)
+    assert (oh["primitive_id"] != MISS).mean() > 0.2
+    _assert_parity(ds.scene, rays, gh, oh, otuv)
+
+
+def test_random_rays_displaced_sphere(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "displaced_sphere")
+    rays = K.random_rays(30000, [-3, -3, -3], [3, 3, 3], seed=13, toward=(0, 0, 0))
+    gh = _queries(gpu_ctx, ds, rays)
+    oh, otuv = B.OracleScene(ds.scene).trace_rays(rays)
+    _assert_parity(ds.scene, rays, gh, oh, otuv)
+
+
+def test_shadow_ray_semantics(gpu_ctx):
+    # terminate-on-first-hit returns some hit within [tmin, tmax] exactly when one exists
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    rays = K.random_rays(20000, [-0.9, 0.1, -0.9], [0.9, 1.9, 0.9], seed=14, tmin=0.1, tmax=1.5)
+    flags = rt.RAY_FLAG_OPAQUE | rt.RAY_FLAG_TERMINATE_ON_FIRST_HIT | rt.RAY_FLAG_SKIP_CLOSEST_HIT
+    gh = _queries(gpu_ctx, ds, rays, flags)
+    oh, otuv = B.OracleScene(ds.scene).trace_rays(rays, brute=True)
+    occluded_g = gh["primitive_id"] != MISS
+    occluded_o = oh["primitive_id"] != MISS
+    assert (occluded_g != occluded_o).mean() < 1e-3
+    t = gh["t"][occluded_g]
+    assert t.min() >= 0.1 and t.max() <= 1.5
+
+
+def test_primary_ids_cornell(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    sc = ds.scene
+    gh = rt.trace_primary(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, sc.width, sc.height)
+    oh, otuv = B.OracleScene(sc).trace_primary()
+    same = (gh["instance_id"] == oh["instance_id"]) & (gh["primitive_id"] == oh["primitive_id"])
+    assert (~same).sum() <= 3, np.nonzero(~same)[0][:10]
+
+
+def test_primary_ids_displaced_sphere(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "displaced_sphere")
+    sc = ds.scene
+    gh = rt.trace_primary(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, sc.width, sc.height)
+    oh, otuv = B.OracleScene(sc).trace_primary()
+    same = (gh["instance_id"] == oh["instance_id"]) & (gh["primitive_id"] == oh["primitive_id"])
+    # camera rays are computed in fp32 on the GPU and fp64 in the oracle: only
+    # near-edge pixels may differ
+    assert (~same).mean() < 2e-4, (~same).sum()
+    hit = same & (oh["primitive_id"] != MISS)
+    assert hit.sum() > 0.25 * len(hit)
+
+
+def test_alpha_mask_any_hit(gpu_ctx):
+    # MASK material with per-vertex alpha: candidates with alpha < cutoff are ignored
+    def make():
+        sc = K.triangle_soup(400, seed=21, opaque=False)
+        rng = np.random.default_rng(22)
+        sc.vertices["color"][:, 3] = rng.random(len(sc.vertices))
+        sc.ubo["fully_opaque"] = 0
+        return sc
+    ds = G.device_scene(gpu_ctx, "mask_soup", make)
+    rays = K.random_rays(20000, [-3, -3, -3], [3, 3, 3], seed=23, toward=(0, 0, 0))
+    gh = _queries(gpu_ctx, ds, rays, flags=rt.RAY_FLAG_NONE, bindings=True)
+    orc = B.OracleScene(ds.scene)
+    oh, otuv = orc.trace_rays(rays, flags=0, brute=True)
+    _assert_parity(ds.scene, rays, gh, oh, otuv)
+    # the alpha test removed hits: opaque traversal finds more
+    op, _ = orc.trace_rays(rays, flags=1, brute=True)
+    assert (op["primitive_id"] != MISS).sum() > (oh["primitive_id"] != MISS).sum()
+
+
+def test_closed_mesh_is_watertight_on_gpu(gpu_ctx):
+    """Rays from inside a closed geodesic sphere at every vertex and every edge
+    midpoint must hit (the fp32 test is evaluated without FMA contraction so shared
+    edges are exactly antisymmetric)."""
+    from rustracer_amd import scenes as S
+
+    def make():
+        b = S.Builder()
+        mid = b.add_material(L.default_material())
+        sp, st = S.geodesic_sphere(12)
+        g = b.add_primitive(S.make_vertices(sp, sp), st.reshape(-1), mid)
+        return b.finish("closed", S.make_instances([(None, g)]), [], [], K._ubo(8, 8), 8, 8)
+    ds = G.device_scene(gpu_ctx, "closed_sphere", make)
+    sc = ds.scene
+    pos = sc.vertices["pos"][:, :3].astype(np.float64)
+    tri = sc.indices.reshape(-1, 3).astype(np.int64)
+    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    targets = np.concatenate([pos, 0.5 * (pos[e[:, 0]] + pos[e[:, 1]])])
+    for origin in ((0, 0, 0), (0.1, -0.2, 0.05), (0.0, 0.3, 0.0)):
+        r = np.zeros(len(targets), L.RAY)
+        r["origin"] = origin
+        d = targets - np.array(origin)
+        r["direction"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        r["tmin"] = 0.0
+        r["tmax"] = 10.0
+        gh = _queries(gpu_ctx, ds, r)
+        assert np.all(gh["primitive_id"] != MISS), (gh["primitive_id"] == MISS).sum()

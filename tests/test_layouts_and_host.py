@@ -1,0 +1,144 @@
+"""Host logic: buffer layouts (numpy dtypes vs the C headers, compiled and measured
+with gcc), the seeded scene generators, camera/UBO conventions and instance records."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from rustracer_amd import layouts as L
+from rustracer_amd import scenes as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "rt/rt.h"
+#define F(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+#define Z(T) printf(#T " %zu\n", sizeof(T));
+int main(void) {
+  Z(rt_vertex) Z(rt_prim_info) Z(rt_material_raw) Z(rt_light) Z(rt_ubo) Z(rt_instance)
+  Z(rt_blas_input) Z(rt_texture) Z(rt_bindings) Z(rt_trace_stats) Z(rt_ray) Z(rt_hit) Z(rt_bvh_info)
+  F(rt_material_raw, metallic_roughness_texture) F(rt_material_raw, transmission_exist)
+  F(rt_material_raw, attenuation_distance) F(rt_material_raw, specular_color_factor)
+  F(rt_material_raw, sg_specular_glossiness_factor) F(rt_material_raw, sg_specular_glossiness_texture)
+  F(rt_ubo, projection_inverse) F(rt_ubo, exposure) F(rt_bindings, num_materials)
+  F(rt_trace_stats, kernel_ms) F(rt_hit, custom_index)
+  return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def c_layout():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "p.c")
+        exe = os.path.join(d, "p")
+        open(src, "w").write(PROBE)
+        subprocess.check_call(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), src, "-o", exe])
+        out = subprocess.check_output([exe], text=True)
+    return {k: int(v) for k, v in (line.split() for line in out.splitlines())}
+
+
+def test_numpy_dtypes_match_c_headers(c_layout):
+    for name, dt in [("rt_vertex", L.VERTEX), ("rt_prim_info", L.PRIM_INFO), ("rt_material_raw", L.MATERIAL),
+                     ("rt_light", L.LIGHT), ("rt_ubo", L.UBO), ("rt_instance", L.INSTANCE), ("rt_ray", L.RAY),
+                     ("rt_hit", L.HIT)]:
+        assert c_layout[name] == dt.itemsize, name
+    for f in ("metallic_roughness_texture", "transmission_exist", "attenuation_distance", "specular_color_factor",
+              "sg_specular_glossiness_factor", "sg_specular_glossiness_texture"):
+        assert c_layout[f"rt_material_raw.{f}"] == L.MATERIAL.fields[f][1], f
+    assert c_layout["rt_ubo.projection_inverse"] == L.UBO.fields["projection_inverse"][1]
+    assert c_layout["rt_ubo.exposure"] == 316
+    assert c_layout["rt_hit.custom_index"] == L.HIT.fields["custom_index"][1]
+
+
+def test_ctypes_structs_match_c_headers(c_layout):
+    import ctypes as C
+    from rustracer_amd import rt
+    for name, st in [("rt_blas_input", rt.BlasInput), ("rt_texture", rt.Texture), ("rt_bindings", rt.Bindings),
+                     ("rt_trace_stats", rt.TraceStats), ("rt_bvh_info", rt.BvhInfo)]:
+        assert c_layout[name] == C.sizeof(st), name
+    assert c_layout["rt_bindings.num_materials"] == rt.Bindings.num_materials.offset
+    assert c_layout["rt_trace_stats.kernel_ms"] == rt.TraceStats.kernel_ms.offset
+
+
+def test_reference_layout_sizes(c_layout):
+    # crates/libs/asset_loader/src/{geometry,material,light}.rs and ubo.rs
+    assert (c_layout["rt_vertex"], c_layout["rt_prim_info"], c_layout["rt_material_raw"], c_layout["rt_light"],
+            c_layout["rt_ubo"], c_layout["rt_instance"]) == (128, 16, 256, 48, 324, 64)
+
+
+def test_geodesic_sphere_is_a_closed_manifold():
+    for f in (1, 3, 16):
+        pos, tri = S.geodesic_sphere(f)
+        assert len(tri) == 20 * f * f and len(pos) == 10 * f * f + 2
+        e = np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1)
+        _, counts = np.unique(e, axis=0, return_counts=True)
+        assert np.all(counts == 2)  # every edge shared by exactly two triangles
+        assert len(pos) - len(counts) + len(tri) == 2  # Euler characteristic of a sphere
+        assert np.allclose(np.linalg.norm(pos, axis=1), 1)
+        # counter-clockwise seen from outside
+        n = np.cross(pos[tri[:, 1]] - pos[tri[:, 0]], pos[tri[:, 2]] - pos[tri[:, 0]])
+        assert np.all(np.einsum("ij,ij->i", n, pos[tri].mean(1)) > 0)
+
+
+def test_workloads_match_their_specification():
+    c = S.cornell_sphere()
+    assert c.num_triangles == 10 + 5120 and (c.width, c.height) == (256, 256)
+    assert c.ubo["number_of_samples"] == 4 and c.ubo["number_of_bounces"] == 4 and c.ubo["random_seed"] == 0
+    assert len(c.plights) == 1 and c.plights["intensity"][0] == 50
+    assert np.all(c.materials["metallic_factor"][:3] == 0) and np.all(c.materials["roughness_factor"][:3] == 1)
+    assert c.materials["metallic_factor"][3] == 1 and np.isclose(c.materials["roughness_factor"][3], 0.3)
+    assert c.fully_opaque() and c.ubo["fully_opaque"] == 1
+
+
+def test_displaced_sphere_small_frequency():
+    d = S.displaced_sphere(width=64, height=36, spp=2, bounces=8, freq=20, group=1024)
+    assert d.num_triangles == 20 * 400 and len(d.geometries) == 8
+    assert np.all(d.geometries["prim_count"][:-1] == 1024)
+    r = np.linalg.norm(d.vertices["pos"][:, :3], axis=1)
+    assert r.min() >= 0.95 - 1e-6 and r.max() <= 1.05 + 1e-6
+    rough = d.materials["roughness_factor"]
+    assert rough.min() >= 0.05 and rough.max() <= 1.0
+    assert len(d.plights) == 3
+    # deterministic in the seed
+    d2 = S.displaced_sphere(width=64, height=36, spp=2, bounces=8, freq=20, group=1024)
+    assert np.array_equal(d.vertices, d2.vertices) and np.array_equal(d.indices, d2.indices)
+
+
+def test_camera_conventions():
+    # look_at_rh maps the eye to the origin and the view direction to -z
+    v = S.look_at_rh((1, 2, 3), (1, 2, 0), (0, 1, 0))
+    assert np.allclose(v @ [1, 2, 3, 1], [0, 0, 0, 1])
+    assert np.allclose(v[:3, :3] @ [0, 0, -1], [0, 0, -1])
+    # the Vulkan-RT projection flips y and maps depth to [0, 1]
+    p = S.perspective_vk(1.0, 90.0, 0.1, 10.0)
+    ndc = p @ [0, 1, -1, 1]
+    assert ndc[1] / ndc[3] < 0
+    z_near, z_far = p @ [0, 0, -0.1, 1], p @ [0, 0, -10, 1]
+    assert np.isclose(z_near[2] / z_near[3], 0) and np.isclose(z_far[2] / z_far[3], 1)
+    # the UBO stores column-major matrices and their inverses
+    u = S.make_ubo(v, p, samples=1, bounces=1)
+    mv = u["model_view"].reshape(4, 4).T
+    mvi = u["model_view_inverse"].reshape(4, 4).T
+    assert np.allclose(mv @ mvi, np.eye(4), atol=1e-6)
+
+
+def test_instance_records():
+    m = np.eye(4)
+    m[:3, 3] = (1, 2, 3)
+    inst = S.make_instances([(m, 5), (None, 7)])
+    assert np.array_equal(inst["transform"][0], m[:3, :4].astype(np.float32))
+    assert inst["custom_index_and_mask"][0] == (5 | 0xFF << 24)
+    assert inst["sbt_offset_and_flags"][1] >> 24 == L.INSTANCE_TRIANGLE_FACING_CULL_DISABLE
+    assert list(inst["acceleration_structure_reference"]) == [5, 7]
+
+
+def test_default_material_matches_gltf_defaults():
+    m = L.default_material()
+    assert m["alpha_mode"] == L.ALPHA_OPAQUE and m["alpha_cutoff"] == np.float32(0.5)
+    assert m["ior"] == np.float32(1.5) and m["specular_factor"] == 1 and np.all(m["specular_color_factor"] == 1)
+    assert m["base_color_texture"]["index"] == -1 and m["attenuation_distance"] == np.finfo(np.float32).max
