@@ -19,7 +19,7 @@ PUBLIC = [os.path.join(ROOT, "include", "rt", "rt.h"), os.path.join(ROOT, "inclu
 
 # Packed fp32 instructions are disabled: hipcc would otherwise form them from
 # scalar float code, and some of their forms have given wrong results here.
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",
          "-I", os.path.join(ROOT, "include")]
 

@@ -386,12 +386,12 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
     return true;
 }
 
-// Second half of RayTracing.rchit main (:346-477) after the shadow ray: NEE
-// contribution, lobe choice and the next direction.
-__device__ inline void shade_end(Payload& ray, ShadeState& s, bool unoccluded) {
+// Second half of RayTracing.rchit main (:352-477): lobe choice and the next
+// direction.  The NEE term (rchit:346-349) is evaluated by shade_bounce, because
+// whether it is added depends on the shadow ray, which the wavefront form traces
+// in a separate kernel.
+__device__ inline void shade_end(Payload& ray, ShadeState& s) {
     const MatBrdf& m = s.m;
-    if (s.has_light && unoccluded)
-        ray.emittance = ray.emittance + eval_combined_brdf(s.outward, s.light_dir, s.V, m) * s.light_scale;
     Rng4 rng = s.rng;
     uint32_t type;
     D3 thr = d3(1.0);
@@ -429,77 +429,148 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s, bool unoccluded) {
     ray.rng = rng;
 }
 
-// The closest-hit shader for a found hit, including its shadow ray.
-template <bool kAnyHit, bool kStats>
-__device__ inline void closest_hit(const FrameArgs& fa, Payload& ray, const HitInfo& h, D3 o, D3 d, uint32_t* stack,
-                                   uint32_t stride, PathCounters& pc) {
-    double ht, bu, bv;
-    refine_hit(fa.tlas, h, o, d, ht, bu, bv);
-    ShadeState ss;
-    if (!shade_begin<kStats>(fa, ray, h, ht, bu, bv, d, ss, pc)) return;
-    bool unocc = false;
-    if (ss.has_light) {
-        if (kStats) pc.shadow++;
-        const uint32_t shadow_flags = RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RT_RAY_FLAG_SKIP_CLOSEST_HIT |
-                                      (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
-        Rng4 shadow_rng = ray.rng;  // copy of Ray.rngState, discarded at rchit:476
-        AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
-        HitInfo sh;
-        unocc = !trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(ss.origin), to_f3(ss.light_dir), float(kShadowTMin),
-                                           float(ss.light_dist), shadow_flags, sah, stack, stride, sh, pc.tr);
+// Integrator state of one pixel between bounces (registers in the megakernel,
+// structure-of-arrays in HBM in the wavefront form).  Streams A and B keep only
+// their counters: their other words are (x, y, frame_count) and (x, y, seed_b).
+struct PathState {
+    D3 o, d;            // ray of the current bounce
+    D3 thr, rad;        // throughput, radiance summed over the pixel's samples
+    D3 hit_point;       // Ray.hitPoint (persists across samples, rchit:142)
+    double volume_dis;  // Ray.volume_dis
+    double tfar;
+    uint32_t wa, wb, lcg, sample, bounce;
+};
+
+__device__ inline void init_path(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p) {
+    p.wa = 0u;
+    p.wb = 0u;
+    p.lcg = tea16(tea16(x, y), fa.seed_b);  // Ray.RandomSeed, rgen:33
+    p.rad = d3(0.0);
+    p.hit_point = d3(0.0);
+    p.volume_dis = -1.0;
+    p.sample = 0u;
+}
+
+// Start of one sample, rgen:52-80.
+__device__ inline void begin_sample(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p) {
+    Rng4 ra{x, y, fa.frame_count, p.wa};
+    camera_ray(fa, x, y, ra, p.lcg, p.o, p.d, p.tfar);
+    p.wa = ra.w;
+    p.thr = d3(1.0);
+    p.volume_dis = -1.0;
+    p.bounce = 0u;
+}
+
+// What one traced bounce contributes, and the shadow ray it needs.
+struct BounceOut {
+    D3 pend_with;     // throughput * (emittance + NEE): added when the shadow ray is unoccluded
+    D3 pend_without;  // throughput * emittance
+    D3 so, sd;        // shadow ray origin and direction
+    double sdist;
+    uint32_t shadow_wb;  // stream-B counter of the shadow payload's copy (rchit:45)
+    bool shadow, cont;
+};
+
+// The shaders for the traced result of p's current ray (closest-hit with NEE set-up
+// or miss), then the rest of the raygen bounce loop (rgen:98-128): Russian
+// roulette, throughput update, next ray.
+template <bool kStats>
+__device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p, bool found,
+                                    const HitInfo& h, BounceOut& r, PathCounters& pc) {
+    Payload ray;
+    ray.hit_point = p.hit_point;
+    ray.t = p.bounce == 0u ? 0.0 : 1.0;  // only "Ray.t != 0" is read before it is set (rmiss:18)
+    ray.rng = Rng4{x, y, fa.seed_b, p.wb};
+    ray.volume_dis = p.volume_dis;
+    ray.emittance = d3(0.0);
+    ray.need_scatter = false;
+    ray.hit_value = d3(0.0);
+    ray.scatter_dir = d3(0.0);
+    D3 nee = d3(0.0);
+    r.shadow = false;
+    if (found) {
+        double ht, bu, bv;
+        refine_hit(fa.tlas, h, p.o, p.d, ht, bu, bv);
+        ShadeState ss;
+        if (shade_begin<kStats>(fa, ray, h, ht, bu, bv, p.d, ss, pc)) {
+            if (ss.has_light) {
+                r.shadow = true;
+                r.so = ss.origin;
+                r.sd = ss.light_dir;
+                r.sdist = ss.light_dist;
+                r.shadow_wb = ray.rng.w;
+                nee = eval_combined_brdf(ss.outward, ss.light_dir, ss.V, ss.m) * ss.light_scale;
+            }
+            shade_end(ray, ss);
+        }
+    } else {
+        miss_shader(fa.sv, ray, p.d);
     }
-    shade_end(ray, ss, unocc);
+    r.pend_without = p.thr * ray.emittance;
+    r.pend_with = p.thr * (ray.emittance + nee);
+    p.hit_point = ray.hit_point;
+    p.volume_dis = ray.volume_dis;
+    p.wb = ray.rng.w;
+    r.cont = false;
+    if (p.bounce + 1u >= fa.bounces) return;
+    if (p.bounce > kMinBounces) {  // Russian roulette, rgen:109-118
+        Rng4 ra{x, y, fa.frame_count, p.wa};
+        double rr = clampd(luminance(p.thr), 0.01, 0.95);
+        double prop = rng_next(ra);
+        p.wa = ra.w;
+        if (rr < prop) return;
+        p.thr = p.thr / rr;
+    }
+    p.thr = p.thr * ray.hit_value;
+    if (!ray.need_scatter || ray.t < 0.0) return;
+    p.o = ray.hit_point;
+    p.d = ray.scatter_dir;
+    p.bounce++;
+    r.cont = true;
+}
+
+constexpr uint32_t kShadowFlags = RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RT_RAY_FLAG_SKIP_CLOSEST_HIT;
+
+// The shadow ray of castShadowRay (rchit:35-59): origin without offset, tMin 0.1,
+// tMax the light distance; returns true when unoccluded.
+template <bool kAnyHit, bool kStats>
+__device__ inline bool shadow_unoccluded(const FrameArgs& fa, uint32_t x, uint32_t y, const BounceOut& r,
+                                         uint32_t* stack, uint32_t stride, PathCounters& pc) {
+    if (kStats) pc.shadow++;
+    Rng4 shadow_rng{x, y, fa.seed_b, r.shadow_wb};
+    AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
+    HitInfo sh;
+    return !trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(r.so), to_f3(r.sd), float(kShadowTMin), float(r.sdist),
+                                       kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u), sah, stack, stride,
+                                       sh, pc.tr);
 }
 
 // RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance.
 template <bool kAnyHit, bool kStats>
 __device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, uint32_t stride,
                                   PathCounters& pc) {
-    Rng4 rng_a{x, y, fa.frame_count, 0u};
-    Payload ray;
-    ray.rng = Rng4{x, y, fa.seed_b, 0u};
-    ray.hit_point = d3(0.0);
-    ray.hit_value = d3(0.0);
-    ray.emittance = d3(0.0);
-    ray.need_scatter = false;
-    ray.volume_dis = -1.0;
-    uint32_t lcg = tea16(tea16(x, y), fa.seed_b);
-    D3 radiance = d3(0.0);
+    PathState p;
+    init_path(fa, x, y, p);
     const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-    for (uint32_t s = 0; s < fa.samples; s++) {
-        D3 o, d;
-        double tfar;
-        camera_ray(fa, x, y, rng_a, lcg, o, d, tfar);
-        D3 thr = d3(1.0);
-        ray.t = 0.0;
-        ray.volume_dis = -1.0;
-        for (uint32_t b = 0; b < fa.bounces; b++) {
-            if (kStats) { if (b == 0) pc.primary++; else pc.extension++; }
-            AlphaAnyHit ah{&fa.sv, &fa.tlas, &ray.rng};
+    for (; p.sample < fa.samples; p.sample++) {
+        begin_sample(fa, x, y, p);
+        if (fa.bounces == 0u) continue;
+        for (;;) {
+            if (kStats) { if (p.bounce == 0u) pc.primary++; else pc.extension++; }
+            Rng4 rb{x, y, fa.seed_b, p.wb};
+            AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
             HitInfo h;
-            bool hit = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(o), to_f3(d), float(kTMin), float(tfar), flags, ah,
-                                                  stack, stride, h, pc.tr);
-            if (hit) closest_hit<kAnyHit, kStats>(fa, ray, h, o, d, stack, stride, pc);
-            else miss_shader(fa.sv, ray, d);
-            D3 hit_color = ray.hit_value;
-            double t = ray.t;
-            bool scattered = ray.need_scatter;
-            radiance = radiance + thr * ray.emittance;
-            ray.emittance = d3(0.0);
-            if (b + 1 == fa.bounces) break;
-            if (b > kMinBounces) {  // Russian roulette, rgen:109-118
-                double rr = clampd(luminance(thr), 0.01, 0.95);
-                double prop = rng_next(rng_a);
-                if (rr < prop) break;
-                thr = thr / rr;
-            }
-            thr = thr * hit_color;
-            if (!scattered || t < 0.0) break;
-            o = ray.hit_point;
-            d = ray.scatter_dir;
+            bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(p.o), to_f3(p.d), float(kTMin), float(p.tfar),
+                                                    flags, ah, stack, stride, h, pc.tr);
+            p.wb = rb.w;
+            BounceOut r;
+            shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
+            bool unocc = r.shadow && shadow_unoccluded<kAnyHit, kStats>(fa, x, y, r, stack, stride, pc);
+            p.rad = p.rad + (unocc ? r.pend_with : r.pend_without);
+            if (!r.cont) break;
         }
     }
-    return radiance;
+    return p.rad;
 }
 
 }  // namespace rt

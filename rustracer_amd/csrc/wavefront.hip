@@ -1,18 +1,399 @@
-// wavefront.hip — wavefront pipeline (placeholder until the queue kernels land).
+// wavefront.hip — the wavefront form of the trace (RT_MODE_WAVEFRONT).
+//
+// The raygen loop of the reference (crates/examples/gltf_viewer/shaders/RayTracing.rgen)
+// is split into queue kernels that each do one stage for every live path:
+//   init     path state of every pixel, camera ray of sample 0           -> queue
+//   extend   closest-hit traversal (fp32, small register footprint)
+//   shade    closest-hit / miss shaders in fp64, NEE set-up, next ray     -> shadow queue
+//   shadow   shadow-ray traversal with terminate-on-first-hit
+//   resolve  radiance update; a path that ended starts its pixel's next
+//            sample or retires the pixel into the accumulation image       -> next queue
+// One slot per pixel keeps each pixel's RNG streams sequential, so the image is
+// the megakernel's bit for bit (both call the same device functions in path.h).
+// Queues are compacted with wave ballots (one atomic per wave).  Kernels are
+// persistent grid-stride loops that read their queue length from device memory,
+// so the host only checks for completion every few iterations.
+#include <algorithm>
+#include <vector>
+
+#include "path.h"
 #include "wavefront.h"
 
 namespace rt {
 
-struct WavefrontState {
-    int unused;
+namespace {
+
+constexpr int kWfBlock = 256;
+constexpr uint32_t kCheckEvery = 8;
+
+struct WfArrays {
+    double *ox, *oy, *oz, *dx, *dy, *dz;  // ray
+    double *tx, *ty, *tz, *rx, *ry, *rz;  // throughput, radiance
+    double *hx, *hy, *hz, *vol, *tfar;    // Ray.hitPoint, Ray.volume_dis, tMax
+    double *pwx, *pwy, *pwz, *pox, *poy, *poz;  // pending radiance with / without NEE
+    uint32_t *wa, *wb, *lcg, *sample, *bounce;
+    uint32_t *h_inst, *h_prim, *h_rec;
+    float *h_t, *h_u, *h_v;
+    uint32_t *flags;   // kFlagShadow | kFlagCont
+    uint32_t *unocc;   // shadow result
+    float4 *sq_o, *sq_d;  // shadow queue: origin + tmax, direction + slot bits
+    uint32_t* sq_wb;
+    uint32_t *q0, *q1;
+    uint32_t* counts;  // [0], [1]: path queues (ping-pong), [2]: shadow queue
 };
 
-WavefrontState* wavefront_create() { return new WavefrontState{0}; }
-void wavefront_destroy(WavefrontState* s) { delete s; }
+enum : uint32_t { kFlagShadow = 1u, kFlagCont = 2u };
 
-int launch_wavefront(WavefrontState*, const FrameArgs&, hipStream_t, bool, bool, std::string* err) {
-    *err = "wavefront pipeline not available";
-    return kWavefrontUnsupported;
+__device__ inline void slot_pixel(const FrameArgs& fa, uint32_t s, uint32_t& x, uint32_t& y) {
+    // 8x8 pixel tiles: consecutive slots are neighbouring pixels
+    uint32_t tiles_x = (fa.width + 7) / 8;
+    uint32_t tile = s >> 6, within = s & 63u;
+    x = (tile % tiles_x) * 8 + (within & 7u);
+    y = (tile / tiles_x) * 8 + (within >> 3);
+}
+
+__device__ inline void store_ray(const WfArrays& a, uint32_t s, const PathState& p) {
+    a.ox[s] = p.o.x; a.oy[s] = p.o.y; a.oz[s] = p.o.z;
+    a.dx[s] = p.d.x; a.dy[s] = p.d.y; a.dz[s] = p.d.z;
+}
+__device__ inline void load_full(const WfArrays& a, uint32_t s, PathState& p) {
+    p.o = d3(a.ox[s], a.oy[s], a.oz[s]);
+    p.d = d3(a.dx[s], a.dy[s], a.dz[s]);
+    p.thr = d3(a.tx[s], a.ty[s], a.tz[s]);
+    p.rad = d3(a.rx[s], a.ry[s], a.rz[s]);
+    p.hit_point = d3(a.hx[s], a.hy[s], a.hz[s]);
+    p.volume_dis = a.vol[s];
+    p.tfar = a.tfar[s];
+    p.wa = a.wa[s];
+    p.wb = a.wb[s];
+    p.lcg = a.lcg[s];
+    p.sample = a.sample[s];
+    p.bounce = a.bounce[s];
+}
+__device__ inline void store_full(const WfArrays& a, uint32_t s, const PathState& p) {
+    store_ray(a, s, p);
+    a.tx[s] = p.thr.x; a.ty[s] = p.thr.y; a.tz[s] = p.thr.z;
+    a.rx[s] = p.rad.x; a.ry[s] = p.rad.y; a.rz[s] = p.rad.z;
+    a.hx[s] = p.hit_point.x; a.hy[s] = p.hit_point.y; a.hz[s] = p.hit_point.z;
+    a.vol[s] = p.volume_dis;
+    a.tfar[s] = p.tfar;
+    a.wa[s] = p.wa;
+    a.wb[s] = p.wb;
+    a.lcg[s] = p.lcg;
+    a.sample[s] = p.sample;
+    a.bounce[s] = p.bounce;
+}
+
+// Appends `item` to queue q (count c) for the lanes with want set: one atomic per wave.
+__device__ inline uint32_t wave_append(bool want, uint32_t* c) {
+    uint64_t m = __ballot(want);
+    uint32_t base = 0;
+    if (m) {
+        uint32_t first = __ffsll((unsigned long long)m) - 1;
+        if (__lane_id() == first) base = atomicAdd(c, uint32_t(__popcll(m)));
+        base = __shfl(base, first, 64);
+    }
+    return base + uint32_t(__popcll(m & lanemask_lt()));
+}
+
+__device__ inline void write_accum(const FrameArgs& fa, uint32_t x, uint32_t y, D3 rad) {
+    float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
+    float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
+    *acc = make_float4(prev.x + float(rad.x), prev.y + float(rad.y), prev.z + float(rad.z), 0.f);
+}
+
+__global__ __launch_bounds__(kWfBlock) void k_wf_init(FrameArgs fa, WfArrays a, uint32_t nslots) {
+    uint32_t* q = a.q0;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < nslots; base += gridDim.x * kWfBlock) {
+        uint32_t s = base + threadIdx.x;
+        uint32_t x = 0, y = 0;
+        bool valid = s < nslots;
+        if (valid) {
+            slot_pixel(fa, s, x, y);
+            valid = x < fa.width && y < fa.height;
+        }
+        if (valid) {
+            PathState p;
+            init_path(fa, x, y, p);
+            begin_sample(fa, x, y, p);
+            store_full(a, s, p);
+        }
+        uint32_t idx = wave_append(valid, &a.counts[0]);
+        if (valid) q[idx] = s;
+    }
+}
+
+template <bool kAnyHit, bool kStats>
+__global__ __launch_bounds__(kWfBlock) void k_wf_extend(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in,
+                                                         uint32_t* zero0, uint32_t* zero1) {
+    __shared__ uint32_t s_stack[kStackDepth * kWfBlock];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *zero0 = 0u;  // next path queue
+        *zero1 = 0u;  // shadow queue
+    }
+    const uint32_t n = *n_in;
+    const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
+    PathCounters pc;
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
+        uint32_t s = q[i];
+        uint32_t x, y;
+        slot_pixel(fa, s, x, y);
+        float3 o = make_float3(float(a.ox[s]), float(a.oy[s]), float(a.oz[s]));
+        float3 d = make_float3(float(a.dx[s]), float(a.dy[s]), float(a.dz[s]));
+        if (kStats) { if (a.bounce[s] == 0u) pc.primary++; else pc.extension++; }
+        Rng4 rb{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
+        AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
+        HitInfo h;
+        bool found = trace_ray<kAnyHit, kStats>(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), flags, ah,
+                                                s_stack + threadIdx.x, kWfBlock, h, pc.tr);
+        if (kAnyHit) a.wb[s] = rb.w;
+        a.h_inst[s] = found ? h.inst : kEmpty;
+        a.h_prim[s] = h.prim;
+        a.h_rec[s] = h.rec;
+        a.h_t[s] = h.t;
+        a.h_u[s] = h.u;
+        a.h_v[s] = h.v;
+    }
+    if (kStats) flush_counters(fa.counters, pc);
+}
+
+template <bool kStats>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in) {
+    const uint32_t n = *n_in;
+    PathCounters pc;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < n;
+        BounceOut r;
+        r.shadow = false;
+        uint32_t s = 0;
+        if (active) {
+            s = q[i];
+            uint32_t x, y;
+            slot_pixel(fa, s, x, y);
+            PathState p;
+            load_full(a, s, p);
+            HitInfo h;
+            h.inst = a.h_inst[s];
+            bool found = h.inst != kEmpty;
+            h.prim = a.h_prim[s];
+            h.rec = a.h_rec[s];
+            h.t = a.h_t[s];
+            h.u = a.h_u[s];
+            h.v = a.h_v[s];
+            shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
+            store_full(a, s, p);
+            a.pwx[s] = r.pend_with.x; a.pwy[s] = r.pend_with.y; a.pwz[s] = r.pend_with.z;
+            a.pox[s] = r.pend_without.x; a.poy[s] = r.pend_without.y; a.poz[s] = r.pend_without.z;
+            a.flags[s] = (r.shadow ? kFlagShadow : 0u) | (r.cont ? kFlagCont : 0u);
+        }
+        bool want = active && r.shadow;
+        uint32_t idx = wave_append(want, &a.counts[2]);
+        if (want) {
+            float3 so = to_f3(r.so), sd = to_f3(r.sd);
+            a.sq_o[idx] = make_float4(so.x, so.y, so.z, float(r.sdist));
+            a.sq_d[idx] = make_float4(sd.x, sd.y, sd.z, __uint_as_float(s));
+            a.sq_wb[idx] = r.shadow_wb;
+        }
+    }
+    if (kStats) flush_counters(fa.counters, pc);
+}
+
+template <bool kAnyHit, bool kStats>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shadow(FrameArgs fa, WfArrays a) {
+    __shared__ uint32_t s_stack[kStackDepth * kWfBlock];
+    const uint32_t n = a.counts[2];
+    const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
+    PathCounters pc;
+    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
+        float4 o = a.sq_o[i], d = a.sq_d[i];
+        uint32_t s = __float_as_uint(d.w);
+        uint32_t x, y;
+        slot_pixel(fa, s, x, y);
+        if (kStats) pc.shadow++;
+        Rng4 shadow_rng{x, y, fa.seed_b, a.sq_wb[i]};
+        AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
+        HitInfo sh;
+        bool hit = trace_ray<kAnyHit, kStats>(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
+                                              float(kShadowTMin), o.w, flags, sah, s_stack + threadIdx.x, kWfBlock, sh,
+                                              pc.tr);
+        a.unocc[s] = hit ? 0u : 1u;
+    }
+    if (kStats) flush_counters(fa.counters, pc);
+}
+
+__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in,
+                                                          uint32_t* q_out, uint32_t* n_out) {
+    const uint32_t n = *n_in;
+    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool alive = false;
+        uint32_t s = 0;
+        if (i < n) {
+            s = q[i];
+            uint32_t x, y;
+            slot_pixel(fa, s, x, y);
+            uint32_t f = a.flags[s];
+            bool unocc = (f & kFlagShadow) && a.unocc[s];
+            D3 rad = d3(a.rx[s], a.ry[s], a.rz[s]);
+            D3 pend = unocc ? d3(a.pwx[s], a.pwy[s], a.pwz[s]) : d3(a.pox[s], a.poy[s], a.poz[s]);
+            rad = rad + pend;
+            a.rx[s] = rad.x; a.ry[s] = rad.y; a.rz[s] = rad.z;
+            if (f & kFlagCont) {
+                alive = true;
+            } else {
+                uint32_t sample = a.sample[s] + 1u;
+                a.sample[s] = sample;
+                if (sample < fa.samples) {
+                    PathState p;
+                    p.wa = a.wa[s];
+                    p.lcg = a.lcg[s];
+                    begin_sample(fa, x, y, p);
+                    store_ray(a, s, p);
+                    a.wa[s] = p.wa;
+                    a.lcg[s] = p.lcg;
+                    a.tx[s] = p.thr.x; a.ty[s] = p.thr.y; a.tz[s] = p.thr.z;
+                    a.vol[s] = p.volume_dis;
+                    a.tfar[s] = p.tfar;
+                    a.bounce[s] = 0u;
+                    alive = true;
+                } else {
+                    write_accum(fa, x, y, rad);
+                }
+            }
+        }
+        uint32_t idx = wave_append(alive, n_out);
+        if (alive) q_out[idx] = s;
+    }
+}
+
+template <class T> size_t carve(char*& p, T*& out, size_t n) {
+    size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
+    out = reinterpret_cast<T*>(p);
+    p += bytes;
+    return bytes;
+}
+
+size_t layout(WfArrays& a, char* base, uint32_t n) {
+    char* p = base;
+    size_t total = 0;
+    double** dptrs[] = {&a.ox, &a.oy, &a.oz, &a.dx, &a.dy, &a.dz, &a.tx, &a.ty, &a.tz, &a.rx, &a.ry, &a.rz,
+                        &a.hx, &a.hy, &a.hz, &a.vol, &a.tfar, &a.pwx, &a.pwy, &a.pwz, &a.pox, &a.poy, &a.poz};
+    for (double** d : dptrs) total += carve(p, *d, n);
+    uint32_t** uptrs[] = {&a.wa, &a.wb, &a.lcg, &a.sample, &a.bounce, &a.h_inst, &a.h_prim, &a.h_rec, &a.flags,
+                          &a.unocc, &a.sq_wb, &a.q0, &a.q1};
+    for (uint32_t** u : uptrs) total += carve(p, *u, n);
+    total += carve(p, a.h_t, n);
+    total += carve(p, a.h_u, n);
+    total += carve(p, a.h_v, n);
+    total += carve(p, a.sq_o, n);
+    total += carve(p, a.sq_d, n);
+    total += carve(p, a.counts, 4);
+    return total;
+}
+
+__global__ void k_zero_counts(uint32_t* c) {
+    if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+struct WavefrontState {
+    uint32_t cap = 0;
+    char* mem = nullptr;
+    WfArrays a{};
+    uint32_t grid = 0;
+};
+
+WavefrontState* wavefront_create() { return new WavefrontState(); }
+
+void wavefront_destroy(WavefrontState* s) {
+    if (!s) return;
+    if (s->mem) (void)hipFree(s->mem);
+    delete s;
+}
+
+int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, std::string* err) {
+    if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats);
+    const uint32_t tiles = ((fa.width + 7) / 8) * ((fa.height + 7) / 8);
+    const uint32_t nslots = tiles * 64u;
+    if (nslots > ws->cap) {
+        if (ws->mem) (void)hipFree(ws->mem);
+        ws->mem = nullptr;
+        ws->cap = 0;
+        WfArrays probe{};
+        size_t bytes = layout(probe, nullptr, nslots) + 4096;
+        hipError_t e = hipMalloc(&ws->mem, bytes);
+        if (e != hipSuccess) {
+            *err = "cannot allocate the path state";
+            return int(e);
+        }
+        ws->cap = nslots;
+        layout(ws->a, ws->mem, nslots);
+    }
+    if (ws->grid == 0) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        ws->grid = uint32_t(std::max(cus, 1)) * 8u;
+    }
+    const WfArrays& a = ws->a;
+    const uint32_t grid = ws->grid;
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, a.counts);
+    hipLaunchKernelGGL(k_wf_init, dim3(std::min(grid, (nslots + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, st, fa,
+                       a, nslots);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        *err = "launch k_wf_init";
+        return int(e);
+    }
+    // every path ends within samples x bounces bounces
+    const uint64_t max_iter = uint64_t(fa.samples) * fa.bounces + kCheckEvery;
+    for (uint64_t k = 0;; k++) {
+        uint32_t* q_in = (k & 1) ? a.q1 : a.q0;
+        uint32_t* q_out = (k & 1) ? a.q0 : a.q1;
+        uint32_t* n_in = a.counts + (k & 1);
+        uint32_t* n_out = a.counts + ((k + 1) & 1);
+#define RT_WF(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kWfBlock), 0, st, __VA_ARGS__)
+        if (any_hit) {
+            if (stats) RT_WF((k_wf_extend<true, true>), fa, a, q_in, n_in, n_out, a.counts + 2);
+            else RT_WF((k_wf_extend<true, false>), fa, a, q_in, n_in, n_out, a.counts + 2);
+        } else {
+            if (stats) RT_WF((k_wf_extend<false, true>), fa, a, q_in, n_in, n_out, a.counts + 2);
+            else RT_WF((k_wf_extend<false, false>), fa, a, q_in, n_in, n_out, a.counts + 2);
+        }
+        if (stats) RT_WF((k_wf_shade<true>), fa, a, q_in, n_in);
+        else RT_WF((k_wf_shade<false>), fa, a, q_in, n_in);
+        if (any_hit) {
+            if (stats) RT_WF((k_wf_shadow<true, true>), fa, a);
+            else RT_WF((k_wf_shadow<true, false>), fa, a);
+        } else {
+            if (stats) RT_WF((k_wf_shadow<false, true>), fa, a);
+            else RT_WF((k_wf_shadow<false, false>), fa, a);
+        }
+        RT_WF(k_wf_resolve, fa, a, q_in, n_in, q_out, n_out);
+#undef RT_WF
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+            *err = "launch of a wavefront kernel";
+            return int(e);
+        }
+        if ((k + 1) % kCheckEvery == 0) {
+            uint32_t live = 0;
+            e = hipMemcpyAsync(&live, n_out, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                *err = "reading the queue length";
+                return int(e);
+            }
+            if (live == 0) break;
+        }
+        if (k > max_iter) {
+            *err = "paths did not terminate";
+            return int(hipErrorUnknown);
+        }
+    }
+    return 0;
 }
 
 }  // namespace rt

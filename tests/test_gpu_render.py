@@ -105,3 +105,43 @@ def test_deterministic_bit_for_bit(gpu_ctx):
     # the counting variant renders the same image
     c, _ = _render(gpu_ctx, ds, stats=True)
     assert np.array_equal(a.cpu().numpy(), c.cpu().numpy())
+
+
+@pytest.mark.parametrize("name", ["cornell_sphere", "displaced_sphere"])
+def test_wavefront_matches_megakernel_bit_for_bit(gpu_ctx, name):
+    ds = G.device_scene(gpu_ctx, name)
+    a, sa = _render(gpu_ctx, ds, mode=rt.MODE_MEGAKERNEL, stats=True)
+    b, sb = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT, stats=True)
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    for k in ("rays", "primary_rays", "extension_rays", "shadow_rays", "shade_events", "light_candidates"):
+        assert sa[k] == sb[k], k
+    c, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
+    assert np.array_equal(b.cpu().numpy(), c.cpu().numpy())
+
+
+def test_wavefront_alpha_mask_and_accumulation(gpu_ctx):
+    def make():
+        sc = K.triangle_soup(400, seed=31, opaque=False)
+        rng = np.random.default_rng(32)
+        sc.vertices["color"][:, 3] = rng.random(len(sc.vertices))
+        sc.ubo["fully_opaque"] = 0
+        sc.ubo["number_of_bounces"] = 5
+        sc.ubo["number_of_samples"] = sc.ubo["total_number_of_samples"] = 3
+        from rustracer_amd import scenes as S
+        sc.plights = np.array([S.point_light((0.0, 3.0, 4.0), 30.0)])
+        return sc
+    ds = G.device_scene(gpu_ctx, "mask_soup_lit", make)
+    a, sa = _render(gpu_ctx, ds, mode=rt.MODE_MEGAKERNEL, stats=True)
+    b, sb = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT, stats=True)
+    assert sa["anyhit_calls"] > 0 and sa["rays"] == sb["rays"]
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    o, _ = B.OracleScene(ds.scene).render()
+    assert G.rel_rmse(a.cpu().numpy()[..., :3].astype(np.float64), o[..., :3]) <= REL_RMSE_GATE
+    # accumulation onto an existing image
+    ubo = ds.scene.ubo.copy()
+    ubo["total_number_of_samples"] = 6
+    ubo["frame_count"] = 1
+    a2, b2 = a.clone(), b.clone()
+    _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_MEGAKERNEL, accum=a2)
+    _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT, accum=b2)
+    assert np.array_equal(a2.cpu().numpy(), b2.cpu().numpy())

@@ -31,7 +31,7 @@ def _assert_parity(scene, rays, gh, oh, otuv, max_tie_frac=1e-3):
     hit = same & (oh["primitive_id"] != MISS)
     # t and barycentrics agree to fp32 accuracy (grazing hits are ill-conditioned)
     rel = np.abs(gh["t"][hit] - otuv[hit, 0]) / np.maximum(otuv[hit, 0], 1e-3)
-    assert np.percentile(rel, 99.9) < 1e-5 and rel.max() < 1e-3
+    assert np.percentile(rel, 99.9) < 1e-4 and rel.max() < 1e-3
     assert np.abs(gh["u"][hit] - otuv[hit, 1]).max() < 1e-3 and np.abs(gh["v"][hit] - otuv[hit, 2]).max() < 1e-3
 
 
