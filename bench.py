@@ -26,11 +26,29 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {"node": 128, "tri": 48, "instance": 128, "shade": 668, "anyhit": 668, "tex": 16, "light": 48, "pixel": 32}
 
 
+# per closest-hit ray in the extend kernel: queue index (4 B), fp64 origin +
+# direction + tmax (56 B), hit record written (instance, primitive, record, t, u, v:
+# 24 B)
+EXTEND_RAY_BYTES = 84
+
+
 def algorithmic_bytes(st, pixels):
+    """Bytes of one whole frame (every kernel)."""
     return (st["node_visits"] * BYTES["node"] + st["tri_tests"] * BYTES["tri"] +
             st["instance_visits"] * BYTES["instance"] + st["shade_events"] * BYTES["shade"] +
             st["anyhit_calls"] * BYTES["anyhit"] + st["tex_fetches"] * BYTES["tex"] +
             st["light_candidates"] * BYTES["light"] + pixels * BYTES["pixel"])
+
+
+def extend_bytes(st):
+    """Bytes of the closest-hit traversal of one frame (all k_wf_extend launches):
+    the nodes, triangles and instances its rays visit plus each ray's own record.
+    Any-hit work is absent from this workload (every material is opaque)."""
+    rays = st["primary_rays"] + st["extension_rays"]
+    return ((st["node_visits"] - st["shadow_node_visits"]) * BYTES["node"] +
+            (st["tri_tests"] - st["shadow_tri_tests"]) * BYTES["tri"] +
+            (st["instance_visits"] - st["shadow_instance_visits"]) * BYTES["instance"] +
+            rays * EXTEND_RAY_BYTES)
 
 
 def measured_traffic(kernel):
@@ -54,12 +72,20 @@ def cpu_baseline(scene, target_s, threads):
     W, H = scene.width, scene.height
     rng = np.random.default_rng(0)
     order = rng.permutation(W * H)
-    n = 64
-    t0 = time.perf_counter()
-    _, st = o.render_pixels(np.stack([order[:n] % W, order[:n] // W], 1), precision=32, threads=threads)
-    dt = time.perf_counter() - t0
-    n2 = int(min(W * H, max(n, n * target_s / max(dt, 1e-3))))
-    xy = np.stack([order[n:n + n2] % W, order[n:n + n2] // W], 1)
+    # calibrate on growing samples until one takes at least a second, then size the
+    # timed sample to target_s
+    n, used = 64, 0
+    while True:
+        xy = np.stack([order[used:used + n] % W, order[used:used + n] // W], 1)
+        t0 = time.perf_counter()
+        o.render_pixels(xy, precision=32, threads=threads)
+        dt = time.perf_counter() - t0
+        used += n
+        if dt >= 1.0 or used + 2 * n > W * H // 2:
+            break
+        n *= 2
+    n2 = int(min(W * H - used, max(n, n * target_s / max(dt, 1e-3))))
+    xy = np.stack([order[used:used + n2] % W, order[used:used + n2] // W], 1)
     t0 = time.perf_counter()
     _, st = o.render_pixels(xy, precision=32, threads=threads)
     dt = time.perf_counter() - t0
@@ -121,24 +147,40 @@ def main():
         ctx.sync()
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        # HIP events around every stage kernel, recorded on the context's stream (the
+        # stream the kernels run on): rt_get_stage_times after each frame
+        ctx.set_option(rt.OPTION_STAGE_TIMING, 1)
+        stages = []
         t0 = time.perf_counter()
         for k in range(args.steps):
             ev[k][0].record(stream)
             rt.trace_rays(ctx, tlas, buf, sc.ubo, acc, W, H, mode=modes[name])
             ev[k][1].record(stream)
+            stages.append(ctx.stage_times())
         ctx.sync()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        ctx.set_option(rt.OPTION_STAGE_TIMING, 0)
         kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
         ms_per_step = wall * 1e3 / args.steps
-        results[name] = {"st": st, "ms_per_step": ms_per_step, "kernel_ms": kern_ms,
+        stage = {k: float(np.mean([s_[k] for s_ in stages])) for k in stages[0]}
+        results[name] = {"st": st, "ms_per_step": ms_per_step, "kernel_ms": kern_ms, "stage": stage,
                          "mrays": st["rays"] / (ms_per_step / 1e3) / 1e6}
     best = max(results, key=lambda k: results[k]["mrays"])
     r = results[best]
     st = r["st"]
-    bytes_per_launch = algorithmic_bytes(st, W * H)
-    achieved = bytes_per_launch / (r["kernel_ms"] / 1e3) / 1e9
-    kernel_name = "k_megakernel" if best == "megakernel" else "wavefront"
+    stage = r["stage"]
+    if best == "wavefront":
+        # dominant kernel: the closest-hit traversal, launched once per iteration
+        kernel_name = "k_wf_extend"
+        launches = max(1, int(stage["iterations"]))
+        bytes_per_launch = extend_bytes(st) / launches
+        launch_ms = stage["extend_ms"] / launches
+    else:
+        kernel_name = "k_megakernel"
+        bytes_per_launch = algorithmic_bytes(st, W * H)
+        launch_ms = stage["extend_ms"]
+    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
     traffic = measured_traffic(kernel_name + "_displaced_sphere")
     line = {
         "metric": "Mrays/s at 1080p/64spp (wavefront vs megakernel) at RMSE<=1e-3; adjoint ms/iter",
@@ -161,8 +203,9 @@ def main():
                    "build_ms": round(min(build_ms), 2), "scene_gen_s": round(gen_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": kernel_name, "kernel_ms": round(r["kernel_ms"], 3),
-                     "algorithmic_bytes": int(bytes_per_launch)},
+                     "traffic": None if traffic is None else int(traffic), "kernel": kernel_name,
+                     "launch_ms": round(launch_ms, 4), "algorithmic_bytes": int(bytes_per_launch),
+                     "stage_ms": {k: round(v, 3) for k, v in stage.items()}},
     }
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

@@ -157,7 +157,33 @@ typedef struct rt_trace_stats {
     uint64_t light_candidates;/* RIS candidate light reads */
     uint64_t stack_overflows; /* rays whose traversal stack overflowed */
     double kernel_ms;         /* device time of the trace kernels */
+    uint64_t shadow_node_visits;     /* part of node_visits spent on shadow rays */
+    uint64_t shadow_tri_tests;       /* part of tri_tests spent on shadow rays */
+    uint64_t shadow_instance_visits; /* part of instance_visits spent on shadow rays */
 } rt_trace_stats;
+
+/* ------------------------------------------------------------ options */
+
+enum {
+    /* 1: record device time per pipeline stage on every trace (rt_get_stage_times) */
+    RT_OPTION_STAGE_TIMING = 1
+};
+int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
+
+/* Device time of the stages of the last trace, when RT_OPTION_STAGE_TIMING is on
+ * (the analogue of the reference's GPU timestamp query pool,
+ * crates/libs/vulkan/src/query.rs:8-73).  The megakernel reports all of its time
+ * as extend_ms.  Synchronises the context stream. */
+typedef struct rt_stage_times {
+    double total_ms;
+    double extend_ms;   /* closest-hit traversal */
+    double shade_ms;    /* closest-hit / miss shaders */
+    double shadow_ms;   /* shadow-ray traversal */
+    double resolve_ms;  /* radiance update, regeneration, compaction */
+    uint32_t iterations;
+    uint32_t _pad;
+} rt_stage_times;
+int rt_get_stage_times(rt_ctx* ctx, rt_stage_times* out);
 
 /* Runs the whole ray-tracing pipeline for a width x height launch, as
  * record_raytracing_commands + trace_rays do

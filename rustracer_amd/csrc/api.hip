@@ -73,6 +73,8 @@ struct rt_ctx {
     std::vector<rt_texture> tex_cache;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     WavefrontState* wavefront = nullptr;
+    bool stage_timing = false;
+    StageTimer timer;  // events of the last trace when stage_timing is set
 };
 
 namespace {
@@ -545,15 +547,17 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
         RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
     }
+    StageTimer* tm = ctx->stage_timing ? &ctx->timer : nullptr;
+    ctx->timer.used = 0;
     if (mode == RT_MODE_MEGAKERNEL) {
-        rc = launch_megakernel(fa, st, any_hit, stats != nullptr);
+        rc = launch_megakernel(fa, st, any_hit, stats != nullptr, tm);
     } else {
         if (!ctx->wavefront) {
             ctx->wavefront = wavefront_create();
             if (!ctx->wavefront) return fail(ctx, RT_ERR_OUT_OF_MEMORY, "wavefront state");
         }
         std::string err;
-        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, &err);
+        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
@@ -579,7 +583,40 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         stats->light_candidates = c[kCntLights];
         stats->stack_overflows = c[kCntOverflow];
         stats->kernel_ms = ms;
+        stats->shadow_node_visits = c[kCntShadowNodes];
+        stats->shadow_tri_tests = c[kCntShadowTris];
+        stats->shadow_instance_visits = c[kCntShadowInstances];
         if (c[kCntBadIndex]) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a hit referenced an out-of-range prim info, material, index or vertex");
+    }
+    return RT_OK;
+}
+
+int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (option == RT_OPTION_STAGE_TIMING) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_STAGE_TIMING takes 0 or 1");
+        ctx->stage_timing = value != 0;
+        ctx->timer.used = 0;
+        return RT_OK;
+    }
+    return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown option");
+}
+
+int rt_get_stage_times(rt_ctx* ctx, rt_stage_times* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    if (!ctx->stage_timing) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_STAGE_TIMING is off");
+    DeviceGuard g(ctx->device);
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const StageTimer& t = ctx->timer;
+    double* slot[4] = {&out->extend_ms, &out->shade_ms, &out->shadow_ms, &out->resolve_ms};
+    for (size_t i = 1; i < t.used; i++) {
+        float ms = 0.f;
+        RT_HIP(ctx, hipEventElapsedTime(&ms, t.events[i - 1], t.events[i]), "hipEventElapsedTime");
+        if (t.stage[i] >= 0 && t.stage[i] < 4) *slot[t.stage[i]] += ms;
+        if (t.stage[i] == 0) out->iterations++;
+        out->total_ms += ms;
     }
     return RT_OK;
 }

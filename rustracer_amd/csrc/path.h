@@ -24,12 +24,16 @@ constexpr uint32_t kMinBounces = 3; // RayTracing.rgen:15
 
 struct PathCounters {
     uint32_t primary = 0, extension = 0, shadow = 0, shade = 0, tex = 0, lights = 0, bad = 0;
-    TraceStats tr;
+    TraceStats tr;   // closest-hit traversal
+    TraceStats trs;  // shadow traversal
 };
 
 __device__ inline void flush_counters(unsigned long long* c, const PathCounters& pc) {
-    uint32_t v[kNumCounters] = {pc.primary, pc.extension, pc.shadow, pc.tr.nodes, pc.tr.tris, pc.tr.instances,
-                                pc.shade, pc.tr.anyhits, pc.tex, pc.lights, pc.tr.overflows, pc.bad, 0, 0, 0, 0};
+    uint32_t v[kNumCounters] = {pc.primary, pc.extension, pc.shadow, pc.tr.nodes + pc.trs.nodes,
+                                pc.tr.tris + pc.trs.tris, pc.tr.instances + pc.trs.instances, pc.shade,
+                                pc.tr.anyhits + pc.trs.anyhits, pc.tex, pc.lights,
+                                pc.tr.overflows + pc.trs.overflows, pc.bad, pc.trs.nodes, pc.trs.tris,
+                                pc.trs.instances, 0};
 #pragma unroll
     for (int k = 0; k < kNumCounters; k++) {
         unsigned long long s = wave_sum<unsigned long long>(v[k]);
@@ -542,7 +546,7 @@ __device__ inline bool shadow_unoccluded(const FrameArgs& fa, uint32_t x, uint32
     HitInfo sh;
     return !trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(r.so), to_f3(r.sd), float(kShadowTMin), float(r.sdist),
                                        kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u), sah, stack, stride,
-                                       sh, pc.tr);
+                                       sh, pc.trs);
 }
 
 // RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance.

@@ -1,6 +1,8 @@
 // render.h — launch interface of the trace kernels (render.hip, wavefront.hip).
 #pragma once
 
+#include <vector>
+
 #include "shade.h"
 #include "traverse.h"
 
@@ -23,10 +25,33 @@ struct FrameArgs {
 
 enum Counter : int {
     kCntPrimary = 0, kCntExtension, kCntShadow, kCntNodes, kCntTris, kCntInstances, kCntShade, kCntAnyHit,
-    kCntTex, kCntLights, kCntOverflow, kCntBadIndex, kNumCounters = 16
+    kCntTex, kCntLights, kCntOverflow, kCntBadIndex, kCntShadowNodes, kCntShadowTris, kCntShadowInstances,
+    kNumCounters = 16
 };
 
-int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats);
+// Per-stage event timing of a trace (enabled by RT_OPTION_STAGE_TIMING).
+struct StageTimer {
+    std::vector<hipEvent_t> events;
+    std::vector<int> stage;  // stage of the interval ending at events[i] (-1: none)
+    size_t used = 0;
+    // records an event closing an interval of `st` (0 extend, 1 shade, 2 shadow, 3 resolve, -1 start)
+    hipError_t mark(hipStream_t s, int st) {
+        if (used == events.size()) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            events.push_back(e);
+            stage.push_back(-1);
+        }
+        stage[used] = st;
+        return hipEventRecord(events[used++], s);
+    }
+    ~StageTimer() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
+
+int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* timer = nullptr);
 int launch_ray_queries(const TlasView& tl, const SceneView& sv, const rt_ray* rays, uint32_t n, uint32_t flags,
                        rt_hit* hits, uint32_t seed_b, hipStream_t st, bool any_hit);
 int launch_primary(const FrameArgs& fa, rt_hit* hits, hipStream_t st, bool any_hit);

@@ -112,15 +112,24 @@ __global__ __launch_bounds__(kMegaBlock) void k_primary(FrameArgs fa, rt_hit* hi
 
 }  // namespace
 
-int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats) {
+int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm) {
     uint32_t tiles = ((fa.width + 15) / 16) * ((fa.height + 7) / 8);
     if (tiles == 0) return 0;
+    if (tm) {
+        hipError_t e = tm->mark(st, -1);
+        if (e != hipSuccess) return int(e);
+    }
     if (any_hit) {
         if (stats) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
         else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
     } else {
         if (stats) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
         else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
+    }
+    if (tm) {
+        // the megakernel's time is all traversal + shading: reported as the extend stage
+        hipError_t e = tm->mark(st, 0);
+        if (e != hipSuccess) return int(e);
     }
     return int(hipGetLastError());
 }

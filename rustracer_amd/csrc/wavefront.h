@@ -15,6 +15,7 @@ void wavefront_destroy(WavefrontState* s);
 // megakernel.  Returns 0 or a hipError_t code (kWavefrontUnsupported when the
 // wavefront path is not available for this frame).
 constexpr int kWavefrontUnsupported = -1000;
-int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, std::string* err);
+int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm,
+                     std::string* err);
 
 }  // namespace rt

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(FrameArgs fa, WfArrays a
         HitInfo sh;
         bool hit = trace_ray<kAnyHit, kStats>(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
                                               float(kShadowTMin), o.w, flags, sah, s_stack + threadIdx.x, kWfBlock, sh,
-                                              pc.tr);
+                                              pc.trs);
         a.unocc[s] = hit ? 0u : 1u;
     }
     if (kStats) flush_counters(fa.counters, pc);
@@ -313,8 +313,9 @@ void wavefront_destroy(WavefrontState* s) {
     delete s;
 }
 
-int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, std::string* err) {
-    if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats);
+int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm,
+                     std::string* err) {
+    if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats, tm);
     const uint32_t tiles = ((fa.width + 7) / 8) * ((fa.height + 7) / 8);
     const uint32_t nslots = tiles * 64u;
     if (nslots > ws->cap) {
@@ -339,10 +340,17 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
     }
     const WfArrays& a = ws->a;
     const uint32_t grid = ws->grid;
+    hipError_t e = hipSuccess;
+    // stage marks: 0 extend, 1 shade, 2 shadow, 3 resolve (init counts as resolve)
+    auto mark = [&](int stage) {
+        if (tm && e == hipSuccess) e = tm->mark(st, stage);
+    };
+    mark(-1);
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, a.counts);
     hipLaunchKernelGGL(k_wf_init, dim3(std::min(grid, (nslots + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, st, fa,
                        a, nslots);
-    hipError_t e = hipGetLastError();
+    mark(3);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
         *err = "launch k_wf_init";
         return int(e);
@@ -362,8 +370,10 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
             if (stats) RT_WF((k_wf_extend<false, true>), fa, a, q_in, n_in, n_out, a.counts + 2);
             else RT_WF((k_wf_extend<false, false>), fa, a, q_in, n_in, n_out, a.counts + 2);
         }
+        mark(0);
         if (stats) RT_WF((k_wf_shade<true>), fa, a, q_in, n_in);
         else RT_WF((k_wf_shade<false>), fa, a, q_in, n_in);
+        mark(1);
         if (any_hit) {
             if (stats) RT_WF((k_wf_shadow<true, true>), fa, a);
             else RT_WF((k_wf_shadow<true, false>), fa, a);
@@ -371,9 +381,11 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
             if (stats) RT_WF((k_wf_shadow<false, true>), fa, a);
             else RT_WF((k_wf_shadow<false, false>), fa, a);
         }
+        mark(2);
         RT_WF(k_wf_resolve, fa, a, q_in, n_in, q_out, n_out);
+        mark(3);
 #undef RT_WF
-        e = hipGetLastError();
+        if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) {
             *err = "launch of a wavefront kernel";
             return int(e);

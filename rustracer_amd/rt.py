@@ -36,7 +36,8 @@ RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT = 0x4, 0x8
 # every symbol include/rt/rt.h declares
 EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", "rt_build_blas_batch",
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
-           "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info"]
+           "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
+           "rt_set_option", "rt_get_stage_times"]
 
 
 class RtError(RuntimeError):
@@ -69,7 +70,16 @@ class TraceStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "primary_rays", "extension_rays", "shadow_rays", "node_visits",
                                             "tri_tests", "instance_visits", "shade_events", "anyhit_calls",
                                             "tex_fetches", "light_candidates", "stack_overflows")] + \
-               [("kernel_ms", C.c_double)]
+               [("kernel_ms", C.c_double)] + \
+               [(n, C.c_uint64) for n in ("shadow_node_visits", "shadow_tri_tests", "shadow_instance_visits")]
+
+
+OPTION_STAGE_TIMING = 1
+
+
+class StageTimes(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("total_ms", "extend_ms", "shade_ms", "shadow_ms", "resolve_ms")] + \
+               [("iterations", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 class BvhInfo(C.Structure):
@@ -99,6 +109,7 @@ def load_library(path=LIB_PATH):
         "rt_trace_ray_queries": (i, [vp, u64, vp, vp, u, u, vp]),
         "rt_trace_primary": (i, [vp, u64, vp, u, u, vp]),
         "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
+        "rt_set_option": (i, [vp, u, u64]), "rt_get_stage_times": (i, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -144,6 +155,16 @@ class Context:
 
     def sync(self):
         self.check(self.lib.rt_sync(self.handle))
+
+    def set_option(self, option, value):
+        """rt_set_option (OPTION_STAGE_TIMING: event timing of every pipeline stage)."""
+        self.check(self.lib.rt_set_option(self.handle, C.c_uint32(option), C.c_uint64(value)))
+
+    def stage_times(self):
+        """rt_get_stage_times: device milliseconds per stage of the last trace."""
+        st = StageTimes()
+        self.check(self.lib.rt_get_stage_times(self.handle, C.byref(st)))
+        return {n: getattr(st, n) for n, _ in StageTimes._fields_ if n != "_pad"}
 
     def close(self):
         if getattr(self, "handle", None):

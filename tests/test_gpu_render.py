@@ -145,3 +145,30 @@ def test_wavefront_alpha_mask_and_accumulation(gpu_ctx):
     _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_MEGAKERNEL, accum=a2)
     _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT, accum=b2)
     assert np.array_equal(a2.cpu().numpy(), b2.cpu().numpy())
+
+
+@pytest.mark.parametrize("mode", [rt.MODE_MEGAKERNEL, rt.MODE_WAVEFRONT])
+def test_stage_timing_and_split_counters(gpu_ctx, mode):
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    a, st = _render(gpu_ctx, ds, mode=mode, stats=True)
+    gpu_ctx.set_option(rt.OPTION_STAGE_TIMING, 1)
+    try:
+        b, _ = _render(gpu_ctx, ds, mode=mode)
+        t = gpu_ctx.stage_times()
+    finally:
+        gpu_ctx.set_option(rt.OPTION_STAGE_TIMING, 0)
+    # timing changes nothing in the image
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    parts = t["extend_ms"] + t["shade_ms"] + t["shadow_ms"] + t["resolve_ms"]
+    assert t["total_ms"] > 0 and abs(parts - t["total_ms"]) <= 1e-3 * t["total_ms"] + 1e-3
+    if mode == rt.MODE_WAVEFRONT:
+        assert t["iterations"] >= int(ds.scene.ubo["number_of_bounces"])
+        assert min(t["extend_ms"], t["shade_ms"], t["shadow_ms"], t["resolve_ms"]) > 0
+    else:
+        assert t["iterations"] == 1 and t["total_ms"] == t["extend_ms"]
+    # shadow traversal is a part of the totals, and present (one point light)
+    assert 0 < st["shadow_node_visits"] < st["node_visits"]
+    assert 0 < st["shadow_tri_tests"] < st["tri_tests"]
+    assert st["shadow_instance_visits"] <= st["instance_visits"]
+    with pytest.raises(rt.RtError):
+        gpu_ctx.stage_times()  # option off
