@@ -165,8 +165,14 @@ typedef struct rt_trace_stats {
 /* ------------------------------------------------------------ options */
 
 enum {
-    /* 1: record device time per pipeline stage on every trace (rt_get_stage_times) */
-    RT_OPTION_STAGE_TIMING = 1
+    /* 1: record device time per pipeline stage on every trace (rt_get_stage_times).
+     * Default 0. */
+    RT_OPTION_STAGE_TIMING = 1,
+    /* 1 (default): rt_build_tlas copies the triangles of every instance whose
+     * transform is exactly the identity into one world-space BVH, traversed before
+     * the TLAS over the other instances.  Hits report the same instance, primitive
+     * and barycentrics as the two-level path.  Applies to TLASes built afterwards. */
+    RT_OPTION_FLATTEN_IDENTITY_INSTANCES = 2
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 
@@ -254,7 +260,13 @@ typedef struct rt_bvh_info {
 } rt_bvh_info;
 
 int rt_blas_info(rt_ctx* ctx, rt_blas blas, rt_bvh_info* out);
+/* The TLAS proper: num_prims = instances reached through it; `tris` points at the
+ * instance records (internal 256-byte layout). */
 int rt_tlas_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out);
+/* The flattened BVH of a TLAS's identity instances (num_prims = 0 when there is
+ * none).  Triangle records are 48 bytes: v0.w = primitive index, v1.w = instance
+ * record index | 0x80000000 if opaque, v2.w = source record in the BLAS pool. */
+int rt_tlas_flat_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out);
 
 #ifdef __cplusplus
 } /* extern "C" */

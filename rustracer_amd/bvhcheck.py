@@ -50,7 +50,7 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
                 errors.append(f"leaf [{c},{c + cnt}) outside [{first_prim},{first_prim + num_prims})")
                 continue
             covered[c - first_prim:c - first_prim + cnt] += 1
-            if kind == "triangle":
+            if kind in ("triangle", "flat"):
                 t = tris[c:c + cnt]
                 pts = np.concatenate([t["v0"], t["v1"][:, :3], t["v2"][:, :3]])
                 if np.any(pts < lo) or np.any(pts > hi):
@@ -67,16 +67,44 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
     return errors
 
 
-INST = np.dtype([("w2o", "<f4", (3, 4)), ("o2w", "<f4", (3, 4)), ("blas", "<u8"), ("custom_index", "<u4"),
-                 ("mask", "<u4"), ("flags", "<u4"), ("instance_id", "<u4"), ("_pad", "<u4", 2),
-                 ("w2o_d", "<f8", 12), ("_pad2", "<u4", 8)])
+INST = np.dtype([("w2o", "<f4", (3, 4)), ("nodes", "<u8"), ("tris", "<u8"), ("root", "<u4"), ("mask", "<u4"),
+                 ("flags", "<u4"), ("instance_id", "<u4"), ("o2w", "<f4", (3, 4)), ("blas", "<u8"),
+                 ("custom_index", "<u4"), ("_pad", "<u4"), ("w2o_d", "<f8", 12), ("_pad2", "<u4", 4)])
+FLAT_OPAQUE = 0x80000000
 
 
 def check_tlas(ctx, tlas, num_instances):
+    """The TLAS proper, the instance records, and the flattened BVH over identity
+    instances: every instance is reached exactly once, through one or the other."""
     info = tlas.info()
     nodes = rt.read_nodes(ctx, info)
-    errs = _check(nodes, info.root, 0, info.num_prims, None, "instance", num_instances)
-    inst = np.frombuffer(rt.read_device(info.tris, INST.itemsize * max(1, info.num_prims)), INST)[:info.num_prims]
+    errs = _check(nodes, info.root, 0, info.num_prims, None, "instance")
+    inst = np.frombuffer(rt.read_device(info.tris, INST.itemsize * max(1, num_instances)), INST)[:num_instances]
     if num_instances and not np.array_equal(np.sort(inst["instance_id"]), np.arange(num_instances)):
-        errs.append("TLAS instance ids are not a permutation")
+        errs.append("instance ids are not a permutation")
+    fi = tlas.flat_info()
+    nt = info.num_prims
+    if fi.num_prims == 0:
+        if nt != num_instances:
+            errs.append(f"TLAS reaches {nt} of {num_instances} instances and there is no flattened BVH")
+        return errs
+    fnodes = rt.read_nodes(ctx, fi)
+    ftris = np.frombuffer(rt.read_device(fi.tris, 48 * fi.num_prims), TRI)
+    errs += _check(fnodes, fi.root, 0, fi.num_prims, ftris, "flat")
+    bits = ftris["v1"][:, 3].copy().view("<u4")
+    ii = bits & ~np.uint32(FLAT_OPAQUE)
+    if np.any(ii < nt) or np.any(ii >= num_instances):
+        errs.append("flattened records reference instances outside the flattened range")
+        return errs
+    counts = np.bincount(ii - nt, minlength=num_instances - nt)
+    if np.any(counts == 0):
+        errs.append(f"{int(np.sum(counts == 0))} instance records after the TLAS ones have no flattened triangles")
+    order = np.lexsort((ftris["prim"], ii))
+    pi, pp = ii[order], ftris["prim"][order]
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    expect = np.arange(len(pp)) - np.repeat(starts, counts)
+    if not np.array_equal(pp, expect.astype(np.uint32)):
+        errs.append("primitive ids of a flattened instance are not a permutation of 0..n-1")
+    if np.any(np.abs(inst["w2o"][nt:] - np.eye(3, 4, dtype=np.float32)) > 0):
+        errs.append("a flattened instance does not have the identity transform")
     return errs

@@ -49,10 +49,35 @@ struct BlasEntry {
 struct TlasEntry {
     DevPtr nodes, inst;
     uint32_t num_nodes = 0, num_inst = 0;
+    uint32_t num_tlas = 0;  // instances in the TLAS proper (inst[0, num_tlas))
     float bounds[6] = {0, 0, 0, 0, 0, 0};
+    DevPtr flat_nodes, flat_tris;  // flattened BVH over the identity instances
+    uint32_t flat_num_nodes = 0, flat_num_tris = 0;
+    float flat_bounds[6] = {0, 0, 0, 0, 0, 0};
     std::vector<std::shared_ptr<BlasPool>> pools;
     bool any_non_opaque = false;
 };
+
+TlasView tlas_view(const TlasEntry& t, uint32_t* err) {
+    TlasView v{};
+    v.nodes = (const WideNode*)t.nodes.get();
+    v.inst = (const InstanceRecord*)t.inst.get();
+    v.root = 0;
+    v.num_inst = t.num_inst;
+    v.error_flags = err;
+    v.num_tlas = t.num_tlas;
+    v.flat_nodes = t.flat_num_tris ? (const WideNode*)t.flat_nodes.get() : nullptr;
+    v.flat_tris = (const float4*)t.flat_tris.get();
+    v.flat_root = 0;
+    return v;
+}
+
+bool is_identity(const float (&m)[3][4]) {
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++)
+            if (m[r][c] != (r == c ? 1.0f : 0.0f)) return false;
+    return true;
+}
 
 }  // namespace
 
@@ -73,6 +98,7 @@ struct rt_ctx {
     std::vector<rt_texture> tex_cache;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     WavefrontState* wavefront = nullptr;
+    bool flatten = true;  // RT_OPTION_FLATTEN_IDENTITY_INSTANCES
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
 };
@@ -167,8 +193,7 @@ int make_frame(rt_ctx* ctx, const TlasEntry& tl, const rt_bindings* b, float* ac
                FrameArgs* fa) {
     const rt_ubo& u = *b->ubo;
     std::memset(fa, 0, sizeof(*fa));
-    fa->tlas = TlasView{(const WideNode*)tl.nodes.get(), (const InstanceRecord*)tl.inst.get(), 0u, tl.num_inst,
-                        ctx->d_err};
+    fa->tlas = tlas_view(tl, ctx->d_err);
     SceneView& sv = fa->sv;
     sv.vertices = b->vertices;
     sv.indices = b->indices;
@@ -430,6 +455,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
     }
     TlasEntry te;
     std::vector<InstanceRecord> recs(count);
+    std::vector<uint32_t> flat_prims(count, 0u);  // > 0: instance goes to the flattened BVH
     for (uint32_t i = 0; i < count; i++) {
         const rt_instance& in = hi[i];
         auto it = ctx->blas_by_addr.find(in.acceleration_structure_reference);
@@ -451,6 +477,9 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
         }
         for (int k = 0; k < 12; k++) r.w2o_d[k] = w[k];
         r.blas = be.d_desc;
+        r.nodes = be.h_desc.nodes;
+        r.tris = be.h_desc.tris;
+        r.root = be.h_desc.root;
         r.custom_index = in.custom_index_and_mask & 0xFFFFFFu;
         r.mask = in.custom_index_and_mask >> 24;
         uint32_t f = in.sbt_offset_and_flags >> 24;
@@ -460,24 +489,47 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
                   (be.h_desc.num_prims == 0 ? kInstEmpty : 0u);
         r.instance_id = i;
         if (!(be.h_desc.flags & kBlasOpaque)) te.any_non_opaque = true;
+        // identity instances that rays can hit are traversed in one world-space BVH
+        if (ctx->flatten && is_identity(in.transform) && r.mask != 0u && be.h_desc.num_prims > 0)
+            flat_prims[i] = be.h_desc.num_prims;
     }
+    std::vector<uint32_t> tlas_ids, flat_ids;
+    uint64_t flat_total = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (flat_prims[i]) {
+            flat_ids.push_back(i);
+            flat_total += flat_prims[i];
+        } else {
+            tlas_ids.push_back(i);
+        }
+    }
+    if (flat_total >= (1ull << kLeafCountShift)) {
+        // too large for one flattened build: keep every instance in the TLAS
+        flat_ids.clear();
+        flat_total = 0;
+        tlas_ids.resize(count);
+        for (uint32_t i = 0; i < count; i++) tlas_ids[i] = i;
+    }
+    const uint32_t nt = uint32_t(tlas_ids.size()), nf = uint32_t(flat_ids.size());
+    std::vector<InstanceRecord> trecs(nt);
+    for (uint32_t k = 0; k < nt; k++) trecs[k] = recs[tlas_ids[k]];
     hipError_t e;
-    DevPtr d_rec(dev_alloc<InstanceRecord>(count + 1, &e));
+    DevPtr d_rec(dev_alloc<InstanceRecord>(nt + 1, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-    DevPtr d_lo(dev_alloc<float4>(count + 1, &e));
+    DevPtr d_lo(dev_alloc<float4>(nt + 1, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-    DevPtr d_hi(dev_alloc<float4>(count + 1, &e));
+    DevPtr d_hi(dev_alloc<float4>(nt + 1, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-    std::vector<uint32_t> seg_start = {0u, count};
+    std::vector<uint32_t> seg_start = {0u, nt};
     DevPtr d_seg(dev_alloc<uint32_t>(2, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-    if (count) RT_HIP(ctx, hipMemcpyAsync(d_rec.get(), recs.data(), sizeof(InstanceRecord) * count, hipMemcpyHostToDevice, st), "copy records");
+    if (nt) RT_HIP(ctx, hipMemcpyAsync(d_rec.get(), trecs.data(), sizeof(InstanceRecord) * nt, hipMemcpyHostToDevice, st), "copy records");
     RT_HIP(ctx, hipMemcpyAsync(d_seg.get(), seg_start.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, st), "copy segments");
-    int rc = prepare_instances(st, (const InstanceRecord*)d_rec.get(), count, (float4*)d_lo.get(), (float4*)d_hi.get());
+    int rc = prepare_instances(st, (const InstanceRecord*)d_rec.get(), nt, (float4*)d_lo.get(), (float4*)d_hi.get());
     if (rc) return hip_fail(ctx, hipError_t(rc), "prepare_instances");
     LbvhResult res;
     std::string err;
-    rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), count, (const uint32_t*)d_seg.get(),
+    rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), nt, (const uint32_t*)d_seg.get(),
                     seg_start, kTlasLeafMax, &res, &err);
     if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "TLAS build: " + err);
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
@@ -486,10 +538,62 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
     te.inst.reset(dev_alloc<InstanceRecord>(count + 1, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
     te.num_inst = count;
-    rc = reorder_instances(st, (const InstanceRecord*)d_rec.get(), (const uint32_t*)perm.get(), count,
+    te.num_tlas = nt;
+    // records: TLAS leaf order first, then the flattened instances in caller order
+    rc = reorder_instances(st, (const InstanceRecord*)d_rec.get(), (const uint32_t*)perm.get(), nt,
                            (InstanceRecord*)te.inst.get());
     if (rc) return hip_fail(ctx, hipError_t(rc), "reorder_instances");
+    std::vector<InstanceRecord> frecs(nf);
+    std::vector<FlatSource> fsrc(nf);
+    uint32_t pos = 0;
+    for (uint32_t k = 0; k < nf; k++) {
+        const InstanceRecord& r = recs[flat_ids[k]];
+        frecs[k] = r;
+        const BlasEntry& be = ctx->blas.at(ctx->blas_by_addr.at(uint64_t(uintptr_t(r.blas))));
+        bool opq = (r.flags & kInstOpaqueGeom) != 0;
+        if (r.flags & kInstForceOpaque) opq = true;
+        if (r.flags & kInstForceNoOpaque) opq = false;
+        fsrc[k] = FlatSource{be.h_desc.tris, be.h_desc.first_prim, pos, (nt + k) | (opq ? kFlatOpaque : 0u), 0u};
+        pos += be.h_desc.num_prims;
+    }
+    if (nf)
+        RT_HIP(ctx, hipMemcpyAsync((InstanceRecord*)te.inst.get() + nt, frecs.data(), sizeof(InstanceRecord) * nf,
+                                   hipMemcpyHostToDevice, st), "copy records");
     RT_HIP(ctx, hipMemcpyAsync(te.bounds, seg_bounds.get(), sizeof(te.bounds), hipMemcpyDeviceToHost, st), "copy bounds");
+    if (nf) {
+        const uint32_t n = uint32_t(flat_total);
+        DevPtr d_src(dev_alloc<FlatSource>(nf, &e));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+        DevPtr d_tri_in(dev_alloc<float4>(3 * size_t(n) + 3, &e));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+        DevPtr d_flo(dev_alloc<float4>(n + 1, &e));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+        DevPtr d_fhi(dev_alloc<float4>(n + 1, &e));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+        DevPtr d_fseg(dev_alloc<uint32_t>(2, &e));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+        std::vector<uint32_t> fseg = {0u, n};
+        RT_HIP(ctx, hipMemcpyAsync(d_src.get(), fsrc.data(), sizeof(FlatSource) * nf, hipMemcpyHostToDevice, st), "copy sources");
+        RT_HIP(ctx, hipMemcpyAsync(d_fseg.get(), fseg.data(), sizeof(uint32_t) * 2, hipMemcpyHostToDevice, st), "copy segments");
+        rc = gather_flat(st, (const FlatSource*)d_src.get(), nf, n, (float4*)d_tri_in.get(), (float4*)d_flo.get(),
+                         (float4*)d_fhi.get());
+        if (rc) return hip_fail(ctx, hipError_t(rc), "gather_flat");
+        LbvhResult fres;
+        rc = lbvh_build(st, (const float4*)d_flo.get(), (const float4*)d_fhi.get(), n, (const uint32_t*)d_fseg.get(),
+                        fseg, kBlasLeafMax, &fres, &err);
+        if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "flattened BVH build: " + err);
+        DevPtr fperm(fres.perm), fseg_bounds(fres.seg_bounds);
+        te.flat_nodes.reset(fres.nodes);
+        te.flat_num_nodes = fres.num_nodes;
+        te.flat_tris.reset(dev_alloc<float4>(3 * size_t(n) + 3, &e));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+        te.flat_num_tris = n;
+        rc = reorder_triangles(st, (const float4*)d_tri_in.get(), (const uint32_t*)fperm.get(), n,
+                               (float4*)te.flat_tris.get());
+        if (rc) return hip_fail(ctx, hipError_t(rc), "reorder_triangles");
+        RT_HIP(ctx, hipMemcpyAsync(te.flat_bounds, fseg_bounds.get(), sizeof(te.flat_bounds), hipMemcpyDeviceToHost, st), "copy bounds");
+        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
     RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
     uint64_t hnd = ctx->next_handle++;
     ctx->tlas.emplace(hnd, std::move(te));
@@ -517,8 +621,24 @@ int rt_tlas_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out) {
     out->root = 0;
     out->num_nodes = t.num_nodes;
     out->first_prim = 0;
-    out->num_prims = t.num_inst;
+    out->num_prims = t.num_tlas;
     std::memcpy(out->bounds, t.bounds, sizeof(out->bounds));
+    return RT_OK;
+}
+
+int rt_tlas_flat_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!out) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out is NULL");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    const TlasEntry& t = it->second;
+    out->nodes = uint64_t(uintptr_t(t.flat_nodes.get()));
+    out->tris = uint64_t(uintptr_t(t.flat_tris.get()));
+    out->root = 0;
+    out->num_nodes = t.flat_num_nodes;
+    out->first_prim = 0;
+    out->num_prims = t.flat_num_tris;
+    std::memcpy(out->bounds, t.flat_bounds, sizeof(out->bounds));
     return RT_OK;
 }
 
@@ -599,6 +719,11 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
         ctx->timer.used = 0;
         return RT_OK;
     }
+    if (option == RT_OPTION_FLATTEN_IDENTITY_INSTANCES) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_FLATTEN_IDENTITY_INSTANCES takes 0 or 1");
+        ctx->flatten = value != 0;
+        return RT_OK;
+    }
     return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown option");
 }
 
@@ -635,8 +760,7 @@ int rt_trace_ray_queries(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const 
     DeviceGuard g(ctx->device);
     FrameArgs fa;
     std::memset(&fa, 0, sizeof(fa));
-    fa.tlas = TlasView{(const WideNode*)it->second.nodes.get(), (const InstanceRecord*)it->second.inst.get(), 0u,
-                       it->second.num_inst, ctx->d_err};
+    fa.tlas = tlas_view(it->second, ctx->d_err);
     uint32_t seed_b = 0;
     if (b) {
         int rc = upload_textures(ctx, b);

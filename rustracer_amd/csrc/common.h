@@ -6,9 +6,10 @@
 //              loads), child references and leaf counts.
 //   Triangle    48 B: three float4 {v.xyz, w}; v0.w carries the primitive index
 //              (gl_PrimitiveID) as bits.  Leaves reference contiguous runs.
-//   InstanceRecord 256 B: world->object and object->world 3x4 rows (fp32), the
-//              BLAS descriptor address, custom index, mask, flags, instance id,
-//              and world->object in fp64 for hit refinement.
+//   InstanceRecord 256 B: world->object rows, BLAS node/triangle pools and root,
+//              mask, flags and instance id first (what traversal reads), then
+//              object->world rows, the BLAS descriptor, custom index and
+//              world->object in fp64 for hit refinement.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,27 +53,43 @@ enum : uint32_t {
 };
 
 struct alignas(16) InstanceRecord {
-    float4 w2o[3];        // rows of the world-to-object 3x4 matrix (traversal)
-    float4 o2w[3];        // rows of the object-to-world 3x4 matrix (gl_ObjectToWorldEXT)
+    // first 80 bytes: what traversal reads when it enters the instance
+    float4 w2o[3];          // rows of the world-to-object 3x4 matrix
+    const WideNode* nodes;  // BLAS node pool
+    const float4* tris;     // BLAS triangle pool
+    uint32_t root;          // BLAS root node
+    uint32_t mask;          // 8-bit visibility mask
+    uint32_t flags;         // kInst*
+    uint32_t instance_id;   // gl_InstanceID (index in the caller's instance array)
+    // shading
+    float4 o2w[3];          // rows of the object-to-world 3x4 matrix (gl_ObjectToWorldEXT)
     const BlasDesc* blas;
     uint32_t custom_index;
-    uint32_t mask;        // 8-bit visibility mask
-    uint32_t flags;       // kInst*
-    uint32_t instance_id; // gl_InstanceID (index in the caller's instance array)
-    uint32_t _pad[2];
-    double w2o_d[12];     // world-to-object in fp64 (hit refinement)
-    uint32_t _pad2[8];
+    uint32_t _pad;
+    double w2o_d[12];       // world-to-object in fp64 (hit refinement)
+    uint32_t _pad2[4];
 };
 static_assert(sizeof(InstanceRecord) == 256, "InstanceRecord is 256 bytes");
 
 enum : uint32_t { kErrStackOverflow = 1u };
 
+// Flattened BVH: instances whose transform is the identity are not entered through
+// the TLAS; their triangles are copied into one BVH in world space whose triangle
+// records carry the instance (v1.w: instance index | kFlatOpaque, v2.w: the
+// record's index in the BLAS pool, v0.w: primitive index as in every record).
+constexpr uint32_t kFlatOpaque = 0x80000000u;
+
 struct TlasView {
     const WideNode* nodes;
-    const InstanceRecord* inst;  // in TLAS leaf order
+    const InstanceRecord* inst;  // TLAS leaf order, then the flattened instances
     uint32_t root;
-    uint32_t num_inst;
+    uint32_t num_inst;           // records in inst
     uint32_t* error_flags;       // kErr* bits raised by kernels (may be NULL)
+    uint32_t num_tlas;           // instances reachable through the TLAS (0: skip it)
+    const WideNode* flat_nodes;  // flattened BVH (NULL: none)
+    const float4* flat_tris;
+    uint32_t flat_root;
+    uint32_t _pad;
 };
 
 // ------------------------------------------------------------ float3 math

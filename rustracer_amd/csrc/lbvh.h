@@ -53,6 +53,23 @@ int prepare_triangles(hipStream_t stream, const TriSegment* d_segs, uint32_t nse
 int reorder_triangles(hipStream_t stream, const float4* d_tri_in, const uint32_t* d_perm, uint32_t n,
                       float4* d_tri_out);
 
+// One flattened instance: `count` triangle records of a BLAS pool starting at
+// record `first_rec`, placed from flattened triangle `start` on and tagged with
+// `inst_bits` (instance index | kFlatOpaque).
+struct FlatSource {
+    const float4* tris;
+    uint32_t first_rec;
+    uint32_t start;
+    uint32_t inst_bits;
+    uint32_t _pad;
+};
+
+// Copies the triangle records of identity instances into one array (input order of
+// a flattened BVH build) with their boxes; records get v1.w = inst_bits and
+// v2.w = source record index.
+int gather_flat(hipStream_t stream, const FlatSource* d_src, uint32_t nsrc, uint32_t n, float4* d_tri, float4* d_lo,
+                float4* d_hi);
+
 // Instance boxes (world space) for the TLAS build.
 int prepare_instances(hipStream_t stream, const InstanceRecord* d_inst, uint32_t n, float4* d_lo,
                       float4* d_hi);

@@ -89,6 +89,27 @@ __global__ void k_reorder_tris(const float4* in, const uint32_t* perm, uint32_t 
     out[3 * size_t(p) + 2] = in[3 * size_t(s) + 2];
 }
 
+__global__ void k_gather_flat(const FlatSource* src, uint32_t nsrc, uint32_t n, float4* tri, float4* lo, float4* hi) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t a = 0, b = nsrc;
+    while (b - a > 1) {
+        uint32_t m = (a + b) >> 1;
+        if (src[m].start <= p) a = m; else b = m;
+    }
+    const FlatSource& s = src[a];
+    const uint32_t rec = s.first_rec + (p - s.start);
+    const float4* t = s.tris + 3 * size_t(rec);
+    float4 v0 = t[0], v1 = t[1], v2 = t[2];
+    tri[3 * size_t(p) + 0] = v0;  // w: primitive index
+    tri[3 * size_t(p) + 1] = make_float4(v1.x, v1.y, v1.z, __uint_as_float(s.inst_bits));
+    tri[3 * size_t(p) + 2] = make_float4(v2.x, v2.y, v2.z, __uint_as_float(rec));
+    float3 l = fmin3(fmin3(xyz(v0), xyz(v1)), xyz(v2));
+    float3 h = fmax3(fmax3(xyz(v0), xyz(v1)), xyz(v2));
+    lo[p] = make_float4(l.x, l.y, l.z, 0.f);
+    hi[p] = make_float4(h.x, h.y, h.z, 0.f);
+}
+
 __global__ void k_prepare_instances(const InstanceRecord* inst, uint32_t n, float4* lo, float4* hi) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -592,6 +613,13 @@ int prepare_triangles(hipStream_t st, const TriSegment* d_segs, uint32_t nseg, u
 int reorder_triangles(hipStream_t st, const float4* in, const uint32_t* perm, uint32_t n, float4* out) {
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_reorder_tris, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, in, perm, n, out);
+    RT_CHECK_LAUNCH();
+    return 0;
+}
+
+int gather_flat(hipStream_t st, const FlatSource* src, uint32_t nsrc, uint32_t n, float4* tri, float4* lo, float4* hi) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_gather_flat, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, src, nsrc, n, tri, lo, hi);
     RT_CHECK_LAUNCH();
     return 0;
 }

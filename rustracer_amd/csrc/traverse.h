@@ -10,9 +10,9 @@
 //
 // Stack: a per-lane ring of kStackDepth entries in LDS (one column per lane, so a
 // wave's pushes hit 64 distinct banks).  When the ring overwrites an entry the
-// traversal restarts from the TLAS root with the current closest t once the ring
-// drains: closest-hit and shadow results stay exact; only the number of any-hit
-// invocations can change.  After kMaxRestarts restarts the ray gives up and raises
+// traversal restarts from the top (flattened BVH, then TLAS root) with the current
+// closest t once the ring drains: closest-hit and shadow results stay exact; only
+// the number of any-hit invocations can change.  After kMaxRestarts restarts the ray gives up and raises
 // kErrStackOverflow, so every traversal ends.
 #pragma once
 
@@ -34,24 +34,24 @@ struct HitInfo {
     uint32_t rec;   // triangle record index in the BLAS pool
 };
 
-// Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2013).
+// Watertight ray/triangle test (Woop, Benthin & Wald, JCGT 2013).  The ray origin
+// is passed separately (it is the traversal ray's origin).
 struct WoopRay {
-    float3 o;
-    int kx, ky, kz;
+    uint32_t k;  // kx | ky << 2 | kz << 4
     float sx, sy, sz;
 };
 
-__device__ inline WoopRay woop_setup(float3 o, float3 d) {
-    WoopRay w;
-    w.o = o;
+__device__ inline WoopRay woop_setup(float3 d) {
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    w.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-    w.kx = w.kz == 2 ? 0 : w.kz + 1;
-    w.ky = w.kx == 2 ? 0 : w.kx + 1;
-    float dz = comp(d, w.kz);
-    if (dz < 0.f) { int t = w.kx; w.kx = w.ky; w.ky = t; }
-    w.sx = comp(d, w.kx) / dz;
-    w.sy = comp(d, w.ky) / dz;
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp(d, kz);
+    if (dz < 0.f) { int t = kx; kx = ky; ky = t; }
+    WoopRay w;
+    w.k = uint32_t(kx) | uint32_t(ky) << 2 | uint32_t(kz) << 4;
+    w.sx = comp(d, kx) / dz;
+    w.sy = comp(d, ky) / dz;
     w.sz = 1.0f / dz;
     return w;
 }
@@ -61,13 +61,14 @@ __device__ inline WoopRay woop_setup(float3 o, float3 d) {
 // edge's function evaluates to exactly the negated value in the neighbouring
 // triangle, which a fused a*b - c*d does not guarantee.  (The sheared vertex
 // coordinates may be fused: a shared vertex gets the same value in both triangles.)
-__device__ inline bool woop_hit(const WoopRay& w, float3 v0, float3 v1, float3 v2, float tmin, float tmax, float& t,
-                                float& bu, float& bv) {
-    float3 A = v0 - w.o, B = v1 - w.o, C = v2 - w.o;
-    float Akz = comp(A, w.kz), Bkz = comp(B, w.kz), Ckz = comp(C, w.kz);
-    float Ax = comp(A, w.kx) - w.sx * Akz, Ay = comp(A, w.ky) - w.sy * Akz;
-    float Bx = comp(B, w.kx) - w.sx * Bkz, By = comp(B, w.ky) - w.sy * Bkz;
-    float Cx = comp(C, w.kx) - w.sx * Ckz, Cy = comp(C, w.ky) - w.sy * Ckz;
+__device__ inline bool woop_hit(const WoopRay& w, float3 o, float3 v0, float3 v1, float3 v2, float tmin, float tmax,
+                                float& t, float& bu, float& bv) {
+    const int kx = int(w.k & 3u), ky = int((w.k >> 2) & 3u), kz = int(w.k >> 4);
+    float3 A = v0 - o, B = v1 - o, C = v2 - o;
+    float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
+    float Ax = comp(A, kx) - w.sx * Akz, Ay = comp(A, ky) - w.sy * Akz;
+    float Bx = comp(B, kx) - w.sx * Bkz, By = comp(B, ky) - w.sy * Bkz;
+    float Cx = comp(C, kx) - w.sx * Ckz, Cy = comp(C, ky) - w.sy * Ckz;
     float U, V, W;
     {
 #pragma clang fp contract(off)
@@ -93,7 +94,7 @@ __device__ inline bool woop_hit(const WoopRay& w, float3 v0, float3 v1, float3 v
 }
 
 struct TravRay {
-    float3 o, d, inv;
+    float3 o, inv;
 };
 
 __device__ inline float safe_inv(float x) {
@@ -132,8 +133,14 @@ struct NoAnyHit {
     __device__ bool ignore(uint32_t, uint32_t, float, float) { return false; }
 };
 
+enum : int { kLevelTlas = 0, kLevelBlas = 1, kLevelFlat = 2 };
+
 // Closest-hit traversal.  `ah.ignore(inst_index, prim, u, v)` is the any-hit
 // shader (true = ignoreIntersectionEXT).  Returns true on a hit.
+//
+// The nearest hit child of a node is kept in a register and only the others go to
+// the stack.  The flattened BVH (identity instances) is traversed first, with the
+// TLAS root beneath it on the stack, so its hits bound the TLAS traversal.
 template <bool kAnyHit, bool kStats, class AnyHit>
 __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, float tmax, uint32_t ray_flags,
                           AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit, TraceStats& st) {
@@ -142,14 +149,13 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
     bool found = false;
     hit.inst = kEmpty;
     hit.prim = kEmpty;
-    TravRay world{o, d, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))};
-    TravRay cur = world;
+    TravRay cur{o, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))};
     WoopRay wr;
     const WideNode* nodes = tl.nodes;
-    const float4* tris = nullptr;
+    const float4* tris = tl.flat_tris;
     uint32_t inst_idx = 0;
     bool inst_opaque = true;
-    bool in_blas = false;
+    int level = kLevelTlas;
     uint32_t blas_base = 0;
     uint32_t sp = 0, base = 0;
     bool lost = false;
@@ -162,8 +168,128 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
             lost = true;
         }
     };
-    push(tl.root);
+    auto to_world = [&]() {
+        cur = TravRay{o, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))};
+    };
+    auto enter = [&]() -> uint32_t {
+        to_world();
+        if (tl.flat_nodes) {
+            if (tl.num_tlas) push(tl.root);
+            blas_base = sp;
+            level = kLevelFlat;
+            nodes = tl.flat_nodes;
+            tris = tl.flat_tris;
+            wr = woop_setup(d);
+            return tl.flat_root;
+        }
+        level = kLevelTlas;
+        nodes = tl.nodes;
+        return tl.root;
+    };
+    uint32_t e = enter();
     for (;;) {
+        if (!(e & kLeafBit)) {
+            // internal node
+            if (kStats) st.nodes++;
+            WideNode n = load_node(nodes, e);
+            float tn[4];
+            uint32_t mask = node_hits(n, cur, tmin, best_t, tn);
+            if (mask) {
+                // order the hit children near-to-far: continue with the nearest,
+                // push the others far first
+                float key[4];
+                uint32_t ent[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    bool h = (mask >> k) & 1u;
+                    key[k] = h ? tn[k] : INFINITY;
+                    uint32_t cnt = n.count[k];
+                    ent[k] = h ? (cnt ? (kLeafBit | ((cnt - 1u) << kLeafCountShift) | n.child[k]) : n.child[k]) : kEmpty;
+                }
+#define RT_CSWAP(i, j)                                                         \
+    if (key[j] < key[i]) {                                                     \
+        float tk = key[i]; key[i] = key[j]; key[j] = tk;                       \
+        uint32_t te = ent[i]; ent[i] = ent[j]; ent[j] = te;                    \
+    }
+                RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+#pragma unroll
+                for (int k = 3; k >= 1; k--)
+                    if (ent[k] != kEmpty) push(ent[k]);
+                e = ent[0];
+                continue;
+            }
+        } else if (level == kLevelTlas) {
+            // TLAS leaf: one instance
+            const uint32_t first = e & kLeafFirstMask;
+            const InstanceRecord& ir = tl.inst[first];
+            if (kStats) st.instances++;
+            const uint32_t fl = ir.flags;
+            if ((ir.mask & 0xFFu) != 0u && !(fl & kInstEmpty)) {
+                float4 r0 = ir.w2o[0], r1 = ir.w2o[1], r2 = ir.w2o[2];
+                float3 oo = mk3(r0.x * o.x + r0.y * o.y + r0.z * o.z + r0.w,
+                                r1.x * o.x + r1.y * o.y + r1.z * o.z + r1.w,
+                                r2.x * o.x + r2.y * o.y + r2.z * o.z + r2.w);
+                float3 od = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+                                r2.x * d.x + r2.y * d.y + r2.z * d.z);
+                cur.o = oo;
+                cur.inv = mk3(safe_inv(od.x), safe_inv(od.y), safe_inv(od.z));
+                wr = woop_setup(od);
+                nodes = ir.nodes;
+                tris = ir.tris;
+                inst_idx = first;
+                bool opq = (fl & kInstOpaqueGeom) != 0;
+                if (fl & kInstForceOpaque) opq = true;
+                if (fl & kInstForceNoOpaque) opq = false;
+                if (ray_flags & RT_RAY_FLAG_OPAQUE) opq = true;
+                if (ray_flags & RT_RAY_FLAG_NO_OPAQUE) opq = false;
+                inst_opaque = opq;
+                level = kLevelBlas;
+                blas_base = sp;
+                e = ir.root;
+                continue;
+            }
+        } else {
+            // triangle leaf of a BLAS or of the flattened BVH
+            const uint32_t first = e & kLeafFirstMask;
+            const uint32_t count = ((e >> kLeafCountShift) & 15u) + 1u;
+            for (uint32_t k = 0; k < count; k++) {
+                const float4* tp = tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                if (kStats) st.tris++;
+                float t, bu, bv;
+                if (!woop_hit(wr, cur.o, xyz(a), xyz(b), xyz(c), tmin, best_t, t, bu, bv)) continue;
+                const uint32_t prim = __float_as_uint(a.w);
+                uint32_t ii = inst_idx, rec = first + k;
+                bool opq = inst_opaque;
+                if (level == kLevelFlat) {
+                    const uint32_t bits = __float_as_uint(b.w);
+                    ii = bits & ~kFlatOpaque;
+                    rec = __float_as_uint(c.w);
+                    opq = (bits & kFlatOpaque) != 0u;
+                    if (ray_flags & RT_RAY_FLAG_OPAQUE) opq = true;
+                    if (ray_flags & RT_RAY_FLAG_NO_OPAQUE) opq = false;
+                }
+                if (found && t == best_t) {
+                    uint32_t hi = tl.inst[ii].instance_id, ho = tl.inst[hit.inst].instance_id;
+                    if (hi > ho || (hi == ho && prim > hit.prim)) continue;
+                }
+                if (kAnyHit && !opq) {
+                    if (kStats) st.anyhits++;
+                    if (ah.ignore(ii, prim, bu, bv)) continue;
+                }
+                found = true;
+                best_t = t;
+                hit.t = t;
+                hit.u = bu;
+                hit.v = bv;
+                hit.inst = ii;
+                hit.prim = prim;
+                hit.rec = rec;
+                if (terminate) return true;
+            }
+        }
+        // next entry from the stack
         if (sp == base) {
             if (!lost) break;
             lost = false;
@@ -172,108 +298,16 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
                 if (tl.error_flags) atomicOr(tl.error_flags, kErrStackOverflow);
                 break;
             }
-            if (in_blas) {
-                in_blas = false;
-                cur = world;
-                nodes = tl.nodes;
-            }
-            push(tl.root);
+            e = enter();
             continue;
         }
         sp--;
-        uint32_t e = lds_stack[(sp & (kStackDepth - 1)) * stride];
-        if (in_blas && sp < blas_base) {
-            in_blas = false;
-            cur = world;
+        e = lds_stack[(sp & (kStackDepth - 1)) * stride];
+        if (level != kLevelTlas && sp < blas_base) {
+            level = kLevelTlas;
+            to_world();
             nodes = tl.nodes;
         }
-        if (e & kLeafBit) {
-            uint32_t first = e & kLeafFirstMask;
-            uint32_t count = ((e >> kLeafCountShift) & 15u) + 1u;
-            if (!in_blas) {
-                // TLAS leaf: one instance
-                const InstanceRecord& ir = tl.inst[first];
-                if (kStats) st.instances++;
-                uint32_t mf = ir.mask;
-                uint32_t fl = ir.flags;
-                if ((mf & 0xFFu) == 0u || (fl & kInstEmpty)) continue;
-                const BlasDesc* bd = ir.blas;
-                float4 r0 = ir.w2o[0], r1 = ir.w2o[1], r2 = ir.w2o[2];
-                float3 oo = mk3(r0.x * o.x + r0.y * o.y + r0.z * o.z + r0.w, r1.x * o.x + r1.y * o.y + r1.z * o.z + r1.w,
-                                r2.x * o.x + r2.y * o.y + r2.z * o.z + r2.w);
-                float3 od = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
-                                r2.x * d.x + r2.y * d.y + r2.z * d.z);
-                cur.o = oo;
-                cur.d = od;
-                cur.inv = mk3(safe_inv(od.x), safe_inv(od.y), safe_inv(od.z));
-                wr = woop_setup(oo, od);
-                nodes = bd->nodes;
-                tris = bd->tris;
-                inst_idx = first;
-                bool opq = (fl & kInstOpaqueGeom) != 0;
-                if (fl & kInstForceOpaque) opq = true;
-                if (fl & kInstForceNoOpaque) opq = false;
-                if (ray_flags & RT_RAY_FLAG_OPAQUE) opq = true;
-                if (ray_flags & RT_RAY_FLAG_NO_OPAQUE) opq = false;
-                inst_opaque = opq;
-                in_blas = true;
-                blas_base = sp;
-                push(bd->root);
-                continue;
-            }
-            for (uint32_t k = 0; k < count; k++) {
-                const float4* tp = tris + 3 * size_t(first + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                if (kStats) st.tris++;
-                float t, bu, bv;
-                if (!woop_hit(wr, xyz(a), xyz(b), xyz(c), tmin, best_t, t, bu, bv)) continue;
-                uint32_t prim = __float_as_uint(a.w);
-                if (found && t == best_t) {
-                    uint32_t hi = tl.inst[inst_idx].instance_id, ho = tl.inst[hit.inst].instance_id;
-                    if (hi > ho || (hi == ho && prim > hit.prim)) continue;
-                }
-                if (kAnyHit && !inst_opaque) {
-                    if (kStats) st.anyhits++;
-                    if (ah.ignore(inst_idx, prim, bu, bv)) continue;
-                }
-                found = true;
-                best_t = t;
-                hit.t = t;
-                hit.u = bu;
-                hit.v = bv;
-                hit.inst = inst_idx;
-                hit.prim = prim;
-                hit.rec = first + k;
-                if (terminate) return true;
-            }
-            continue;
-        }
-        // internal node
-        if (kStats) st.nodes++;
-        WideNode n = load_node(nodes, e);
-        float tn[4];
-        uint32_t mask = node_hits(n, cur, tmin, best_t, tn);
-        if (!mask) continue;
-        // order the hit children near-to-far; push far first
-        float key[4];
-        uint32_t ent[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            bool h = (mask >> k) & 1u;
-            key[k] = h ? tn[k] : INFINITY;
-            uint32_t cnt = n.count[k];
-            ent[k] = h ? (cnt ? (kLeafBit | ((cnt - 1u) << kLeafCountShift) | n.child[k]) : n.child[k]) : kEmpty;
-        }
-#define RT_CSWAP(i, j)                                                         \
-    if (key[j] < key[i]) {                                                     \
-        float tk = key[i]; key[i] = key[j]; key[j] = tk;                       \
-        uint32_t te = ent[i]; ent[i] = ent[j]; ent[j] = te;                    \
-    }
-        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
-#undef RT_CSWAP
-#pragma unroll
-        for (int k = 3; k >= 0; k--)
-            if (ent[k] != kEmpty) push(ent[k]);
     }
     return found;
 }

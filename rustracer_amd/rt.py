@@ -37,7 +37,7 @@ RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT = 0x4, 0x8
 EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", "rt_build_blas_batch",
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
-           "rt_set_option", "rt_get_stage_times"]
+           "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info"]
 
 
 class RtError(RuntimeError):
@@ -75,6 +75,7 @@ class TraceStats(C.Structure):
 
 
 OPTION_STAGE_TIMING = 1
+OPTION_FLATTEN_IDENTITY_INSTANCES = 2
 
 
 class StageTimes(C.Structure):
@@ -110,6 +111,7 @@ def load_library(path=LIB_PATH):
         "rt_trace_primary": (i, [vp, u64, vp, u, u, vp]),
         "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
         "rt_set_option": (i, [vp, u, u64]), "rt_get_stage_times": (i, [vp, vp]),
+        "rt_tlas_flat_info": (i, [vp, u64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -216,6 +218,12 @@ class TopAS:
     def info(self):
         i = BvhInfo()
         self.ctx.check(self.ctx.lib.rt_tlas_info(self.ctx.handle, self.handle, C.byref(i)))
+        return i
+
+    def flat_info(self):
+        """rt_tlas_flat_info: the world-space BVH over the identity instances."""
+        i = BvhInfo()
+        self.ctx.check(self.ctx.lib.rt_tlas_flat_info(self.ctx.handle, self.handle, C.byref(i)))
         return i
 
     def __del__(self):

@@ -12,12 +12,16 @@ MISS = 0xFFFFFFFF
 
 
 class DeviceScene:
-    def __init__(self, ctx, scene):
+    def __init__(self, ctx, scene, flatten=True):
         import torch
         self.scene = scene
         self.buffers = rt.Buffers(scene)
         torch.cuda.synchronize()
-        self.blases, self.tlas = rt.create_as(ctx, scene, self.buffers)
+        ctx.set_option(rt.OPTION_FLATTEN_IDENTITY_INSTANCES, int(flatten))
+        try:
+            self.blases, self.tlas = rt.create_as(ctx, scene, self.buffers)
+        finally:
+            ctx.set_option(rt.OPTION_FLATTEN_IDENTITY_INSTANCES, 1)
         ctx.sync()
 
 
@@ -29,11 +33,13 @@ def host_scene(name):
 _dev = {}
 
 
-def device_scene(ctx, name, factory=None):
-    key = (id(ctx), name)
+def device_scene(ctx, name, factory=None, flatten=True):
+    """Buffers + BLASes + TLAS of a named scene, cached per context.  flatten=False
+    builds the TLAS with every instance entered through it (two-level traversal)."""
+    key = (id(ctx), name, flatten)
     if key not in _dev:
         sc = factory() if factory is not None else host_scene(name)
-        _dev[key] = DeviceScene(ctx, sc)
+        _dev[key] = DeviceScene(ctx, sc, flatten)
     return _dev[key]
 
 

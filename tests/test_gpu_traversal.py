@@ -35,8 +35,9 @@ def _assert_parity(scene, rays, gh, oh, otuv, max_tie_frac=1e-3):
     assert np.abs(gh["u"][hit] - otuv[hit, 1]).max() < 1e-3 and np.abs(gh["v"][hit] - otuv[hit, 2]).max() < 1e-3
 
 
-def test_random_rays_cornell(gpu_ctx):
-    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+@pytest.mark.parametrize("flatten", [True, False], ids=["flat", "two_level"])
+def test_random_rays_cornell(gpu_ctx, flatten):
+    ds = G.device_scene(gpu_ctx, "cornell_sphere", flatten=flatten)
     rays = K.random_rays(50000, [-0.95, 0.05, -0.95], [0.95, 1.95, 3.0], seed=11)
     gh = _queries(gpu_ctx, ds, rays)
     oh, otuv = B.OracleScene(ds.scene).trace_rays(rays, brute=True)
@@ -52,8 +53,9 @@ def test_random_rays_instanced_affine(gpu_ctx):
     _assert_parity(ds.scene, rays, gh, oh, otuv)
 
 
-def test_random_rays_displaced_sphere(gpu_ctx):
-    ds = G.device_scene(gpu_ctx, "displaced_sphere")
+@pytest.mark.parametrize("flatten", [True, False], ids=["flat", "two_level"])
+def test_random_rays_displaced_sphere(gpu_ctx, flatten):
+    ds = G.device_scene(gpu_ctx, "displaced_sphere", flatten=flatten)
     rays = K.random_rays(30000, [-3, -3, -3], [3, 3, 3], seed=13, toward=(0, 0, 0))
     gh = _queries(gpu_ctx, ds, rays)
     oh, otuv = B.OracleScene(ds.scene).trace_rays(rays)
