@@ -172,7 +172,12 @@ enum {
      * transform is exactly the identity into one world-space BVH, traversed before
      * the TLAS over the other instances.  Hits report the same instance, primitive
      * and barycentrics as the two-level path.  Applies to TLASes built afterwards. */
-    RT_OPTION_FLATTEN_IDENTITY_INSTANCES = 2
+    RT_OPTION_FLATTEN_IDENTITY_INSTANCES = 2,
+    /* Wavefront traversal kernels: 0 (default) LDS stack ring of 32 entries,
+     * registers for 5 waves/SIMD; 1: 16 entries, 6 waves; 2: 16 entries, 8 waves;
+     * 3: 64 entries (LDS holds 2 waves/SIMD).  Results are identical; speed differs by scene (a
+     * ring too shallow for the BVH depth restarts traversals). */
+    RT_OPTION_TRAVERSAL_VARIANT = 3
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 

@@ -99,6 +99,7 @@ struct rt_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     WavefrontState* wavefront = nullptr;
     bool flatten = true;  // RT_OPTION_FLATTEN_IDENTITY_INSTANCES
+    int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
 };
@@ -677,7 +678,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
             if (!ctx->wavefront) return fail(ctx, RT_ERR_OUT_OF_MEMORY, "wavefront state");
         }
         std::string err;
-        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, tm, &err);
+        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, ctx->variant, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
@@ -717,6 +718,12 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_STAGE_TIMING takes 0 or 1");
         ctx->stage_timing = value != 0;
         ctx->timer.used = 0;
+        return RT_OK;
+    }
+    if (option == RT_OPTION_TRAVERSAL_VARIANT) {
+        if (value >= uint64_t(wavefront_num_variants()))
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_TRAVERSAL_VARIANT out of range");
+        ctx->variant = int(value);
         return RT_OK;
     }
     if (option == RT_OPTION_FLATTEN_IDENTITY_INSTANCES) {

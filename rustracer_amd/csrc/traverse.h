@@ -8,7 +8,7 @@
 // termination on the first accepted hit, no face culling.  Ties in t are broken
 // by the smaller (instance, primitive) so results do not depend on BVH shape.
 //
-// Stack: a per-lane ring of kStackDepth entries in LDS (one column per lane, so a
+// Stack: a per-lane ring of kDepth entries in LDS (one column per lane, so a
 // wave's pushes hit 64 distinct banks).  When the ring overwrites an entry the
 // traversal restarts from the top (flattened BVH, then TLAS root) with the current
 // closest t once the ring drains: closest-hit and shadow results stay exact; only
@@ -20,7 +20,7 @@
 
 namespace rt {
 
-constexpr int kStackDepth = 32;  // power of two
+constexpr int kStackDepth = 32;  // default ring depth (a power of two)
 constexpr uint32_t kMaxRestarts = 64;
 
 struct TraceStats {
@@ -141,9 +141,10 @@ enum : int { kLevelTlas = 0, kLevelBlas = 1, kLevelFlat = 2 };
 // The nearest hit child of a node is kept in a register and only the others go to
 // the stack.  The flattened BVH (identity instances) is traversed first, with the
 // TLAS root beneath it on the stack, so its hits bound the TLAS traversal.
-template <bool kAnyHit, bool kStats, class AnyHit>
+template <bool kAnyHit, bool kStats, int kDepth = kStackDepth, class AnyHit>
 __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, float tmax, uint32_t ray_flags,
                           AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit, TraceStats& st) {
+    static_assert((kDepth & (kDepth - 1)) == 0, "the stack ring depth is a power of two");
     const bool terminate = (ray_flags & RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT) != 0;
     float best_t = tmax;
     bool found = false;
@@ -161,9 +162,9 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
     bool lost = false;
     uint32_t restarts = 0;
     auto push = [&](uint32_t e) {
-        lds_stack[(sp & (kStackDepth - 1)) * stride] = e;
+        lds_stack[(sp & (kDepth - 1)) * stride] = e;
         sp++;
-        if (sp - base > uint32_t(kStackDepth)) {
+        if (sp - base > uint32_t(kDepth)) {
             base++;
             lost = true;
         }
@@ -302,7 +303,7 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
             continue;
         }
         sp--;
-        e = lds_stack[(sp & (kStackDepth - 1)) * stride];
+        e = lds_stack[(sp & (kDepth - 1)) * stride];
         if (level != kLevelTlas && sp < blas_base) {
             level = kLevelTlas;
             to_world();

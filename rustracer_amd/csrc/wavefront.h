@@ -15,7 +15,10 @@ void wavefront_destroy(WavefrontState* s);
 // megakernel.  Returns 0 or a hipError_t code (kWavefrontUnsupported when the
 // wavefront path is not available for this frame).
 constexpr int kWavefrontUnsupported = -1000;
-int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm,
-                     std::string* err);
+// `variant` selects the traversal kernels' LDS stack depth and register target
+// (0 .. wavefront_num_variants() - 1).
+int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, int variant,
+                     StageTimer* tm, std::string* err);
+int wavefront_num_variants();
 
 }  // namespace rt

@@ -14,6 +14,7 @@
 // persistent grid-stride loops that read their queue length from device memory,
 // so the host only checks for completion every few iterations.
 #include <algorithm>
+#include <unordered_map>
 #include <vector>
 
 #include "path.h"
@@ -123,10 +124,20 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_init(FrameArgs fa, WfArrays a, 
     }
 }
 
-template <bool kAnyHit, bool kStats>
-__global__ __launch_bounds__(kWfBlock) void k_wf_extend(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in,
-                                                         uint32_t* zero0, uint32_t* zero1) {
-    __shared__ uint32_t s_stack[kStackDepth * kWfBlock];
+// Traversal variants of the extend and shadow kernels: LDS stack depth and the
+// waves per SIMD the register allocation targets (the compiler may spill above
+// its natural occupancy; the any-hit kernels keep their natural allocation).
+// Selected per context (RT_OPTION_TRAVERSAL_VARIANT).
+struct TravVariant {
+    int depth, waves;
+};
+constexpr TravVariant kTravVariants[] = {{32, 5}, {16, 6}, {16, 8}, {64, 2}};
+constexpr int kNumTravVariants = int(sizeof(kTravVariants) / sizeof(kTravVariants[0]));
+
+template <bool kAnyHit, bool kStats, int kDepth, int kWaves>
+__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
+void k_wf_extend(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in, uint32_t* zero0, uint32_t* zero1) {
+    __shared__ uint32_t s_stack[kDepth * kWfBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *zero0 = 0u;  // next path queue
         *zero1 = 0u;  // shadow queue
@@ -144,8 +155,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_extend(FrameArgs fa, WfArrays a
         Rng4 rb{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
         AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
         HitInfo h;
-        bool found = trace_ray<kAnyHit, kStats>(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), flags, ah,
-                                                s_stack + threadIdx.x, kWfBlock, h, pc.tr);
+        bool found = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), flags, ah,
+                                                        s_stack + threadIdx.x, kWfBlock, h, pc.tr);
         if (kAnyHit) a.wb[s] = rb.w;
         a.h_inst[s] = found ? h.inst : kEmpty;
         a.h_prim[s] = h.prim;
@@ -199,9 +210,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(FrameArgs fa, WfArrays a,
     if (kStats) flush_counters(fa.counters, pc);
 }
 
-template <bool kAnyHit, bool kStats>
-__global__ __launch_bounds__(kWfBlock) void k_wf_shadow(FrameArgs fa, WfArrays a) {
-    __shared__ uint32_t s_stack[kStackDepth * kWfBlock];
+template <bool kAnyHit, bool kStats, int kDepth, int kWaves>
+__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
+void k_wf_shadow(FrameArgs fa, WfArrays a) {
+    __shared__ uint32_t s_stack[kDepth * kWfBlock];
     const uint32_t n = a.counts[2];
     const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
     PathCounters pc;
@@ -214,9 +226,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(FrameArgs fa, WfArrays a
         Rng4 shadow_rng{x, y, fa.seed_b, a.sq_wb[i]};
         AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
         HitInfo sh;
-        bool hit = trace_ray<kAnyHit, kStats>(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
-                                              float(kShadowTMin), o.w, flags, sah, s_stack + threadIdx.x, kWfBlock, sh,
-                                              pc.trs);
+        bool hit = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
+                                                      float(kShadowTMin), o.w, flags, sah, s_stack + threadIdx.x, kWfBlock,
+                                                      sh, pc.trs);
         a.unocc[s] = hit ? 0u : 1u;
     }
     if (kStats) flush_counters(fa.counters, pc);
@@ -296,13 +308,60 @@ __global__ void k_zero_counts(uint32_t* c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0u;
 }
 
+using ExtendFn = void (*)(FrameArgs, WfArrays, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
+using ShadowFn = void (*)(FrameArgs, WfArrays);
+
+// [variant][any_hit][stats]
+template <int V> struct TravKernels {
+    static constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
+    static ExtendFn extend(bool ah, bool st) {
+        return ah ? (st ? k_wf_extend<true, true, D, W> : k_wf_extend<true, false, D, W>)
+                  : (st ? k_wf_extend<false, true, D, W> : k_wf_extend<false, false, D, W>);
+    }
+    static ShadowFn shadow(bool ah, bool st) {
+        return ah ? (st ? k_wf_shadow<true, true, D, W> : k_wf_shadow<true, false, D, W>)
+                  : (st ? k_wf_shadow<false, true, D, W> : k_wf_shadow<false, false, D, W>);
+    }
+};
+
+ExtendFn extend_kernel(int v, bool ah, bool st) {
+    switch (v) {
+        case 1: return TravKernels<1>::extend(ah, st);
+        case 2: return TravKernels<2>::extend(ah, st);
+        case 3: return TravKernels<3>::extend(ah, st);
+        default: return TravKernels<0>::extend(ah, st);
+    }
+}
+ShadowFn shadow_kernel(int v, bool ah, bool st) {
+    switch (v) {
+        case 1: return TravKernels<1>::shadow(ah, st);
+        case 2: return TravKernels<2>::shadow(ah, st);
+        case 3: return TravKernels<3>::shadow(ah, st);
+        default: return TravKernels<0>::shadow(ah, st);
+    }
+}
+static_assert(kNumTravVariants == 4, "extend_kernel / shadow_kernel list every variant");
+
 }  // namespace
+
+int wavefront_num_variants() { return kNumTravVariants; }
 
 struct WavefrontState {
     uint32_t cap = 0;
     char* mem = nullptr;
     WfArrays a{};
-    uint32_t grid = 0;
+    int cus = 0;
+    std::unordered_map<const void*, uint32_t> grids;  // persistent grid per kernel (occupancy x CUs)
+    // grid of a persistent launch of kernel `f`: as many blocks as can be resident
+    uint32_t grid_for(const void* f) {
+        auto it = grids.find(f);
+        if (it != grids.end()) return it->second;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWfBlock, 0) != hipSuccess || per_cu < 1) per_cu = 4;
+        uint32_t g = uint32_t(std::max(cus, 1)) * uint32_t(per_cu);
+        grids.emplace(f, g);
+        return g;
+    }
 };
 
 WavefrontState* wavefront_create() { return new WavefrontState(); }
@@ -313,8 +372,12 @@ void wavefront_destroy(WavefrontState* s) {
     delete s;
 }
 
-int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm,
-                     std::string* err) {
+int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, int variant,
+                     StageTimer* tm, std::string* err) {
+    if (variant < 0 || variant >= kNumTravVariants) {
+        *err = "unknown traversal variant";
+        return kWavefrontUnsupported;
+    }
     if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats, tm);
     const uint32_t tiles = ((fa.width + 7) / 8) * ((fa.height + 7) / 8);
     const uint32_t nslots = tiles * 64u;
@@ -332,14 +395,21 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
         ws->cap = nslots;
         layout(ws->a, ws->mem, nslots);
     }
-    if (ws->grid == 0) {
+    if (ws->cus == 0) {
         int dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        ws->grid = uint32_t(std::max(cus, 1)) * 8u;
+        ws->cus = std::max(cus, 1);
     }
     const WfArrays& a = ws->a;
-    const uint32_t grid = ws->grid;
+    const ExtendFn k_extend = extend_kernel(variant, any_hit, stats);
+    const ShadowFn k_shadow = shadow_kernel(variant, any_hit, stats);
+    const void* k_shade = stats ? (const void*)k_wf_shade<true> : (const void*)k_wf_shade<false>;
+    const uint32_t g_extend = ws->grid_for((const void*)k_extend);
+    const uint32_t g_shadow = ws->grid_for((const void*)k_shadow);
+    const uint32_t g_shade = ws->grid_for(k_shade);
+    const uint32_t g_resolve = ws->grid_for((const void*)k_wf_resolve);
+    const uint32_t grid = g_resolve;
     hipError_t e = hipSuccess;
     // stage marks: 0 extend, 1 shade, 2 shadow, 3 resolve (init counts as resolve)
     auto mark = [&](int stage) {
@@ -362,29 +432,15 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
         uint32_t* q_out = (k & 1) ? a.q0 : a.q1;
         uint32_t* n_in = a.counts + (k & 1);
         uint32_t* n_out = a.counts + ((k + 1) & 1);
-#define RT_WF(kern, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kWfBlock), 0, st, __VA_ARGS__)
-        if (any_hit) {
-            if (stats) RT_WF((k_wf_extend<true, true>), fa, a, q_in, n_in, n_out, a.counts + 2);
-            else RT_WF((k_wf_extend<true, false>), fa, a, q_in, n_in, n_out, a.counts + 2);
-        } else {
-            if (stats) RT_WF((k_wf_extend<false, true>), fa, a, q_in, n_in, n_out, a.counts + 2);
-            else RT_WF((k_wf_extend<false, false>), fa, a, q_in, n_in, n_out, a.counts + 2);
-        }
+        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, st, fa, a, q_in, n_in, n_out, a.counts + 2);
         mark(0);
-        if (stats) RT_WF((k_wf_shade<true>), fa, a, q_in, n_in);
-        else RT_WF((k_wf_shade<false>), fa, a, q_in, n_in);
+        if (stats) hipLaunchKernelGGL((k_wf_shade<true>), dim3(g_shade), dim3(kWfBlock), 0, st, fa, a, q_in, n_in);
+        else hipLaunchKernelGGL((k_wf_shade<false>), dim3(g_shade), dim3(kWfBlock), 0, st, fa, a, q_in, n_in);
         mark(1);
-        if (any_hit) {
-            if (stats) RT_WF((k_wf_shadow<true, true>), fa, a);
-            else RT_WF((k_wf_shadow<true, false>), fa, a);
-        } else {
-            if (stats) RT_WF((k_wf_shadow<false, true>), fa, a);
-            else RT_WF((k_wf_shadow<false, false>), fa, a);
-        }
+        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, st, fa, a);
         mark(2);
-        RT_WF(k_wf_resolve, fa, a, q_in, n_in, q_out, n_out);
+        hipLaunchKernelGGL(k_wf_resolve, dim3(g_resolve), dim3(kWfBlock), 0, st, fa, a, q_in, n_in, q_out, n_out);
         mark(3);
-#undef RT_WF
         if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) {
             *err = "launch of a wavefront kernel";

@@ -76,6 +76,8 @@ class TraceStats(C.Structure):
 
 OPTION_STAGE_TIMING = 1
 OPTION_FLATTEN_IDENTITY_INSTANCES = 2
+OPTION_TRAVERSAL_VARIANT = 3
+NUM_TRAVERSAL_VARIANTS = 4
 
 
 class StageTimes(C.Structure):

@@ -402,4 +402,154 @@ def displaced_sphere(width=1920, height=1080, spp=64, bounces=8, seed=1, freq=31
     return b.finish("displaced_sphere", inst, plights, [L.default_dlight()], ubo, width, height)
 
 
-WORKLOADS = {"cornell_sphere": cornell_sphere, "displaced_sphere": displaced_sphere}
+def _interp_matrix(size, cells):
+    """(size, cells) matrix that resamples a periodic lattice of `cells` values to
+    `size` texel centres with smoothstep weights (rows sum to 1)."""
+    u = (np.arange(size) + 0.5) / size * cells - 0.5
+    i0 = np.floor(u).astype(np.int64)
+    f = u - i0
+    w = f * f * (3.0 - 2.0 * f)
+    m = np.zeros((size, cells))
+    np.add.at(m, (np.arange(size), i0 % cells), 1.0 - w)
+    np.add.at(m, (np.arange(size), (i0 + 1) % cells), w)
+    return m
+
+
+def noise_image(rng, size, channels, cells=(8, 32), amps=(0.7, 0.3)):
+    """Seeded tileable value noise in [0, 1], (size, size, channels) float64: random
+    lattices of `cells` per side, smoothstep-resampled (separable), octaves summed
+    with `amps`."""
+    out = np.zeros((size, size, channels))
+    for c, a in zip(cells, amps):
+        m = _interp_matrix(size, c)
+        for ch in range(channels):
+            out[..., ch] += a * (m @ rng.random((c, c)) @ m.T)
+    return np.clip(out / sum(amps), 0.0, 1.0)
+
+
+def spherical_uv(p):
+    d = p / np.linalg.norm(p, axis=1, keepdims=True)
+    u = np.arctan2(d[:, 2], d[:, 0]) / (2 * np.pi) + 0.5
+    v = np.arccos(np.clip(d[:, 1], -1.0, 1.0)) / np.pi
+    return np.stack([u, v], 1)
+
+
+def _random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def textured_instances(width=1920, height=1080, spp=32, bounces=6, seed=2, num_instances=1000, tex_size=2048,
+                       tri_range=(10_000, 200_000), num_meshes=8):
+    """Eight seeded meshes (displaced, stretched geodesic spheres of 10k-200k
+    triangles, log-uniform) instanced 1000 times with uniform random rotation and
+    uniform scale in [0.5, 2] over a textured ground plane.  Per mesh family three
+    RGBA8 noise textures: base colour (sRGB, noise alpha), metallic-roughness (UNORM:
+    g roughness, b metallic) and specular-glossiness (sRGB).  Each mesh has three
+    material variants, as three geometries over the same vertices and indices:
+    metallic-roughness OPAQUE, metallic-roughness MASK (cutoff 0.5, alpha from the
+    base colour texture), and specular-glossiness (roughness >= 0.3 everywhere); instances pick MASK with
+    probability 0.3 and specular-glossiness with 0.2.  Four point lights and a
+    directional light; 1920x1080, 32 spp, depth 6."""
+    rng = np.random.default_rng(seed)
+    b = Builder()
+    textures = [Texture(np.full((1, 1, 4), 255, np.uint8), L.FMT_RGBA8_UNORM)]  # slot 0: the dummy texture
+
+    def add_texture(img, fmt):
+        textures.append(Texture(np.ascontiguousarray((img * 255.0 + 0.5).astype(np.uint8)), fmt))
+        return len(textures) - 1
+
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    tri_counts = np.exp(rng.uniform(lo, hi, num_meshes))
+    geos = []  # per mesh: (opaque geo, mask geo, sg geo)
+    for m in range(num_meshes):
+        freq = max(4, int(round(np.sqrt(tri_counts[m] / 20.0))))
+        pos, tri = geodesic_sphere(freq)
+        disp = value_noise_3d(pos, seed * 100 + m, octaves=3, base_freq=2.0 + m % 3)
+        pos = pos * (1.0 + rng.uniform(0.05, 0.3) * disp)[:, None]
+        pos = pos * rng.uniform(0.6, 1.4, 3)
+        nrm = vertex_normals(pos, tri)
+        uv = spherical_uv(pos) * np.array([rng.integers(1, 4), rng.integers(1, 3)])
+        verts = make_vertices(pos, nrm, uv=uv)
+        base = noise_image(rng, tex_size, 4)
+        base[..., :3] = 0.15 + 0.8 * base[..., :3]
+        mr = noise_image(rng, tex_size, 4)
+        mr[..., 0] = 1.0
+        mr[..., 1] = 0.3 + 0.7 * mr[..., 1]  # roughness >= 0.3: bounded point-light highlights
+        mr[..., 2] = (mr[..., 2] > 0.55).astype(np.float64)
+        sg = noise_image(rng, tex_size, 4)
+        sg[..., :3] = 0.02 + 0.6 * sg[..., :3]
+        sg[..., 3] = 0.7 * sg[..., 3]  # glossiness <= 0.7
+        t_base = add_texture(base, L.FMT_RGBA8_SRGB)
+        t_mr = add_texture(mr, L.FMT_RGBA8_UNORM)
+        t_sg = add_texture(sg, L.FMT_RGBA8_SRGB)
+        tint = rng.uniform(0.7, 1.0, 3)
+        mats = []
+        for variant in range(3):
+            mt = L.default_material()
+            mt["base_color"] = (*tint, 1.0)
+            mt["base_color_texture"]["index"] = t_base
+            mt["base_color_texture"]["tex_coord"] = 0
+            mt["metallic_roughness_texture"]["index"] = t_mr
+            mt["metallic_roughness_texture"]["tex_coord"] = 0
+            if variant == 1:
+                mt["alpha_mode"] = L.ALPHA_MASK
+                mt["alpha_cutoff"] = 0.5
+            if variant == 2:
+                mt["workflow"] = 1  # specular-glossiness
+                mt["sg_diffuse_factor"] = (*tint, 1.0)
+                mt["sg_specular_glossiness_factor"] = (1.0, 1.0, 1.0, 1.0)
+                mt["sg_diffuse_texture"]["index"] = t_base
+                mt["sg_diffuse_texture"]["tex_coord"] = 0
+                mt["sg_specular_glossiness_texture"]["index"] = t_sg
+                mt["sg_specular_glossiness_texture"]["tex_coord"] = 0
+            mats.append(b.add_material(mt))
+        g0 = b.add_primitive(verts, tri.reshape(-1).astype(np.uint32), mats[0])
+        v_off, i_off = b.prims[g0][0], b.prims[g0][1]
+        ntri = len(tri)
+        for variant in (1, 2):  # same vertices and indices, other material
+            b.prims.append((v_off, i_off, mats[variant], ntri))
+        geos.append((g0, g0 + 1, g0 + 2))
+    # textured ground plane (identity instance)
+    gp = np.array([[-60, 0, -60], [60, 0, -60], [60, 0, 60], [-60, 0, 60]], np.float64)
+    guv = np.array([[0, 0], [12, 0], [12, 12], [0, 12]], np.float64)
+    gb = noise_image(rng, tex_size, 4)
+    gb[..., :3] = 0.3 + 0.5 * gb[..., :3]
+    mt = L.default_material()
+    mt["base_color_texture"]["index"] = add_texture(gb, L.FMT_RGBA8_SRGB)
+    mt["base_color_texture"]["tex_coord"] = 0
+    mt["metallic_factor"] = 0.0
+    mt["roughness_factor"] = 0.8
+    ground = b.add_primitive(make_vertices(gp, np.tile([0.0, 1.0, 0.0], (4, 1)), uv=guv),
+                             np.array([0, 2, 1, 0, 3, 2], np.uint32), b.add_material(mt))
+    entries = [(None, ground)]
+    for i in range(num_instances):
+        mesh = int(rng.integers(num_meshes))
+        r = rng.random()
+        variant = 1 if r < 0.3 else (2 if r < 0.5 else 0)
+        m4 = np.eye(4)
+        s = rng.uniform(0.5, 2.0)
+        m4[:3, :3] = _random_rotation(rng) * s
+        m4[:3, 3] = (rng.uniform(-25, 25), rng.uniform(1.0, 6.0), rng.uniform(-25, 25))
+        entries.append((m4, geos[mesh][variant]))
+    inst = make_instances(entries)
+    # lights above every instance (instances reach y < 10)
+    plights = [point_light((0.0, 16.0, 0.0), 600.0, (1.0, 0.95, 0.9)),
+               point_light((-15.0, 14.0, 10.0), 400.0, (0.8, 0.9, 1.0)),
+               point_light((15.0, 13.0, -10.0), 400.0, (1.0, 0.85, 0.7)),
+               point_light((5.0, 12.0, 20.0), 200.0, (1.0, 1.0, 1.0))]
+    sun = L.default_dlight()
+    sun["transform"] = (0.3, -1.0, 0.2, 0.0)
+    sun["intensity"] = 2.0
+    view = look_at_rh((0.0, 14.0, 34.0), (0.0, 1.5, 0.0), (0, 1, 0))
+    proj = perspective_vk(width / height, 50.0, 0.1, 200.0)
+    ubo = make_ubo(view, proj, samples=spp, bounces=bounces, random_seed=seed, fully_opaque=False)
+    return b.finish("textured_instances", inst, plights, [sun], ubo, width, height, textures)
+
+
+WORKLOADS = {"cornell_sphere": cornell_sphere, "displaced_sphere": displaced_sphere,
+             "textured_instances": textured_instances}

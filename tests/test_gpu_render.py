@@ -172,3 +172,30 @@ def test_stage_timing_and_split_counters(gpu_ctx, mode):
     assert st["shadow_instance_visits"] <= st["instance_visits"]
     with pytest.raises(rt.RtError):
         gpu_ctx.stage_times()  # option off
+
+
+@pytest.mark.parametrize("name", ["cornell_sphere", "mask_soup"])
+def test_traversal_variants_render_the_same_image(gpu_ctx, name):
+    """Every wavefront traversal variant (stack depth, register target) gives the
+    megakernel's image bit for bit."""
+    if name == "mask_soup":
+        def make():
+            sc = K.triangle_soup(400, seed=31, opaque=False)
+            rng = np.random.default_rng(32)
+            sc.vertices["color"][:, 3] = rng.random(len(sc.vertices))
+            sc.ubo["fully_opaque"] = 0
+            sc.ubo["number_of_bounces"] = 5
+            return sc
+        ds = G.device_scene(gpu_ctx, "mask_soup_variants", make)
+    else:
+        ds = G.device_scene(gpu_ctx, name)
+    ref, _ = _render(gpu_ctx, ds, mode=rt.MODE_MEGAKERNEL)
+    try:
+        for v in range(rt.NUM_TRAVERSAL_VARIANTS):
+            gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, v)
+            img, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
+            assert np.array_equal(ref.cpu().numpy(), img.cpu().numpy()), v
+    finally:
+        gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
+    with pytest.raises(rt.RtError):
+        gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, rt.NUM_TRAVERSAL_VARIANTS)
