@@ -77,7 +77,7 @@ def lib():
         sig = {
             "orc_rand_sequence": (u, [vp, u, vp]), "orc_tea16": (u, [u, u]), "orc_pcg4d": (None, [vp, vp]),
             "orc_lcg_sequence": (u, [u, u, vp]), "orc_stream_b_seed": (u, [u, u]),
-            "orc_mat_build": (None, [vp]), "orc_eval_combined_brdf": (None, [dp, dp, dp, vp, dp]),
+            "orc_mat_build": (None, [vp]), "orc_scene_set_eval_textures": (None, [vp, vp, C.c_uint32]), "orc_eval_combined_brdf": (None, [dp, dp, dp, vp, dp]),
             "orc_brdf_probability": (None, [vp, dp, dp, dp]),
             "orc_sample_indirect": (i, [dp, dp, dp, dp, vp, u, dp, dp]),
             "orc_ggx_d": (d, [d, d]), "orc_smith_g2_height_correlated": (d, [d, d, d]),
@@ -156,6 +156,25 @@ class OracleScene:
         self.handle = lib().orc_scene_create(C.byref(self.desc))
         if not self.handle:
             raise ValueError("oracle rejected the scene (instance references out of range)")
+
+    def set_eval_textures(self, textures):
+        """Detached evaluation: BRDF values use `textures` (a list parallel to the
+        scene's textures, each a (h, w, 4) array in the same format) while lobe
+        choices, directions and densities keep the scene's textures.  None
+        switches it off."""
+        if textures is None:
+            lib().orc_scene_set_eval_textures(self.handle, None, 0)
+            self._eval = None
+            return
+        arr = (RtTexture * len(textures))()
+        keep = []
+        for k, (t, data) in enumerate(zip(self.scene.textures, textures)):
+            data = np.ascontiguousarray(data, t.data.dtype)
+            keep.append(data)
+            h, w = data.shape[:2]
+            arr[k] = RtTexture(data.ctypes.data, w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter, t.min_filter)
+        self._eval = (arr, keep)
+        lib().orc_scene_set_eval_textures(self.handle, C.addressof(arr), len(textures))
 
     def close(self):
         if getattr(self, "handle", None) and _lib is not None:

@@ -436,8 +436,12 @@ inline void texture_sample(const rt_texture& t, double u, double v, double out[4
     texel_fetch(t, x1, y0, t10);
     texel_fetch(t, x0, y1, t01);
     texel_fetch(t, x1, y1, t11);
-    for (int k = 0; k < 4; k++)
-        out[k] = (1 - a) * (1 - b) * t00[k] + a * (1 - b) * t10[k] + (1 - a) * b * t01[k] + a * b * t11[k];
+    // bilinear as two nested lerps x + w (y - x): a constant neighbourhood filters to
+    // exactly that constant (as fixed-point texture hardware gives it)
+    for (int k = 0; k < 4; k++) {
+        double top = t00[k] + a * (t10[k] - t00[k]), bot = t01[k] + a * (t11[k] - t01[k]);
+        out[k] = top + b * (bot - top);
+    }
 }
 
 // ------------------------------------------------- ray/triangle (watertight)
@@ -588,6 +592,11 @@ struct HitRec {
 
 struct Scene {
     orc_scene_desc d;
+    // Detached evaluation (the adjoint's definition of the derivative): when not
+    // empty, closest-hit evaluates the BRDF values (NEE and the sampled weights)
+    // with these textures while lobe choices, directions and densities keep using
+    // d.textures.
+    std::vector<rt_texture> eval_tex;
     std::vector<std::vector<double>> tri_pos;  // per BLAS: 9 doubles per triangle
     std::vector<Bvh> blas_bvh;
     std::vector<Box> blas_box;
@@ -695,6 +704,44 @@ template <class R> V4<R> fetch_tex(const Scene& s, int32_t index, V4<R> uv0and1,
     double out[4] = {1, 1, 1, 1};
     if (index >= 0 && uint32_t(index) < s.d.num_textures) texture_sample(s.d.textures[index], double(u), double(v), out);
     return {R(out[0]), R(out[1]), R(out[2]), R(out[3])};
+}
+
+template <class R> V4<R> fetch_tex_eval(const Scene& s, int32_t index, V4<R> uv0and1, int32_t coord) {
+    R u = 0, v = 0;
+    uint32_t c = uint32_t(coord);
+    if (c == 0) { u = uv0and1.x; v = uv0and1.y; }
+    else if (c == 1) { u = uv0and1.z; v = uv0and1.w; }
+    double out[4] = {1, 1, 1, 1};
+    if (index >= 0 && size_t(index) < s.eval_tex.size()) texture_sample(s.eval_tex[size_t(index)], double(u), double(v), out);
+    return {R(out[0]), R(out[1]), R(out[2]), R(out[3])};
+}
+
+// The sampled weight of evalIndirectCombinedBRDF for a direction L drawn with the
+// sampling material ms, with the BRDF values of me and the density of ms
+// (PBR.glsl:649-729 with f / pdf split): specular F D G2 / (G1(v) D)|ms, diffuse
+// (1 - F s) c_diff Fd (1 - t), transmission c_diff t.
+template <class R> V3<R> detached_weight(uint32_t type, V3<R> N, V3<R> L, V3<R> V, const MatBrdf<R>& ms,
+                                         const MatBrdf<R>& me, bool delta) {
+    if (type == SPECULAR_TYPE) {
+        V3<R> H = normalize(L + V);
+        R hdl = std::max(R(0.00001), std::min(R(1), dot(H, L)));
+        R ndl = std::max(R(0.00001), std::min(R(1), dot(N, L)));
+        R ndv = std::max(R(0.00001), std::min(R(1), dot(N, V)));
+        R ndh = saturate(dot(N, H));
+        V3<R> F = fresnel_schlick(me.F0, shadowed_f90(me.F90), hdl);
+        R ae = me.roughness * me.roughness, as = ms.roughness * ms.roughness;
+        if (delta || as == R(0)) return F * me.specular_factor;
+        R g1v_e = smith_g1_ggx(ae, ndv, ae * ae, ndv * ndv), g1v_s = smith_g1_ggx(as, ndv, as * as, ndv * ndv);
+        R g2_e = smith_g2_over_g1(ae, ae * ae, ndl, ndv) * g1v_e;
+        R d_e = ggx_d(ae * ae, ndh), d_s = ggx_d(as * as, ndh);
+        return F * (g2_e * d_e / (g1v_s * d_s) * me.specular_factor);
+    }
+    if (type == DIFFUSE_TYPE) {
+        BrdfData<R> d = prepare_brdf(N, L, V, me);
+        R fb = frostbite(d.roughness, d.n_dot_l, d.n_dot_v, d.l_dot_h);
+        return (V3<R>(R(1)) - d.F * me.specular_factor) * d.c_diff * fb * (R(1) - me.transmission);
+    }
+    return vmax(V3<R>(R(0)), me.c_diff * me.transmission);
 }
 
 template <class R> bool anyhit_ignores(const AnyHitCtx& ah, uint32_t custom_index, uint32_t prim, R bu, R bv) {
@@ -824,6 +871,7 @@ template <class R> HitRec trace(const Scene& s, const double o[3], const double 
 // --------------------------------------------------------------- integrator
 template <class R> struct Payload {  // RayPayload, lib/RayTracingCommons.glsl:11-24
     V3<R> hit_value, hit_point;
+    V3<R> hit_value_s;  // detached evaluation: the weight of the sampling material
     R t;
     V3<R> scatter_direction;
     bool need_scatter;
@@ -913,6 +961,7 @@ template <class R> struct Tracer {
         // HasSky (skybox cube) is not supported: the constant term of the sky-off branch.
         acc = acc + V3<R>(R(0.01));
         ray.hit_value = V3<R>(R(0));
+        ray.hit_value_s = ray.hit_value;
         ray.need_scatter = false;
         ray.emittance = acc;
         if (s.d.num_dlights == 0) ray.emittance = V3<R>(R(0));
@@ -1048,6 +1097,23 @@ template <class R> struct Tracer {
         m.attenuation_color = load3<R>(mat.attenuation_color);
         m.attenuation_distance = R(mat.attenuation_distance);
         m.volume = mat.volume_exists != 0;
+        const bool detached = !s.eval_tex.empty();
+        MatBrdf<R> me = m;  // BRDF values (detached evaluation), else m itself
+        if (detached && mat.workflow != RT_WORKFLOW_SPECULAR_GLOSSINESS) {
+            V4<R> c4 = vcol * load4<R>(mat.base_color);
+            if (mat.base_color_texture.index >= 0)
+                c4 = c4 * fetch_tex_eval<R>(s, mat.base_color_texture.index, uv, mat.base_color_texture.tex_coord);
+            R me_metal = R(mat.metallic_factor), me_rough = R(mat.roughness_factor);
+            if (mat.metallic_roughness_texture.index >= 0) {
+                V4<R> tx = fetch_tex_eval<R>(s, mat.metallic_roughness_texture.index, uv, mat.metallic_roughness_texture.tex_coord);
+                me_rough = me_rough * tx.y;
+                me_metal = me_metal * tx.z;
+            }
+            me.base_color = c4.xyz();
+            me.metallic = me_metal;
+            me.roughness = me_rough;
+            mat_build(me);
+        }
         R displacement = length(origin - last_hit);
         m.t_diff = displacement;
         const rt_light* light = nullptr;
@@ -1057,7 +1123,7 @@ template <class R> struct Tracer {
             R ldist = length(lv);
             lv = normalize(lv);
             if (shadow_unoccluded(origin, lv, ldist, ray.rng)) {
-                V3<R> f = eval_combined_brdf(outward, lv, V, m);
+                V3<R> f = eval_combined_brdf(outward, lv, V, detached ? me : m);
                 ray.emittance = ray.emittance + f * (lw * R(light->intensity)) * load3<R>(light->color);
             }
         }
@@ -1096,6 +1162,9 @@ template <class R> struct Tracer {
         R uy = R(rand_next(rng));
         V3<R> dir(R(0)), w(R(0));
         ray.need_scatter = sample_indirect(ux, uy, outward, geo_normal, V, m, type, dir, w, ray.volume_dis);
+        ray.hit_value_s = thr * w;
+        if (detached && ray.need_scatter)
+            w = detached_weight(type, outward, dir, V, m, me, metallic == R(1) && roughness == R(0));
         thr = thr * w;
         ray.hit_point = origin;
         ray.scatter_direction = dir;
@@ -1107,11 +1176,13 @@ template <class R> struct Tracer {
     V3<R> pixel(uint32_t x, uint32_t y, uint32_t W, uint32_t H) {
         uint32_t seed_b = stream_b_seed(ubo.random_seed, ubo.frame_count);
         Payload<R> ray;
+        ray.hit_value = ray.hit_value_s = V3<R>(R(0));
         ray.random_seed = tea16(tea16(x, y), seed_b);
         U4 rng_a{x, y, ubo.frame_count, 0};
         ray.rng = U4{x, y, seed_b, 0};
         ray.hit_point = V3<R>(R(0));
         ray.hit_value = V3<R>(R(0));
+        ray.hit_value_s = ray.hit_value;
         ray.emittance = V3<R>(R(0));
         ray.need_scatter = false;
         ray.t = R(0);
@@ -1149,7 +1220,7 @@ template <class R> struct Tracer {
                 direction = mat4_vec4(mvi, R(0), R(0), R(-1), R(0));
                 tfar = R(10) * R(10000.0);
             }
-            V3<R> thr(R(1));
+            V3<R> thr(R(1)), thr_s(R(1));
             ray.t = R(0);
             ray.volume_dis = R(-1);
             uint32_t flags = ubo.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
@@ -1166,12 +1237,14 @@ template <class R> struct Tracer {
                 ray.emittance = V3<R>(R(0));
                 if (b + 1 == ubo.number_of_bounces) break;
                 if (b > 3) {  // Russian roulette, rgen:109-118 (MIN_BOUNCES = 3)
-                    R rr = clampr(luminance(thr), R(0.01), R(0.95));
+                    R rr = clampr(luminance(s.eval_tex.empty() ? thr : thr_s), R(0.01), R(0.95));
                     R prop = R(rand_next(rng_a));
                     if (rr < prop) break;
                     thr = thr / rr;
+                    thr_s = thr_s / rr;
                 }
                 thr = thr * hit_color;
+                if (!s.eval_tex.empty()) thr_s = thr_s * ray.hit_value_s;
                 if (!scattered || t < R(0)) break;
                 o = ray.hit_point;
                 d = ray.scatter_direction;
@@ -1282,6 +1355,10 @@ void* orc_scene_create(const orc_scene_desc* desc) {
     return build_scene(desc);
 }
 void orc_scene_destroy(void* s) { delete (Scene*)s; }
+void orc_scene_set_eval_textures(void* s, const rt_texture* tex, uint32_t n) {
+    Scene* sc = (Scene*)s;
+    sc->eval_tex.assign(tex, tex + n);
+}
 
 static void fill_hit(const HitRec& h, const Scene& s, rt_hit* out, double* tuv) {
     std::memset(out, 0, sizeof(*out));

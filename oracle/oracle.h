@@ -105,6 +105,11 @@ int orc_intersect_triangle(const double o[3], const double d[3], const double v0
 /* --- scene --- */
 void* orc_scene_create(const orc_scene_desc* desc);
 void orc_scene_destroy(void* scene);
+/* Detached evaluation for finite differences of the adjoint's derivative: BRDF
+ * values (NEE, sampled weights) use `tex` in place of the scene's textures while
+ * lobe choices, directions, densities and Russian roulette keep the scene's
+ * textures.  n = 0 switches it off.  The arrays must outlive the renders. */
+void orc_scene_set_eval_textures(void* scene, const rt_texture* tex, uint32_t n);
 /* rays: n x {ox,oy,oz,tmin,dx,dy,dz,tmax} (float).  hits: n x rt_hit (t,u,v in
  * float for layout compatibility) plus n x 3 doubles of (t,u,v) when hits_tuv is
  * non-NULL.  brute != 0 tests every triangle of every instance.  Without

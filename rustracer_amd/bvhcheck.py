@@ -67,9 +67,9 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
     return errors
 
 
-INST = np.dtype([("w2o", "<f4", (3, 4)), ("nodes", "<u8"), ("tris", "<u8"), ("root", "<u4"), ("mask", "<u4"),
+INST = np.dtype([("w2o_d", "<f8", (3, 4)), ("nodes", "<u8"), ("tris", "<u8"), ("root", "<u4"), ("mask", "<u4"),
                  ("flags", "<u4"), ("instance_id", "<u4"), ("o2w", "<f4", (3, 4)), ("blas", "<u8"),
-                 ("custom_index", "<u4"), ("_pad", "<u4"), ("w2o_d", "<f8", 12), ("_pad2", "<u4", 4)])
+                 ("custom_index", "<u4"), ("_pad", "<u4", 17)])
 FLAT_OPAQUE = 0x80000000
 
 
@@ -105,6 +105,6 @@ def check_tlas(ctx, tlas, num_instances):
     expect = np.arange(len(pp)) - np.repeat(starts, counts)
     if not np.array_equal(pp, expect.astype(np.uint32)):
         errs.append("primitive ids of a flattened instance are not a permutation of 0..n-1")
-    if np.any(np.abs(inst["w2o"][nt:] - np.eye(3, 4, dtype=np.float32)) > 0):
+    if np.any(np.abs(inst["w2o_d"][nt:] - np.eye(3, 4)) > 0):
         errs.append("a flattened instance does not have the identity transform")
     return errs

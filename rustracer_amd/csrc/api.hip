@@ -473,7 +473,6 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
         InstanceRecord& r = recs[i];
         std::memset(&r, 0, sizeof(r));
         for (int row = 0; row < 3; row++) {
-            r.w2o[row] = make_float4(float(w[row * 4]), float(w[row * 4 + 1]), float(w[row * 4 + 2]), float(w[row * 4 + 3]));
             r.o2w[row] = make_float4(in.transform[row][0], in.transform[row][1], in.transform[row][2], in.transform[row][3]);
         }
         for (int k = 0; k < 12; k++) r.w2o_d[k] = w[k];

@@ -6,10 +6,10 @@
 //              loads), child references and leaf counts.
 //   Triangle    48 B: three float4 {v.xyz, w}; v0.w carries the primitive index
 //              (gl_PrimitiveID) as bits.  Leaves reference contiguous runs.
-//   InstanceRecord 256 B: world->object rows, BLAS node/triangle pools and root,
-//              mask, flags and instance id first (what traversal reads), then
-//              object->world rows, the BLAS descriptor, custom index and
-//              world->object in fp64 for hit refinement.
+//   InstanceRecord 256 B: world->object rows in fp64, BLAS node/triangle pools
+//              and root, mask, flags and instance id first (what traversal
+//              reads), then object->world rows, the BLAS descriptor and the
+//              custom index.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,21 +53,19 @@ enum : uint32_t {
 };
 
 struct alignas(16) InstanceRecord {
-    // first 80 bytes: what traversal reads when it enters the instance
-    float4 w2o[3];          // rows of the world-to-object 3x4 matrix
+    // first 128 bytes: what traversal reads when it enters the instance
+    double w2o_d[12];       // world-to-object 3x4 rows in fp64 (traversal, hit refinement)
     const WideNode* nodes;  // BLAS node pool
     const float4* tris;     // BLAS triangle pool
     uint32_t root;          // BLAS root node
     uint32_t mask;          // 8-bit visibility mask
     uint32_t flags;         // kInst*
     uint32_t instance_id;   // gl_InstanceID (index in the caller's instance array)
-    // shading
+    // shading and the TLAS build
     float4 o2w[3];          // rows of the object-to-world 3x4 matrix (gl_ObjectToWorldEXT)
     const BlasDesc* blas;
     uint32_t custom_index;
-    uint32_t _pad;
-    double w2o_d[12];       // world-to-object in fp64 (hit refinement)
-    uint32_t _pad2[4];
+    uint32_t _pad[17];
 };
 static_assert(sizeof(InstanceRecord) == 256, "InstanceRecord is 256 bytes");
 

@@ -183,6 +183,12 @@ struct ShadeState {
     D3 light_scale;   // light_weight * intensity * color
     Rng4 rng;         // local stream after the RIS draws
     double displacement;
+    // read by the adjoint only (dead in the forward kernels)
+    const rt_material_raw* mat;
+    D4 uv, vcol;
+    uint32_t lobe;    // kDiffuse / kSpecular / kTransmission chosen by shade_end
+    double p_lobe;    // its probability (1 for the metallic-1 roughness-0 mirror)
+    bool delta;       // the metallic-1 roughness-0 mirror (no lobe draw)
 };
 
 __device__ inline D4 mix4(D4 a, D4 b, D4 c, double b0, double bu, double bv) {
@@ -344,6 +350,9 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
     m.attenuation_distance = mat.attenuation_distance;
     m.volume = mat.volume_exists != 0;
     s.displacement = dlength(origin - last_hit);
+    s.mat = &mat;
+    s.uv = uv;
+    s.vcol = vcol;
     s.origin = origin;
     s.geo_normal = geo_normal;
     s.outward = outward;
@@ -399,7 +408,9 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s) {
     Rng4 rng = s.rng;
     uint32_t type;
     D3 thr = d3(1.0);
-    if (m.metallic == 1.0 && m.roughness == 0.0) {
+    s.delta = m.metallic == 1.0 && m.roughness == 0.0;
+    s.p_lobe = 1.0;
+    if (s.delta) {
         type = kSpecular;
     } else {
         D3 p = brdf_probability(m, s.V, s.outward);
@@ -407,10 +418,12 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s) {
         if (r < p.x) {
             type = kSpecular;
             thr = thr / p.x;
+            s.p_lobe = p.x;
             if (ray.volume_dis >= 0.0) ray.volume_dis += s.displacement;
         } else if (r >= p.x && r <= p.x + p.y) {
             type = kDiffuse;
             thr = thr / p.y;
+            s.p_lobe = p.y;
             if (ray.volume_dis >= 0.0) ray.volume_dis += s.displacement;
         } else {
             type = kTransmission;
@@ -419,8 +432,10 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s) {
                 else if (s.front) ray.volume_dis = 0.0;
             }
             thr = thr / p.z;
+            s.p_lobe = p.z;
         }
     }
+    s.lobe = type;
     D3 origin = s.origin;
     if (type == kTransmission) origin = offset_ray(origin, -s.geo_normal);
     double ux = rng_next(rng);
@@ -473,14 +488,22 @@ struct BounceOut {
     double sdist;
     uint32_t shadow_wb;  // stream-B counter of the shadow payload's copy (rchit:45)
     bool shadow, cont;
+    // read by the adjoint only
+    D3 emit, nee;     // emittance, NEE term (before the shadow test)
+    D3 mult;          // factor from this bounce's throughput to the next one's
+};
+
+// Closest-hit observer of shade_bounce (the adjoint records the shaded vertex).
+struct NoTap {
+    __device__ void vertex(const ShadeState&, const Payload&) {}
 };
 
 // The shaders for the traced result of p's current ray (closest-hit with NEE set-up
 // or miss), then the rest of the raygen bounce loop (rgen:98-128): Russian
 // roulette, throughput update, next ray.
-template <bool kStats>
+template <bool kStats, class Tap>
 __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p, bool found,
-                                    const HitInfo& h, BounceOut& r, PathCounters& pc) {
+                                    const HitInfo& h, BounceOut& r, PathCounters& pc, Tap& tap) {
     Payload ray;
     ray.hit_point = p.hit_point;
     ray.t = p.bounce == 0u ? 0.0 : 1.0;  // only "Ray.t != 0" is read before it is set (rmiss:18)
@@ -506,31 +529,44 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
                 nee = eval_combined_brdf(ss.outward, ss.light_dir, ss.V, ss.m) * ss.light_scale;
             }
             shade_end(ray, ss);
+            tap.vertex(ss, ray);
         }
     } else {
         miss_shader(fa.sv, ray, p.d);
     }
     r.pend_without = p.thr * ray.emittance;
     r.pend_with = p.thr * (ray.emittance + nee);
+    r.emit = ray.emittance;
+    r.nee = nee;
+    r.mult = d3(0.0);
     p.hit_point = ray.hit_point;
     p.volume_dis = ray.volume_dis;
     p.wb = ray.rng.w;
     r.cont = false;
     if (p.bounce + 1u >= fa.bounces) return;
+    double rr = 1.0;
     if (p.bounce > kMinBounces) {  // Russian roulette, rgen:109-118
         Rng4 ra{x, y, fa.frame_count, p.wa};
-        double rr = clampd(luminance(p.thr), 0.01, 0.95);
+        rr = clampd(luminance(p.thr), 0.01, 0.95);
         double prop = rng_next(ra);
         p.wa = ra.w;
         if (rr < prop) return;
         p.thr = p.thr / rr;
     }
     p.thr = p.thr * ray.hit_value;
+    r.mult = ray.hit_value / rr;
     if (!ray.need_scatter || ray.t < 0.0) return;
     p.o = ray.hit_point;
     p.d = ray.scatter_dir;
     p.bounce++;
     r.cont = true;
+}
+
+template <bool kStats>
+__device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p, bool found,
+                                    const HitInfo& h, BounceOut& r, PathCounters& pc) {
+    NoTap tap;
+    shade_bounce<kStats>(fa, x, y, p, found, h, r, pc, tap);
 }
 
 constexpr uint32_t kShadowFlags = RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RT_RAY_FLAG_SKIP_CLOSEST_HIT;
@@ -544,7 +580,9 @@ __device__ inline bool shadow_unoccluded(const FrameArgs& fa, uint32_t x, uint32
     Rng4 shadow_rng{x, y, fa.seed_b, r.shadow_wb};
     AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
     HitInfo sh;
-    return !trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(r.so), to_f3(r.sd), float(kShadowTMin), float(r.sdist),
+    // the fp32 ray, as the wavefront shadow queue holds it
+    const float3 so = to_f3(r.so), sd = to_f3(r.sd);
+    return !trace_ray<kAnyHit, kStats>(fa.tlas, so, sd, RayFromF32{so, sd}, float(kShadowTMin), float(r.sdist),
                                        kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u), sah, stack, stride,
                                        sh, pc.trs);
 }
@@ -564,8 +602,8 @@ __device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, u
             Rng4 rb{x, y, fa.seed_b, p.wb};
             AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
             HitInfo h;
-            bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(p.o), to_f3(p.d), float(kTMin), float(p.tfar),
-                                                    flags, ah, stack, stride, h, pc.tr);
+            bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(p.o), to_f3(p.d), RayFromF64{p.o, p.d}, float(kTMin),
+                                                    float(p.tfar), flags, ah, stack, stride, h, pc.tr);
             p.wb = rb.w;
             BounceOut r;
             shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);

@@ -53,9 +53,9 @@ __global__ __launch_bounds__(kMegaBlock) void k_ray_queries(TlasView tl, SceneVi
     AlphaAnyHit ah{&sv, &tl, &rng};
     HitInfo h;
     TraceStats st;
-    bool found = trace_ray<kAnyHit, false>(tl, mk3(r.origin[0], r.origin[1], r.origin[2]),
-                                          mk3(r.direction[0], r.direction[1], r.direction[2]), r.tmin, r.tmax, flags,
-                                          ah, s_stack + threadIdx.x, kMegaBlock, h, st);
+    const float3 ro = mk3(r.origin[0], r.origin[1], r.origin[2]), rd = mk3(r.direction[0], r.direction[1], r.direction[2]);
+    bool found = trace_ray<kAnyHit, false>(tl, ro, rd, RayFromF32{ro, rd}, r.tmin, r.tmax, flags, ah,
+                                          s_stack + threadIdx.x, kMegaBlock, h, st);
     rt_hit out;
     out._pad[0] = out._pad[1] = 0;
     if (found) {
@@ -90,8 +90,8 @@ __global__ __launch_bounds__(kMegaBlock) void k_primary(FrameArgs fa, rt_hit* hi
     HitInfo h;
     TraceStats st;
     uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-    bool found = trace_ray<kAnyHit, false>(fa.tlas, to_f3(o), to_f3(d), float(kTMin), float(tfar), flags, ah,
-                                          s_stack + threadIdx.x, kMegaBlock, h, st);
+    bool found = trace_ray<kAnyHit, false>(fa.tlas, to_f3(o), to_f3(d), RayFromF64{o, d}, float(kTMin), float(tfar),
+                                          flags, ah, s_stack + threadIdx.x, kMegaBlock, h, st);
     rt_hit out;
     out._pad[0] = out._pad[1] = 0;
     if (found) {

@@ -119,6 +119,11 @@ __device__ inline D4 texel(const rt_texture& t, const float* lut, int x, int y) 
     return D4{r / 255.0, g / 255.0, b / 255.0, a / 255.0};
 }
 
+__device__ inline double bilerp(double t00, double t10, double t01, double t11, double a, double b) {
+    double top = t00 + a * (t10 - t00), bot = t01 + a * (t11 - t01);
+    return top + b * (bot - top);
+}
+
 __device__ inline D4 sample_texture(const SceneView& sv, int32_t index, double u, double v) {
     if (index < 0 || uint32_t(index) >= sv.num_textures) return D4{1.0, 1.0, 1.0, 1.0};
     const rt_texture& t = sv.textures[index];
@@ -135,9 +140,11 @@ __device__ inline D4 sample_texture(const SceneView& sv, int32_t index, double u
     int y0 = wrap_coord(int(iv), H, t.wrap_t), y1 = wrap_coord(int(iv) + 1, H, t.wrap_t);
     D4 t00 = texel(t, sv.srgb_lut, x0, y0), t10 = texel(t, sv.srgb_lut, x1, y0);
     D4 t01 = texel(t, sv.srgb_lut, x0, y1), t11 = texel(t, sv.srgb_lut, x1, y1);
-    double w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
-    return D4{w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x, w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
-              w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z, w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w};
+    // two nested lerps x + w (y - x): a constant neighbourhood filters to exactly
+    // that constant, as texture hardware (fixed-point weights) gives it, whatever
+    // the rounding or contraction of the arithmetic
+    return D4{bilerp(t00.x, t10.x, t01.x, t11.x, a, b), bilerp(t00.y, t10.y, t01.y, t11.y, a, b),
+              bilerp(t00.z, t10.z, t01.z, t11.z, a, b), bilerp(t00.w, t10.w, t01.w, t11.w, a, b)};
 }
 
 // getUV, lib/RayTracingCommons.glsl:65-72

@@ -133,6 +133,26 @@ struct NoAnyHit {
     __device__ bool ignore(uint32_t, uint32_t, float, float) { return false; }
 };
 
+// Source of the ray in fp64, read when the traversal enters an instance: the ray is
+// taken to object space in fp64 (InstanceRecord::w2o_d) and rounded to fp32 there,
+// so the object-space ray carries the rounding of object-space coordinates rather
+// than that of the (larger) world coordinates.
+struct RayFromF32 {  // the caller's ray is fp32
+    float3 o, d;
+    __device__ void get(D3& O, D3& Dd) const {
+        O = D3{double(o.x), double(o.y), double(o.z)};
+        Dd = D3{double(d.x), double(d.y), double(d.z)};
+    }
+};
+struct RayFromF64 {  // the caller holds the fp64 ray
+    const D3& o;
+    const D3& d;
+    __device__ void get(D3& O, D3& Dd) const {
+        O = o;
+        Dd = d;
+    }
+};
+
 enum : int { kLevelTlas = 0, kLevelBlas = 1, kLevelFlat = 2 };
 
 // Closest-hit traversal.  `ah.ignore(inst_index, prim, u, v)` is the any-hit
@@ -141,9 +161,10 @@ enum : int { kLevelTlas = 0, kLevelBlas = 1, kLevelFlat = 2 };
 // The nearest hit child of a node is kept in a register and only the others go to
 // the stack.  The flattened BVH (identity instances) is traversed first, with the
 // TLAS root beneath it on the stack, so its hits bound the TLAS traversal.
-template <bool kAnyHit, bool kStats, int kDepth = kStackDepth, class AnyHit>
-__device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, float tmax, uint32_t ray_flags,
-                          AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit, TraceStats& st) {
+template <bool kAnyHit, bool kStats, int kDepth = kStackDepth, class AnyHit, class RaySrc>
+__device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& rs, float tmin, float tmax,
+                          uint32_t ray_flags, AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit,
+                          TraceStats& st) {
     static_assert((kDepth & (kDepth - 1)) == 0, "the stack ring depth is a power of two");
     const bool terminate = (ray_flags & RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT) != 0;
     float best_t = tmax;
@@ -227,12 +248,15 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, float tmin, fl
             if (kStats) st.instances++;
             const uint32_t fl = ir.flags;
             if ((ir.mask & 0xFFu) != 0u && !(fl & kInstEmpty)) {
-                float4 r0 = ir.w2o[0], r1 = ir.w2o[1], r2 = ir.w2o[2];
-                float3 oo = mk3(r0.x * o.x + r0.y * o.y + r0.z * o.z + r0.w,
-                                r1.x * o.x + r1.y * o.y + r1.z * o.z + r1.w,
-                                r2.x * o.x + r2.y * o.y + r2.z * o.z + r2.w);
-                float3 od = mk3(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
-                                r2.x * d.x + r2.y * d.y + r2.z * d.z);
+                D3 O, Dd;
+                rs.get(O, Dd);
+                const double* m = ir.w2o_d;
+                float3 oo = mk3(float(m[0] * O.x + m[1] * O.y + m[2] * O.z + m[3]),
+                                float(m[4] * O.x + m[5] * O.y + m[6] * O.z + m[7]),
+                                float(m[8] * O.x + m[9] * O.y + m[10] * O.z + m[11]));
+                float3 od = mk3(float(m[0] * Dd.x + m[1] * Dd.y + m[2] * Dd.z),
+                                float(m[4] * Dd.x + m[5] * Dd.y + m[6] * Dd.z),
+                                float(m[8] * Dd.x + m[9] * Dd.y + m[10] * Dd.z));
                 cur.o = oo;
                 cur.inv = mk3(safe_inv(od.x), safe_inv(od.y), safe_inv(od.z));
                 wr = woop_setup(od);

@@ -85,6 +85,17 @@ __device__ inline void store_full(const WfArrays& a, uint32_t s, const PathState
     a.bounce[s] = p.bounce;
 }
 
+// The fp64 ray of a path slot, re-read from the path state when traversal enters
+// an instance (keeps it out of the traversal's registers).
+struct RayFromSlot {
+    const WfArrays& a;
+    uint32_t s;
+    __device__ void get(D3& O, D3& Dd) const {
+        O = d3(a.ox[s], a.oy[s], a.oz[s]);
+        Dd = d3(a.dx[s], a.dy[s], a.dz[s]);
+    }
+};
+
 // Appends `item` to queue q (count c) for the lanes with want set: one atomic per wave.
 __device__ inline uint32_t wave_append(bool want, uint32_t* c) {
     uint64_t m = __ballot(want);
@@ -155,8 +166,8 @@ void k_wf_extend(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_
         Rng4 rb{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
         AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
         HitInfo h;
-        bool found = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), flags, ah,
-                                                        s_stack + threadIdx.x, kWfBlock, h, pc.tr);
+        bool found = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, o, d, RayFromSlot{a, s}, float(kTMin), float(a.tfar[s]),
+                                                        flags, ah, s_stack + threadIdx.x, kWfBlock, h, pc.tr);
         if (kAnyHit) a.wb[s] = rb.w;
         a.h_inst[s] = found ? h.inst : kEmpty;
         a.h_prim[s] = h.prim;
@@ -226,9 +237,9 @@ void k_wf_shadow(FrameArgs fa, WfArrays a) {
         Rng4 shadow_rng{x, y, fa.seed_b, a.sq_wb[i]};
         AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
         HitInfo sh;
-        bool hit = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
-                                                      float(kShadowTMin), o.w, flags, sah, s_stack + threadIdx.x, kWfBlock,
-                                                      sh, pc.trs);
+        const float3 so = make_float3(o.x, o.y, o.z), sd = make_float3(d.x, d.y, d.z);
+        bool hit = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, so, sd, RayFromF32{so, sd}, float(kShadowTMin), o.w,
+                                                      flags, sah, s_stack + threadIdx.x, kWfBlock, sh, pc.trs);
         a.unocc[s] = hit ? 0u : 1u;
     }
     if (kStats) flush_counters(fa.counters, pc);

@@ -333,3 +333,20 @@ def test_watertight_triangle_test(oracle_lib):
                                                  B.dptr(v2), 0, 1e9, B.dptr(out))
     assert not oracle_lib.orc_intersect_triangle(B.dptr(v(0.2, 0.2, 1)), B.dptr(v(0, 0, -1)), B.dptr(v0), B.dptr(v1),
                                                  B.dptr(v2), 0, 0.5, B.dptr(out))
+
+
+def test_bilinear_filter_reproduces_constant_neighbourhoods_exactly(oracle_lib):
+    """Texture hardware filters with fixed-point weights, so four equal texels give
+    that value exactly; a metallic texture of 255 must give metallic == 1.0 (the
+    diffuse lobe then has exactly zero weight).  Summing four rounded weights does
+    not guarantee this; nested lerps do."""
+    from rustracer_amd import layouts as L
+    img = np.full((8, 8, 4), 255, np.uint8)
+    img[..., 1] = 77
+    out = np.zeros(4)
+    t = B.RtTexture(img.ctypes.data, 8, 8, L.FMT_RGBA8_UNORM, L.WRAP_REPEAT, L.WRAP_REPEAT, L.FILTER_LINEAR,
+                    L.FILTER_LINEAR)
+    rng = np.random.default_rng(5)
+    for u, v in rng.random((2000, 2)) * 3.0:
+        oracle_lib.orc_texture_sample(C.byref(t), u, v, B.dptr(out))
+        assert out[0] == 1.0 and out[2] == 1.0 and out[1] == 77 / 255
