@@ -162,6 +162,26 @@ typedef struct rt_trace_stats {
     uint64_t shadow_instance_visits; /* part of instance_visits spent on shadow rays */
 } rt_trace_stats;
 
+/* ------------------------------------------------------------ adjoint */
+
+/* Path-replay adjoint of rt_trace_rays: accumulates into grad_textures[i] the
+ * gradient of  sum_pixels dot(d_grad_accum, accum)  with respect to the texels of
+ * texture i, for every texture i with a non-NULL entry.  Such a texture must be
+ * RT_FMT_RGBA32F; its gradient buffer is a device array in the same layout
+ * (width * height * 4 floats), added to (not cleared).  Gradients flow into the
+ * base-colour texture (R, G, B) and the metallic-roughness texture (G roughness,
+ * B metallic) of metallic-roughness materials; other uses of a texture do not
+ * contribute.  Lobe choices, directions, sampling densities and Russian roulette
+ * are held at the values of the forward trace with the same bindings and UBO
+ * (the paths are replayed exactly), so the result is the derivative of the
+ * image's estimate with the sampling detached.  d_grad_accum is the device
+ * array dLoss/d(accum), width * height * 4 floats (alpha ignored).  At most 16
+ * bounces (RT_ERR_UNSUPPORTED above).  Replaces the reference's
+ * record_raytracing_commands for an optimisation loop; there is no adjoint in
+ * the reference (crates/examples/gltf_viewer/src/main.rs:244-270 traces only). */
+int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const float* d_grad_accum,
+                          uint32_t width, uint32_t height, float* const* grad_textures, rt_trace_stats* stats);
+
 /* ------------------------------------------------------------ options */
 
 enum {

@@ -17,6 +17,7 @@
 #include "lbvh.h"
 #include "render.h"
 #include "wavefront.h"
+#include "adjoint.h"
 
 using namespace rt;
 
@@ -99,6 +100,8 @@ struct rt_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     WavefrontState* wavefront = nullptr;
     bool flatten = true;  // RT_OPTION_FLATTEN_IDENTITY_INSTANCES
+    float** d_grad_ptrs = nullptr;  // adjoint: device copy of the gradient pointers
+    uint32_t grad_cap = 0;
     int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
@@ -291,6 +294,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->wavefront) wavefront_destroy(ctx->wavefront);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
+    if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_textures) (void)hipFree(ctx->d_textures);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -707,6 +711,59 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         stats->shadow_tri_tests = c[kCntShadowTris];
         stats->shadow_instance_visits = c[kCntShadowInstances];
         if (c[kCntBadIndex]) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a hit referenced an out-of-range prim info, material, index or vertex");
+    }
+    return RT_OK;
+}
+
+int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const float* d_grad_accum,
+                          uint32_t width, uint32_t height, float* const* grad_textures, rt_trace_stats* stats) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    int rc = validate_bindings(ctx, b);
+    if (rc) return rc;
+    if (!d_grad_accum) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "d_grad_accum is NULL");
+    if (width == 0 || height == 0 || uint64_t(width) * height > (1ull << 31))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    if (b->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, "HasSky (skybox miss) is not supported");
+    if (b->ubo->number_of_bounces > kMaxAdjointBounces)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint replays at most 16 bounces");
+    const uint32_t nt = b->num_textures;
+    if (nt && !grad_textures) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "grad_textures is NULL");
+    for (uint32_t i = 0; i < nt; i++)
+        if (grad_textures[i] && b->textures[i].format != RT_FMT_RGBA32F)
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has a gradient buffer but is not RGBA32F");
+    DeviceGuard g(ctx->device);
+    rc = upload_textures(ctx, b);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    if (nt > ctx->grad_cap) {
+        if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
+        ctx->d_grad_ptrs = nullptr;
+        ctx->grad_cap = 0;
+        RT_HIP(ctx, hipMalloc(&ctx->d_grad_ptrs, sizeof(float*) * nt), "hipMalloc");
+        ctx->grad_cap = nt;
+    }
+    if (nt) {
+        // the pointer table must stay valid until the kernel has read it
+        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+        RT_HIP(ctx, hipMemcpy(ctx->d_grad_ptrs, grad_textures, sizeof(float*) * nt, hipMemcpyHostToDevice), "copy gradient pointers");
+    }
+    FrameArgs fa;
+    make_frame(ctx, it->second, b, nullptr, width, height, &fa);
+    if (stats) {
+        RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
+        RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
+    }
+    rc = launch_adjoint(fa, d_grad_accum, ctx->d_grad_ptrs, nt, st, !fa.fully_opaque);
+    if (rc) return hip_fail(ctx, hipError_t(rc), "adjoint launch");
+    if (stats) {
+        RT_HIP(ctx, hipEventRecord(ctx->ev1, st), "hipEventRecord");
+        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+        float ms = 0.f;
+        RT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+        std::memset(stats, 0, sizeof(*stats));
+        stats->kernel_ms = ms;
     }
     return RT_OK;
 }

@@ -491,6 +491,7 @@ struct BounceOut {
     // read by the adjoint only
     D3 emit, nee;     // emittance, NEE term (before the shadow test)
     D3 mult;          // factor from this bounce's throughput to the next one's
+    double rr;        // Russian-roulette probability applied (1: none)
 };
 
 // Closest-hit observer of shade_bounce (the adjoint records the shaded vertex).
@@ -539,6 +540,7 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
     r.emit = ray.emittance;
     r.nee = nee;
     r.mult = d3(0.0);
+    r.rr = 1.0;
     p.hit_point = ray.hit_point;
     p.volume_dis = ray.volume_dis;
     p.wb = ray.rng.w;
@@ -555,6 +557,7 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
     }
     p.thr = p.thr * ray.hit_value;
     r.mult = ray.hit_value / rr;
+    r.rr = rr;
     if (!ray.need_scatter || ray.t < 0.0) return;
     p.o = ray.hit_point;
     p.d = ray.scatter_dir;

@@ -37,7 +37,7 @@ RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT = 0x4, 0x8
 EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", "rt_build_blas_batch",
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
-           "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info"]
+           "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint"]
 
 
 class RtError(RuntimeError):
@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
         "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
         "rt_set_option": (i, [vp, u, u64]), "rt_get_stage_times": (i, [vp, vp]),
         "rt_tlas_flat_info": (i, [vp, u64, vp]),
+        "rt_trace_rays_adjoint": (i, [vp, u64, vp, vp, u, u, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -200,6 +201,15 @@ class Buffers:
             h, w = t.data.shape[:2]
             self._tex[k] = Texture(self.textures[k].data_ptr(), w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter,
                                    t.min_filter)
+
+    def set_texture(self, k, tensor):
+        """Points texture slot k at `tensor` (device, (h, w, 4), the slot's format:
+        uint8 for RGBA8, float32 for RGBA32F): the renderer then reads the tensor's
+        memory directly, e.g. a parameter texture an optimiser updates in place."""
+        t = self.scene.textures[k]
+        h, w = tensor.shape[:2]
+        self.textures[k] = tensor
+        self._tex[k] = Texture(tensor.data_ptr(), w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter, t.min_filter)
 
     def bindings(self, ubo):
         """rt_bindings for one trace; `ubo` (UBO record) must stay alive during the call."""
@@ -311,6 +321,23 @@ def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERN
     if st is None:
         return None
     return {n: getattr(st, n) for n, _ in TraceStats._fields_}
+
+
+def trace_rays_adjoint(ctx, tlas, buffers, ubo, grad_accum, width, height, grads, stats=False):
+    """rt_trace_rays_adjoint: adds to grads[k] (float32 device tensors shaped like
+    RGBA32F texture k) the gradient of sum(grad_accum * accum) of the trace with the
+    same buffers and UBO.  `grads` maps texture index -> tensor."""
+    b = buffers.bindings(ubo)
+    n = len(buffers.scene.textures)
+    ptrs = (C.c_void_p * max(1, n))()
+    for k, g in grads.items():
+        ptrs[k] = g.data_ptr()
+    st = TraceStats() if stats else None
+    ctx.check(ctx.lib.rt_trace_rays_adjoint(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(grad_accum.data_ptr()),
+                                            width, height, ptrs, C.byref(st) if st is not None else None))
+    if st is None:
+        return None
+    return {n_: getattr(st, n_) for n_, _ in TraceStats._fields_}
 
 
 def trace_ray_queries(ctx, tlas, rays, flags=0, buffers=None, ubo=None):

@@ -402,6 +402,44 @@ def displaced_sphere(width=1920, height=1080, spp=64, bounces=8, seed=1, freq=31
     return b.finish("displaced_sphere", inst, plights, [L.default_dlight()], ubo, width, height)
 
 
+def cornell_params(width=512, height=512, spp=16, bounces=4, tex_size=512, seed=0, target=False):
+    """The Cornell-box scene with parameter textures on the white walls (floor,
+    ceiling, back): an RGBA32F base-colour texture (slot 1) and an RGBA32F
+    metallic-roughness texture (slot 2, G roughness, B metallic), the material's
+    factors 1.  Parameters start at 0.5 everywhere; target=True gives seeded noise
+    textures instead (the image an optimisation fits).  512x512, 16 spp, depth 4
+    (no Russian roulette before bounce 4)."""
+    sc = cornell_sphere(width=width, height=height, spp=spp, bounces=bounces, seed=seed)
+    if target:
+        rng = np.random.default_rng(1000 + seed)
+        base = noise_image(rng, tex_size, 4, cells=(4, 16))
+        base[..., :3] = 0.1 + 0.85 * base[..., :3]
+        mr = noise_image(rng, tex_size, 4, cells=(4, 16))
+        mr[..., 1] = 0.2 + 0.75 * mr[..., 1]
+        mr[..., 2] = 0.05 + 0.9 * mr[..., 2]
+    else:
+        base = np.full((tex_size, tex_size, 4), 0.5)
+        mr = np.full((tex_size, tex_size, 4), 0.5)
+    base[..., 3] = 1.0
+    mr[..., 0] = 1.0
+    mr[..., 3] = 1.0
+    sc.textures = [Texture(np.full((1, 1, 4), 255, np.uint8), L.FMT_RGBA8_UNORM),
+                   Texture(base.astype(np.float32), L.FMT_RGBA32F),
+                   Texture(mr.astype(np.float32), L.FMT_RGBA32F)]
+    white = int(sc.prim_infos["material_id"][int(sc.instances["acceleration_structure_reference"][0])])
+    m = sc.materials[white]
+    m["base_color"] = (1.0, 1.0, 1.0, 1.0)
+    m["metallic_factor"] = 1.0
+    m["roughness_factor"] = 1.0
+    m["base_color_texture"]["index"] = 1
+    m["base_color_texture"]["tex_coord"] = 0
+    m["metallic_roughness_texture"]["index"] = 2
+    m["metallic_roughness_texture"]["tex_coord"] = 0
+    sc.materials[white] = m
+    sc.name = "cornell_params_target" if target else "cornell_params"
+    return sc
+
+
 def _interp_matrix(size, cells):
     """(size, cells) matrix that resamples a periodic lattice of `cells` values to
     `size` texel centres with smoothstep weights (rows sum to 1)."""
@@ -552,4 +590,4 @@ def textured_instances(width=1920, height=1080, spp=32, bounces=6, seed=2, num_i
 
 
 WORKLOADS = {"cornell_sphere": cornell_sphere, "displaced_sphere": displaced_sphere,
-             "textured_instances": textured_instances}
+             "textured_instances": textured_instances, "cornell_params": cornell_params}

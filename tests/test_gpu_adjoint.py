@@ -1,0 +1,120 @@
+"""Path-replay adjoint (rt_trace_rays_adjoint) against central finite differences
+of the FP64 oracle's detached evaluation: the oracle renders with the sampling
+(lobe choices, directions, densities, Russian roulette) of the unperturbed
+parameters and the BRDF values of the perturbed ones, which is the function whose
+derivative the adjoint computes.  Scene: the Cornell box with RGBA32F base-colour
+and metallic-roughness parameter textures on the white walls (restates the
+closest-hit shading of crates/examples/gltf_viewer/shaders/RayTracing.rchit
+:136-416 and the raygen loop of RayTracing.rgen :85-128)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import binding as B
+from rustracer_amd import diff, rt
+from rustracer_amd import scenes as S
+from tests import gpu_kit as G
+
+pytestmark = pytest.mark.gpu
+H_FD = 1e-4
+
+
+def _small():
+    return S.cornell_params(width=48, height=48, spp=4, bounces=4, tex_size=12)
+
+
+def _adjoint(ctx, ds, g):
+    sc = ds.scene
+    grads = {k: torch.zeros(sc.textures[k].data.shape, dtype=torch.float32, device="cuda") for k in (1, 2)}
+    rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, sc.width, sc.height, grads)
+    ctx.sync()
+    return {k: v.cpu().numpy().astype(np.float64) for k, v in grads.items()}
+
+
+def test_adjoint_matches_finite_differences_on_256_texels(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "cornell_params_small", _small)
+    sc = ds.scene
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=(sc.height, sc.width, 4)).astype(np.float32)
+    g[..., 3] = 0
+    adj = _adjoint(gpu_ctx, ds, torch.from_numpy(g).cuda())
+    # forward images agree (no path flips between GPU and oracle on this scene)
+    acc = rt.new_accumulation(sc.width, sc.height)
+    rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, acc, sc.width, sc.height)
+    gpu_ctx.sync()
+    o = B.OracleScene(sc)
+    ref, _ = o.render()
+    assert G.rel_rmse(acc.cpu().numpy()[..., :3].astype(np.float64), ref[..., :3]) < 1e-6
+    base = [t.data.astype(np.float32) for t in sc.textures]
+    g64 = g.astype(np.float64)
+    # 256 random (texture, texel, channel): base colour R, G, B and roughness, metallic
+    entries = []
+    size = sc.textures[1].data.shape[0]
+    for _ in range(256):
+        k = int(rng.integers(1, 3))
+        ch = int(rng.integers(0, 3)) if k == 1 else int(rng.integers(1, 3))
+        entries.append((k, int(rng.integers(size)), int(rng.integers(size)), ch))
+    fd = np.zeros(len(entries))
+    for i, (k, ty, tx, ch) in enumerate(entries):
+        vals, steps = [], []
+        for sgn in (1, -1):
+            ev = [b.copy() for b in base]
+            v0 = ev[k][ty, tx, ch]
+            ev[k][ty, tx, ch] = np.float32(float(v0) + sgn * H_FD)  # the texture is fp32: use the step it holds
+            steps.append(float(ev[k][ty, tx, ch]) - float(v0))
+            o.set_eval_textures(ev)
+            img, _ = o.render()
+            vals.append(float((img * g64).sum()))
+        fd[i] = (vals[0] - vals[1]) / (steps[0] - steps[1])
+    o.set_eval_textures(None)
+    a = np.array([adj[k][ty, tx, ch] for k, ty, tx, ch in entries])
+    scale = np.abs(fd).max()
+    print("adjoint vs FD: max |fd|", scale, "max abs err", np.abs(a - fd).max(), "nonzero", int((np.abs(fd) > 0).sum()))
+    assert scale > 0 and (np.abs(fd) > 1e-6 * scale).sum() > 200
+    assert np.all(np.abs(a - fd) <= 2e-4 * scale + 2e-4 * np.abs(fd)), np.abs(a - fd).max()
+    # channels the adjoint does not own stay zero (base alpha; metallic-roughness R, A)
+    assert np.all(adj[1][..., 3] == 0) and np.all(adj[2][..., 0] == 0) and np.all(adj[2][..., 3] == 0)
+
+
+def test_adjoint_rejects_gradient_buffers_for_non_float_textures(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "cornell_params_small", _small)
+    sc = ds.scene
+    g = torch.zeros((sc.height, sc.width, 4), dtype=torch.float32, device="cuda")
+    bad = {0: torch.zeros((1, 1, 4), dtype=torch.float32, device="cuda")}  # slot 0 is RGBA8
+    with pytest.raises(rt.RtError):
+        rt.trace_rays_adjoint(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, g, sc.width, sc.height, bad)
+
+
+def test_autograd_function_and_an_optimisation_step(gpu_ctx):
+    """loss = sum((accum - target)^2): torch.autograd gives the adjoint's gradient for
+    g = 2 (accum - target), and gradient steps on the parameters lower the loss."""
+    sc = S.cornell_params(width=64, height=64, spp=4, bounces=4, tex_size=16)
+    tgt_scene = S.cornell_params(width=64, height=64, spp=4, bounces=4, tex_size=16, target=True)
+    tds = G.DeviceScene(gpu_ctx, tgt_scene)
+    target = rt.new_accumulation(64, 64)
+    rt.trace_rays(gpu_ctx, tds.tlas, tds.buffers, tgt_scene.ubo, target, 64, 64)
+    gpu_ctx.sync()
+    dscene = diff.DifferentiableScene(gpu_ctx, sc, [1, 2])
+    img = dscene.render()
+    loss = ((img - target) ** 2).sum()
+    loss.backward()
+    g = (2 * (img - target)).detach().contiguous()
+    ref = {k: torch.zeros_like(p) for k, p in zip([1, 2], dscene.params)}
+    rt.trace_rays_adjoint(gpu_ctx, dscene.tlas, dscene.buffers, sc.ubo, g, 64, 64, ref)
+    gpu_ctx.sync()
+    for k, p in zip([1, 2], dscene.params):
+        assert torch.allclose(p.grad, ref[k], rtol=1e-5, atol=1e-6 * float(ref[k].abs().max()))
+    losses = [float(loss)]
+    opt = torch.optim.Adam(dscene.params, lr=0.02)
+    for _ in range(6):
+        opt.zero_grad()
+        img = dscene.render()
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            for p in dscene.params:
+                p.clamp_(0.02, 1.0)
+        losses.append(float(loss))
+    print("losses", losses)
+    assert losses[-1] < 0.9 * losses[0]
