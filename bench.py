@@ -95,6 +95,94 @@ def cpu_baseline(scene, target_s, threads):
                       f"({st['rays']} rays in {dt:.1f} s), FP32 C++ restatement, OpenMP"}
 
 
+def adjoint_iteration(ctx, steps, warmup):
+    """One optimisation iteration of the parameter-texture workload: the Cornell box
+    at 512x512, 16 spp, depth 4 with 512x512 RGBA32F base-colour and
+    metallic-roughness parameter textures: forward trace, L2 loss gradient, adjoint
+    (path replay).  Returns milliseconds per iteration and its split."""
+    import numpy as np
+    import torch
+    from rustracer_amd import diff, rt, scenes
+    sc = scenes.cornell_params()
+    tsc = scenes.cornell_params(target=True)
+    tbuf = rt.Buffers(tsc)
+    _, ttlas = rt.create_as(ctx, tsc, tbuf)
+    target = rt.new_accumulation(tsc.width, tsc.height)
+    rt.trace_rays(ctx, ttlas, tbuf, tsc.ubo, target, tsc.width, tsc.height, mode=rt.MODE_WAVEFRONT)
+    ds = diff.DifferentiableScene(ctx, sc, [1, 2])
+    W, H = sc.width, sc.height
+    grads = {k: torch.zeros_like(p) for k, p in zip(ds.slots, ds.params)}
+    acc = rt.new_accumulation(W, H)
+
+    def once():
+        rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=rt.MODE_WAVEFRONT)
+        ev.record()
+        g = (2.0 * (acc - target)).contiguous()
+        rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads)
+
+    for _ in range(warmup):
+        ev = torch.cuda.Event(enable_timing=True)
+        once()
+    torch.cuda.synchronize()
+    times, fwd = [], []
+    for _ in range(steps):
+        e0, ev, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        once()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+        fwd.append(e0.elapsed_time(ev))
+    return {"ms_per_iter": round(float(np.median(times)), 3), "forward_ms": round(float(np.median(fwd)), 3),
+            "adjoint_ms": round(float(np.median(times)) - float(np.median(fwd)), 3),
+            "workload": "Cornell box 512x512, 16 spp, depth 4, 512^2 RGBA32F base-colour + metallic-roughness "
+                        "parameter textures; forward + L2 gradient + path-replay adjoint"}
+
+
+def batch_benchmark(ctx, steps):
+    """The many-scene batch: 64 seeded scenes (room + displaced sphere of 1k-500k
+    triangles, 128^2 - 1024^2, 16 spp, depth 6) traced by one rt_trace_rays_batch
+    (shared persistent queue kernels) against the same frames traced one
+    rt_trace_rays each.  Returns Mrays/s of both."""
+    import numpy as np
+    import torch
+    from rustracer_amd import rt, scenes
+    t0 = time.perf_counter()
+    scs = scenes.batch_scenes()
+    gen_s = time.perf_counter() - t0
+    items, keep = [], []
+    for sc in scs:
+        buf = rt.Buffers(sc)
+        blases, tlas = rt.create_as(ctx, sc, buf)
+        acc = rt.new_accumulation(sc.width, sc.height)
+        keep.append((buf, blases, tlas, acc))
+        items.append((tlas, buf, sc.ubo, acc, sc.width, sc.height))
+    ctx.sync()
+    st = rt.trace_rays_batch(ctx, items, stats=True)
+    rays = st["rays"]
+    rt.trace_rays_batch(ctx, items)
+    t_batch = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rt.trace_rays_batch(ctx, items)
+        torch.cuda.synchronize()
+        t_batch.append(time.perf_counter() - t0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for tlas, buf, ubo, acc, w, h in items:
+        rt.trace_rays(ctx, tlas, buf, ubo, acc, w, h, mode=rt.MODE_WAVEFRONT)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    tb = float(np.median(t_batch))
+    return {"scenes": len(scs), "triangles": int(sum(s.num_triangles for s in scs)),
+            "pixels": int(sum(s.width * s.height for s in scs)), "rays": int(rays),
+            "batch_ms": round(tb * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
+            "batch_mrays_per_s": round(rays / tb / 1e6, 1), "sequential_mrays_per_s": round(rays / t_seq / 1e6, 1),
+            "scene_gen_s": round(gen_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +192,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path-tracing frame shards by image tiles, but this build runs "
@@ -207,6 +297,10 @@ def main():
                      "launch_ms": round(launch_ms, 4), "algorithmic_bytes": int(bytes_per_launch),
                      "stage_ms": {k: round(v, 3) for k, v in stage.items()}},
     }
+    if not args.no_adjoint:
+        line["config"]["adjoint"] = adjoint_iteration(ctx, max(3, args.steps), 1)
+    if not args.no_batch:
+        line["config"]["batch"] = batch_benchmark(ctx, 2)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, threads)

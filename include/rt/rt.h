@@ -47,8 +47,9 @@ typedef uint64_t rt_tlas;
 
 /* ---------------------------------------------------------------- context */
 
-/* Creates a context on HIP device `hip_device`.  `hip_stream` is a hipStream_t to
- * enqueue on, or NULL for a stream owned by the context. */
+/* Creates a context on HIP device `hip_device`.  `hip_stream` is the hipStream_t
+ * the context enqueues on; NULL means the device's default (null) stream, so the
+ * calls are ordered with whatever else the caller enqueues there. */
 int rt_create(rt_ctx** out, int hip_device, void* hip_stream);
 void rt_destroy(rt_ctx* ctx);
 /* Message of the last failed call on this context ("" if none).  ctx may be NULL
@@ -161,6 +162,26 @@ typedef struct rt_trace_stats {
     uint64_t shadow_tri_tests;       /* part of tri_tests spent on shadow rays */
     uint64_t shadow_instance_visits; /* part of instance_visits spent on shadow rays */
 } rt_trace_stats;
+
+/* ------------------------------------------------------------ batch */
+
+/* One frame of a batch: the arguments of one rt_trace_rays call. */
+typedef struct rt_batch_item {
+    rt_tlas tlas;
+    const rt_bindings* bindings;  /* host pointer; its UBO and texture arrays too */
+    float* accum;                 /* device rgba32f image, width x height */
+    uint32_t width, height;
+} rt_batch_item;
+
+/* Traces `count` frames (different scenes, resolutions, UBOs) as one set of
+ * persistent wavefront queue kernels: the frames' path slots are laid end to end
+ * and every kernel serves all of them, so small frames still fill the GPU.  Each
+ * accumulation image equals what rt_trace_rays(mode) gives for that item alone.
+ * RT_MODE_MEGAKERNEL traces the items one after another.  The reference renders
+ * one scene per vkCmdTraceRaysKHR (crates/examples/gltf_viewer/src/main.rs:244-270);
+ * this is the many-scene form of the same call.  Synchronises the stream. */
+int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count, uint32_t mode,
+                        rt_trace_stats* stats);
 
 /* ------------------------------------------------------------ adjoint */
 

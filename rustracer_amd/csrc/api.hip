@@ -85,7 +85,6 @@ bool is_identity(const float (&m)[3][4]) {
 struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     std::string err;
     uint64_t next_handle = 1;
     std::unordered_map<uint64_t, BlasEntry> blas;
@@ -101,6 +100,8 @@ struct rt_ctx {
     WavefrontState* wavefront = nullptr;
     bool flatten = true;  // RT_OPTION_FLATTEN_IDENTITY_INSTANCES
     float** d_grad_ptrs = nullptr;  // adjoint: device copy of the gradient pointers
+    rt_texture* d_batch_tex = nullptr;  // batch: texture descriptors of every item, end to end
+    size_t batch_tex_cap = 0;
     uint32_t grad_cap = 0;
     int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
     bool stage_timing = false;
@@ -255,13 +256,9 @@ int rt_create(rt_ctx** out, int hip_device, void* hip_stream) {
     auto* ctx = new rt_ctx();
     ctx->device = hip_device;
     DeviceGuard g(hip_device);
-    if (hip_stream) {
-        ctx->stream = (hipStream_t)hip_stream;
-    } else {
-        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) { delete ctx; return fail(nullptr, RT_ERR_HIP, "hipStreamCreate failed"); }
-        ctx->own_stream = true;
-    }
+    // NULL: the device's default stream, which orders the calls with other work
+    // the caller enqueues there (e.g. a framework's default stream)
+    ctx->stream = (hipStream_t)hip_stream;
     ctx->d_counters = dev_alloc<unsigned long long>(kNumCounters, &e);
     if (e == hipSuccess) ctx->d_srgb = dev_alloc<float>(256, &e);
     if (e == hipSuccess) ctx->d_err = dev_alloc<uint32_t>(1, &e);
@@ -295,11 +292,11 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
     if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
+    if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_textures) (void)hipFree(ctx->d_textures);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -647,6 +644,37 @@ int rt_tlas_flat_info(rt_ctx* ctx, rt_tlas tlas, rt_bvh_info* out) {
 }
 
 // ----------------------------------------------------------------- trace
+// Counters and device time of the trace just enqueued (ev0 recorded before it).
+static int read_stats(rt_ctx* ctx, rt_trace_stats* stats) {
+    hipStream_t st = ctx->stream;
+    RT_HIP(ctx, hipEventRecord(ctx->ev1, st), "hipEventRecord");
+    unsigned long long c[kNumCounters];
+    RT_HIP(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost, st), "copy counters");
+    RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    float ms = 0.f;
+    RT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+    std::memset(stats, 0, sizeof(*stats));
+    stats->primary_rays = c[kCntPrimary];
+    stats->extension_rays = c[kCntExtension];
+    stats->shadow_rays = c[kCntShadow];
+    stats->rays = c[kCntPrimary] + c[kCntExtension] + c[kCntShadow];
+    stats->node_visits = c[kCntNodes];
+    stats->tri_tests = c[kCntTris];
+    stats->instance_visits = c[kCntInstances];
+    stats->shade_events = c[kCntShade];
+    stats->anyhit_calls = c[kCntAnyHit];
+    stats->tex_fetches = c[kCntTex];
+    stats->light_candidates = c[kCntLights];
+    stats->stack_overflows = c[kCntOverflow];
+    stats->kernel_ms = ms;
+    stats->shadow_node_visits = c[kCntShadowNodes];
+    stats->shadow_tri_tests = c[kCntShadowTris];
+    stats->shadow_instance_visits = c[kCntShadowInstances];
+    if (c[kCntBadIndex])
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a hit referenced an out-of-range prim info, material, index or vertex");
+    return RT_OK;
+}
+
 int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum, uint8_t* out_rgba8, uint32_t width,
                   uint32_t height, uint32_t mode, rt_trace_stats* stats) {
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
@@ -686,32 +714,85 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
     if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
-    if (stats) {
-        RT_HIP(ctx, hipEventRecord(ctx->ev1, st), "hipEventRecord");
-        unsigned long long c[kNumCounters];
-        RT_HIP(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost, st), "copy counters");
-        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
-        float ms = 0.f;
-        RT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
-        std::memset(stats, 0, sizeof(*stats));
-        stats->primary_rays = c[kCntPrimary];
-        stats->extension_rays = c[kCntExtension];
-        stats->shadow_rays = c[kCntShadow];
-        stats->rays = c[kCntPrimary] + c[kCntExtension] + c[kCntShadow];
-        stats->node_visits = c[kCntNodes];
-        stats->tri_tests = c[kCntTris];
-        stats->instance_visits = c[kCntInstances];
-        stats->shade_events = c[kCntShade];
-        stats->anyhit_calls = c[kCntAnyHit];
-        stats->tex_fetches = c[kCntTex];
-        stats->light_candidates = c[kCntLights];
-        stats->stack_overflows = c[kCntOverflow];
-        stats->kernel_ms = ms;
-        stats->shadow_node_visits = c[kCntShadowNodes];
-        stats->shadow_tri_tests = c[kCntShadowTris];
-        stats->shadow_instance_visits = c[kCntShadowInstances];
-        if (c[kCntBadIndex]) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a hit referenced an out-of-range prim info, material, index or vertex");
+    if (stats) return read_stats(ctx, stats);
+    return RT_OK;
+}
+
+int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count, uint32_t mode,
+                        rt_trace_stats* stats) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (count && !items) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "items is NULL");
+    if (mode != RT_MODE_MEGAKERNEL && mode != RT_MODE_WAVEFRONT) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown mode");
+    size_t ntex = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const rt_batch_item& it = items[i];
+        std::string where = "item " + std::to_string(i) + ": ";
+        int rc = validate_bindings(ctx, it.bindings);
+        if (rc) return fail(ctx, rc, where + ctx->err);
+        if (!it.accum) return fail(ctx, RT_ERR_INVALID_ARGUMENT, where + "accum is NULL");
+        if (it.width == 0 || it.height == 0 || uint64_t(it.width) * it.height > (1ull << 31))
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, where + "width/height out of range");
+        if (ctx->tlas.find(it.tlas) == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, where + "unknown TLAS handle");
+        if (it.bindings->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, where + "HasSky (skybox miss) is not supported");
+        ntex += it.bindings->num_textures;
     }
+    if (count == 0) return RT_OK;
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    // texture descriptors of every item, end to end
+    RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (ntex > ctx->batch_tex_cap) {
+        if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
+        ctx->d_batch_tex = nullptr;
+        ctx->batch_tex_cap = 0;
+        hipError_t e;
+        ctx->d_batch_tex = dev_alloc<rt_texture>(ntex, &e);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc textures");
+        ctx->batch_tex_cap = ntex;
+    }
+    std::vector<FrameArgs> frames(count);
+    size_t toff = 0;
+    bool any_hit = false;
+    for (uint32_t i = 0; i < count; i++) {
+        const rt_batch_item& it = items[i];
+        make_frame(ctx, ctx->tlas.at(it.tlas), it.bindings, it.accum, it.width, it.height, &frames[i]);
+        uint32_t n = it.bindings->num_textures;
+        if (n)
+            RT_HIP(ctx, hipMemcpy(ctx->d_batch_tex + toff, it.bindings->textures, sizeof(rt_texture) * n,
+                                  hipMemcpyHostToDevice), "copy textures");
+        frames[i].sv.textures = ctx->d_batch_tex + toff;
+        toff += n;
+        any_hit = any_hit || !frames[i].fully_opaque;
+    }
+    if (stats) {
+        RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
+        RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
+    }
+    StageTimer* tm = ctx->stage_timing ? &ctx->timer : nullptr;
+    ctx->timer.used = 0;
+    int rc = 0;
+    if (mode == RT_MODE_MEGAKERNEL) {
+        for (uint32_t i = 0; i < count && rc == 0; i++)
+            rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, stats != nullptr, tm);
+        if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
+    } else {
+        if (!ctx->wavefront) {
+            ctx->wavefront = wavefront_create();
+            if (!ctx->wavefront) return fail(ctx, RT_ERR_OUT_OF_MEMORY, "wavefront state");
+        }
+        // frames without samples or bounces take the megakernel (it applies the reset)
+        for (uint32_t i = 0; i < count && rc == 0; i++)
+            if (frames[i].samples == 0u || frames[i].bounces == 0u)
+                rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, stats != nullptr);
+        if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
+        std::string err;
+        rc = launch_wavefront_batch(ctx->wavefront, frames.data(), count, st, any_hit, stats != nullptr, ctx->variant, tm,
+                                    &err);
+        if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
+        if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
+    }
+    if (stats) return read_stats(ctx, stats);
+    RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
     return RT_OK;
 }
 
@@ -739,6 +820,7 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     hipStream_t st = ctx->stream;
     if (nt > ctx->grad_cap) {
         if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
+    if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
         ctx->d_grad_ptrs = nullptr;
         ctx->grad_cap = 0;
         RT_HIP(ctx, hipMalloc(&ctx->d_grad_ptrs, sizeof(float*) * nt), "hipMalloc");

@@ -20,5 +20,11 @@ constexpr int kWavefrontUnsupported = -1000;
 int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, int variant,
                      StageTimer* tm, std::string* err);
 int wavefront_num_variants();
+// Renders `count` frames (host array) in one set of queue kernels: their slots are
+// laid end to end and every kernel serves all of them.  Each frame's image is
+// the one launch_wavefront gives it alone.  Frames with no samples or bounces
+// are skipped (the caller renders them with the megakernel).
+int launch_wavefront_batch(WavefrontState* s, const FrameArgs* frames, uint32_t count, hipStream_t st, bool any_hit,
+                           bool stats, int variant, StageTimer* tm, std::string* err);
 
 }  // namespace rt

@@ -108,23 +108,54 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* c) {
     return base + uint32_t(__popcll(m & lanemask_lt()));
 }
 
+// Where a slot's frame arguments come from: one frame (kernel argument), or a
+// batch of frames in device memory with the slots of frame i in
+// [base[i], base[i + 1]) (rt_trace_rays_batch).
+struct OneFrame {
+    FrameArgs fa;
+    __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
+        slot_pixel(fa, s, x, y);
+        return fa;
+    }
+    __device__ unsigned long long* counters() const { return fa.counters; }
+};
+struct ManyFrames {
+    const FrameArgs* fas;
+    const uint32_t* base;  // n + 1 slot offsets
+    uint32_t n;
+    unsigned long long* cnt;
+    __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
+        uint32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            uint32_t m = (lo + hi) >> 1;
+            if (base[m] <= s) lo = m; else hi = m;
+        }
+        const FrameArgs& fa = fas[lo];
+        slot_pixel(fa, s - base[lo], x, y);
+        return fa;
+    }
+    __device__ unsigned long long* counters() const { return cnt; }
+};
+
 __device__ inline void write_accum(const FrameArgs& fa, uint32_t x, uint32_t y, D3 rad) {
     float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
     float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
     *acc = make_float4(prev.x + float(rad.x), prev.y + float(rad.y), prev.z + float(rad.z), 0.f);
 }
 
-__global__ __launch_bounds__(kWfBlock) void k_wf_init(FrameArgs fa, WfArrays a, uint32_t nslots) {
+template <class Frames>
+__global__ __launch_bounds__(kWfBlock) void k_wf_init(Frames fr, WfArrays a, uint32_t nslots) {
     uint32_t* q = a.q0;
     for (uint32_t base = blockIdx.x * kWfBlock; base < nslots; base += gridDim.x * kWfBlock) {
         uint32_t s = base + threadIdx.x;
         uint32_t x = 0, y = 0;
         bool valid = s < nslots;
         if (valid) {
-            slot_pixel(fa, s, x, y);
+            const FrameArgs& fa = fr.at(s, x, y);
             valid = x < fa.width && y < fa.height;
         }
         if (valid) {
+            const FrameArgs& fa = fr.at(s, x, y);
             PathState p;
             init_path(fa, x, y, p);
             begin_sample(fa, x, y, p);
@@ -145,21 +176,21 @@ struct TravVariant {
 constexpr TravVariant kTravVariants[] = {{32, 5}, {16, 6}, {16, 8}, {64, 2}};
 constexpr int kNumTravVariants = int(sizeof(kTravVariants) / sizeof(kTravVariants[0]));
 
-template <bool kAnyHit, bool kStats, int kDepth, int kWaves>
+template <class Frames, bool kAnyHit, bool kStats, int kDepth, int kWaves>
 __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
-void k_wf_extend(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in, uint32_t* zero0, uint32_t* zero1) {
+void k_wf_extend(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in, uint32_t* zero0, uint32_t* zero1) {
     __shared__ uint32_t s_stack[kDepth * kWfBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *zero0 = 0u;  // next path queue
         *zero1 = 0u;  // shadow queue
     }
     const uint32_t n = *n_in;
-    const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
     PathCounters pc;
     for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
         uint32_t s = q[i];
         uint32_t x, y;
-        slot_pixel(fa, s, x, y);
+        const FrameArgs& fa = fr.at(s, x, y);
+        const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
         float3 o = make_float3(float(a.ox[s]), float(a.oy[s]), float(a.oz[s]));
         float3 d = make_float3(float(a.dx[s]), float(a.dy[s]), float(a.dz[s]));
         if (kStats) { if (a.bounce[s] == 0u) pc.primary++; else pc.extension++; }
@@ -176,11 +207,11 @@ void k_wf_extend(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_
         a.h_u[s] = h.u;
         a.h_v[s] = h.v;
     }
-    if (kStats) flush_counters(fa.counters, pc);
+    if (kStats) flush_counters(fr.counters(), pc);
 }
 
-template <bool kStats>
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in) {
+template <class Frames, bool kStats>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in) {
     const uint32_t n = *n_in;
     PathCounters pc;
     for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
@@ -192,7 +223,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(FrameArgs fa, WfArrays a,
         if (active) {
             s = q[i];
             uint32_t x, y;
-            slot_pixel(fa, s, x, y);
+            const FrameArgs& fa = fr.at(s, x, y);
             PathState p;
             load_full(a, s, p);
             HitInfo h;
@@ -218,21 +249,21 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(FrameArgs fa, WfArrays a,
             a.sq_wb[idx] = r.shadow_wb;
         }
     }
-    if (kStats) flush_counters(fa.counters, pc);
+    if (kStats) flush_counters(fr.counters(), pc);
 }
 
-template <bool kAnyHit, bool kStats, int kDepth, int kWaves>
+template <class Frames, bool kAnyHit, bool kStats, int kDepth, int kWaves>
 __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
-void k_wf_shadow(FrameArgs fa, WfArrays a) {
+void k_wf_shadow(Frames fr, WfArrays a) {
     __shared__ uint32_t s_stack[kDepth * kWfBlock];
     const uint32_t n = a.counts[2];
-    const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
     PathCounters pc;
     for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
         float4 o = a.sq_o[i], d = a.sq_d[i];
         uint32_t s = __float_as_uint(d.w);
         uint32_t x, y;
-        slot_pixel(fa, s, x, y);
+        const FrameArgs& fa = fr.at(s, x, y);
+        const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
         if (kStats) pc.shadow++;
         Rng4 shadow_rng{x, y, fa.seed_b, a.sq_wb[i]};
         AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
@@ -242,10 +273,11 @@ void k_wf_shadow(FrameArgs fa, WfArrays a) {
                                                       flags, sah, s_stack + threadIdx.x, kWfBlock, sh, pc.trs);
         a.unocc[s] = hit ? 0u : 1u;
     }
-    if (kStats) flush_counters(fa.counters, pc);
+    if (kStats) flush_counters(fr.counters(), pc);
 }
 
-__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(FrameArgs fa, WfArrays a, const uint32_t* q, const uint32_t* n_in,
+template <class Frames>
+__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in,
                                                           uint32_t* q_out, uint32_t* n_out) {
     const uint32_t n = *n_in;
     for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
@@ -255,7 +287,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(FrameArgs fa, WfArrays 
         if (i < n) {
             s = q[i];
             uint32_t x, y;
-            slot_pixel(fa, s, x, y);
+            const FrameArgs& fa = fr.at(s, x, y);
             uint32_t f = a.flags[s];
             bool unocc = (f & kFlagShadow) && a.unocc[s];
             D3 rad = d3(a.rx[s], a.ry[s], a.rz[s]);
@@ -319,39 +351,37 @@ __global__ void k_zero_counts(uint32_t* c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0u;
 }
 
-using ExtendFn = void (*)(FrameArgs, WfArrays, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
-using ShadowFn = void (*)(FrameArgs, WfArrays);
-
-// [variant][any_hit][stats]
-template <int V> struct TravKernels {
-    static constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
-    static ExtendFn extend(bool ah, bool st) {
-        return ah ? (st ? k_wf_extend<true, true, D, W> : k_wf_extend<true, false, D, W>)
-                  : (st ? k_wf_extend<false, true, D, W> : k_wf_extend<false, false, D, W>);
+template <class Frames> struct Kernels {
+    using ExtendFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
+    using ShadowFn = void (*)(Frames, WfArrays);
+    template <int V> static ExtendFn extend_v(bool ah, bool st) {
+        constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
+        return ah ? (st ? k_wf_extend<Frames, true, true, D, W> : k_wf_extend<Frames, true, false, D, W>)
+                  : (st ? k_wf_extend<Frames, false, true, D, W> : k_wf_extend<Frames, false, false, D, W>);
     }
-    static ShadowFn shadow(bool ah, bool st) {
-        return ah ? (st ? k_wf_shadow<true, true, D, W> : k_wf_shadow<true, false, D, W>)
-                  : (st ? k_wf_shadow<false, true, D, W> : k_wf_shadow<false, false, D, W>);
+    template <int V> static ShadowFn shadow_v(bool ah, bool st) {
+        constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
+        return ah ? (st ? k_wf_shadow<Frames, true, true, D, W> : k_wf_shadow<Frames, true, false, D, W>)
+                  : (st ? k_wf_shadow<Frames, false, true, D, W> : k_wf_shadow<Frames, false, false, D, W>);
+    }
+    static ExtendFn extend(int v, bool ah, bool st) {
+        switch (v) {
+            case 1: return extend_v<1>(ah, st);
+            case 2: return extend_v<2>(ah, st);
+            case 3: return extend_v<3>(ah, st);
+            default: return extend_v<0>(ah, st);
+        }
+    }
+    static ShadowFn shadow(int v, bool ah, bool st) {
+        switch (v) {
+            case 1: return shadow_v<1>(ah, st);
+            case 2: return shadow_v<2>(ah, st);
+            case 3: return shadow_v<3>(ah, st);
+            default: return shadow_v<0>(ah, st);
+        }
     }
 };
-
-ExtendFn extend_kernel(int v, bool ah, bool st) {
-    switch (v) {
-        case 1: return TravKernels<1>::extend(ah, st);
-        case 2: return TravKernels<2>::extend(ah, st);
-        case 3: return TravKernels<3>::extend(ah, st);
-        default: return TravKernels<0>::extend(ah, st);
-    }
-}
-ShadowFn shadow_kernel(int v, bool ah, bool st) {
-    switch (v) {
-        case 1: return TravKernels<1>::shadow(ah, st);
-        case 2: return TravKernels<2>::shadow(ah, st);
-        case 3: return TravKernels<3>::shadow(ah, st);
-        default: return TravKernels<0>::shadow(ah, st);
-    }
-}
-static_assert(kNumTravVariants == 4, "extend_kernel / shadow_kernel list every variant");
+static_assert(kNumTravVariants == 4, "Kernels::extend / shadow list every variant");
 
 }  // namespace
 
@@ -361,6 +391,8 @@ struct WavefrontState {
     uint32_t cap = 0;
     char* mem = nullptr;
     WfArrays a{};
+    char* batch_mem = nullptr;  // frame arguments + slot offsets of a batch
+    size_t batch_cap = 0;
     int cus = 0;
     std::unordered_map<const void*, uint32_t> grids;  // persistent grid per kernel (occupancy x CUs)
     // grid of a persistent launch of kernel `f`: as many blocks as can be resident
@@ -380,32 +412,35 @@ WavefrontState* wavefront_create() { return new WavefrontState(); }
 void wavefront_destroy(WavefrontState* s) {
     if (!s) return;
     if (s->mem) (void)hipFree(s->mem);
+    if (s->batch_mem) (void)hipFree(s->batch_mem);
     delete s;
 }
 
-int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, int variant,
-                     StageTimer* tm, std::string* err) {
-    if (variant < 0 || variant >= kNumTravVariants) {
-        *err = "unknown traversal variant";
-        return kWavefrontUnsupported;
+namespace {
+
+// Grows the path state to `nslots` slots (one per pixel).
+int ensure_slots(WavefrontState* ws, uint32_t nslots, std::string* err) {
+    if (nslots <= ws->cap) return 0;
+    if (ws->mem) (void)hipFree(ws->mem);
+    ws->mem = nullptr;
+    ws->cap = 0;
+    WfArrays probe{};
+    size_t bytes = layout(probe, nullptr, nslots) + 4096;
+    hipError_t e = hipMalloc(&ws->mem, bytes);
+    if (e != hipSuccess) {
+        *err = "cannot allocate the path state";
+        return int(e);
     }
-    if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats, tm);
-    const uint32_t tiles = ((fa.width + 7) / 8) * ((fa.height + 7) / 8);
-    const uint32_t nslots = tiles * 64u;
-    if (nslots > ws->cap) {
-        if (ws->mem) (void)hipFree(ws->mem);
-        ws->mem = nullptr;
-        ws->cap = 0;
-        WfArrays probe{};
-        size_t bytes = layout(probe, nullptr, nslots) + 4096;
-        hipError_t e = hipMalloc(&ws->mem, bytes);
-        if (e != hipSuccess) {
-            *err = "cannot allocate the path state";
-            return int(e);
-        }
-        ws->cap = nslots;
-        layout(ws->a, ws->mem, nslots);
-    }
+    ws->cap = nslots;
+    layout(ws->a, ws->mem, nslots);
+    return 0;
+}
+
+// The queue loop over `nslots` slots whose frames `fr` gives; every path ends within
+// max_samples * max_bounces bounces.
+template <class Frames>
+int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_t max_steps, hipStream_t st,
+                  bool any_hit, bool stats, int variant, StageTimer* tm, std::string* err) {
     if (ws->cus == 0) {
         int dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
@@ -413,14 +448,14 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
         ws->cus = std::max(cus, 1);
     }
     const WfArrays& a = ws->a;
-    const ExtendFn k_extend = extend_kernel(variant, any_hit, stats);
-    const ShadowFn k_shadow = shadow_kernel(variant, any_hit, stats);
-    const void* k_shade = stats ? (const void*)k_wf_shade<true> : (const void*)k_wf_shade<false>;
+    using K = Kernels<Frames>;
+    const typename K::ExtendFn k_extend = K::extend(variant, any_hit, stats);
+    const typename K::ShadowFn k_shadow = K::shadow(variant, any_hit, stats);
+    const void* k_shade = stats ? (const void*)k_wf_shade<Frames, true> : (const void*)k_wf_shade<Frames, false>;
     const uint32_t g_extend = ws->grid_for((const void*)k_extend);
     const uint32_t g_shadow = ws->grid_for((const void*)k_shadow);
     const uint32_t g_shade = ws->grid_for(k_shade);
-    const uint32_t g_resolve = ws->grid_for((const void*)k_wf_resolve);
-    const uint32_t grid = g_resolve;
+    const uint32_t g_resolve = ws->grid_for((const void*)k_wf_resolve<Frames>);
     hipError_t e = hipSuccess;
     // stage marks: 0 extend, 1 shade, 2 shadow, 3 resolve (init counts as resolve)
     auto mark = [&](int stage) {
@@ -428,29 +463,28 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
     };
     mark(-1);
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, a.counts);
-    hipLaunchKernelGGL(k_wf_init, dim3(std::min(grid, (nslots + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, st, fa,
-                       a, nslots);
+    hipLaunchKernelGGL(k_wf_init<Frames>, dim3(std::min(g_resolve, (nslots + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock),
+                       0, st, fr, a, nslots);
     mark(3);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
         *err = "launch k_wf_init";
         return int(e);
     }
-    // every path ends within samples x bounces bounces
-    const uint64_t max_iter = uint64_t(fa.samples) * fa.bounces + kCheckEvery;
+    const uint64_t max_iter = max_steps + kCheckEvery;
     for (uint64_t k = 0;; k++) {
         uint32_t* q_in = (k & 1) ? a.q1 : a.q0;
         uint32_t* q_out = (k & 1) ? a.q0 : a.q1;
         uint32_t* n_in = a.counts + (k & 1);
         uint32_t* n_out = a.counts + ((k + 1) & 1);
-        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, st, fa, a, q_in, n_in, n_out, a.counts + 2);
+        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, st, fr, a, q_in, n_in, n_out, a.counts + 2);
         mark(0);
-        if (stats) hipLaunchKernelGGL((k_wf_shade<true>), dim3(g_shade), dim3(kWfBlock), 0, st, fa, a, q_in, n_in);
-        else hipLaunchKernelGGL((k_wf_shade<false>), dim3(g_shade), dim3(kWfBlock), 0, st, fa, a, q_in, n_in);
+        if (stats) hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, st, fr, a, q_in, n_in);
+        else hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, st, fr, a, q_in, n_in);
         mark(1);
-        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, st, fa, a);
+        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, st, fr, a);
         mark(2);
-        hipLaunchKernelGGL(k_wf_resolve, dim3(g_resolve), dim3(kWfBlock), 0, st, fa, a, q_in, n_in, q_out, n_out);
+        hipLaunchKernelGGL(k_wf_resolve<Frames>, dim3(g_resolve), dim3(kWfBlock), 0, st, fr, a, q_in, n_in, q_out, n_out);
         mark(3);
         if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) {
@@ -473,6 +507,77 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
         }
     }
     return 0;
+}
+
+uint32_t frame_slots(const FrameArgs& fa) { return ((fa.width + 7) / 8) * ((fa.height + 7) / 8) * 64u; }
+
+}  // namespace
+
+int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, int variant,
+                     StageTimer* tm, std::string* err) {
+    if (variant < 0 || variant >= kNumTravVariants) {
+        *err = "unknown traversal variant";
+        return kWavefrontUnsupported;
+    }
+    if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats, tm);
+    const uint32_t nslots = frame_slots(fa);
+    int rc = ensure_slots(ws, nslots, err);
+    if (rc) return rc;
+    return run_wavefront(ws, OneFrame{fa}, nslots, uint64_t(fa.samples) * fa.bounces, st, any_hit, stats, variant, tm,
+                         err);
+}
+
+int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t count, hipStream_t st, bool any_hit,
+                           bool stats, int variant, StageTimer* tm, std::string* err) {
+    if (variant < 0 || variant >= kNumTravVariants) {
+        *err = "unknown traversal variant";
+        return kWavefrontUnsupported;
+    }
+    std::vector<uint32_t> base(count + 1, 0u);
+    uint64_t total = 0, steps = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        base[i] = uint32_t(total);
+        if (frames[i].samples && frames[i].bounces) total += frame_slots(frames[i]);
+        steps = std::max(steps, uint64_t(frames[i].samples) * frames[i].bounces);
+        if (total >= (1ull << 31)) {
+            *err = "the batch exceeds 2^31 path slots";
+            return kWavefrontUnsupported;
+        }
+    }
+    base[count] = uint32_t(total);
+    if (total == 0) return 0;
+    int rc = ensure_slots(ws, uint32_t(total), err);
+    if (rc) return rc;
+    // frame arguments and slot offsets in device memory
+    const size_t fbytes = sizeof(FrameArgs) * count, bbytes = sizeof(uint32_t) * (count + 1);
+    if (fbytes + bbytes > ws->batch_cap) {
+        if (ws->batch_mem) (void)hipFree(ws->batch_mem);
+        ws->batch_mem = nullptr;
+        ws->batch_cap = 0;
+        hipError_t e = hipMalloc(&ws->batch_mem, fbytes + bbytes + 256);
+        if (e != hipSuccess) {
+            *err = "cannot allocate the batch table";
+            return int(e);
+        }
+        ws->batch_cap = fbytes + bbytes;
+    }
+    FrameArgs* d_frames = reinterpret_cast<FrameArgs*>(ws->batch_mem);
+    uint32_t* d_base = reinterpret_cast<uint32_t*>(ws->batch_mem + ((fbytes + 255) & ~size_t(255)));
+    // frames with no samples or bounces own no slots: the image reset still applies
+    hipError_t e = hipMemcpyAsync(d_frames, frames, fbytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_base, base.data(), bbytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        *err = "copy of the batch table";
+        return int(e);
+    }
+    ManyFrames fr{d_frames, d_base, count, frames[0].counters};
+    rc = run_wavefront(ws, fr, uint32_t(total), steps, st, any_hit, stats, variant, tm, err);
+    // the host arrays the copies read must outlive them
+    if (rc == 0) {
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = int(e);
+    }
+    return rc;
 }
 
 }  // namespace rt

@@ -37,7 +37,8 @@ RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT = 0x4, 0x8
 EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", "rt_build_blas_batch",
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
-           "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint"]
+           "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint",
+           "rt_trace_rays_batch"]
 
 
 class RtError(RuntimeError):
@@ -64,6 +65,11 @@ class Bindings(C.Structure):
                 ("num_dlights", C.c_uint32), ("plights", C.c_void_p), ("num_plights", C.c_uint32),
                 ("textures", C.c_void_p), ("num_textures", C.c_uint32), ("num_vertices", C.c_uint32),
                 ("num_indices", C.c_uint32), ("num_prim_infos", C.c_uint32), ("num_materials", C.c_uint32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("tlas", C.c_uint64), ("bindings", C.c_void_p), ("accum", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
 
 
 class TraceStats(C.Structure):
@@ -115,6 +121,7 @@ def load_library(path=LIB_PATH):
         "rt_set_option": (i, [vp, u, u64]), "rt_get_stage_times": (i, [vp, vp]),
         "rt_tlas_flat_info": (i, [vp, u64, vp]),
         "rt_trace_rays_adjoint": (i, [vp, u64, vp, vp, u, u, vp, vp]),
+        "rt_trace_rays_batch": (i, [vp, vp, u, u, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -211,11 +218,16 @@ class Buffers:
         self.textures[k] = tensor
         self._tex[k] = Texture(tensor.data_ptr(), w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter, t.min_filter)
 
-    def bindings(self, ubo):
-        """rt_bindings for one trace; `ubo` (UBO record) must stay alive during the call."""
+    def bindings(self, ubo, keep=None):
+        """rt_bindings for one trace.  The UBO copy it points at lives in this object
+        (replaced by the next call) or, when `keep` is a list, is appended to it."""
         s = self.scene
-        self._ubo = np.ascontiguousarray(ubo)
-        return Bindings(self._ubo.ctypes.data, self.vertices.data_ptr(), self.indices.data_ptr(),
+        u = np.array(ubo, copy=True)
+        if keep is None:
+            self._ubo = u
+        else:
+            keep.append(u)
+        return Bindings(u.ctypes.data, self.vertices.data_ptr(), self.indices.data_ptr(),
                         self.prim_infos.data_ptr(), self.materials.data_ptr(),
                         self.dlights.data_ptr() if self.dlights is not None else None, len(s.dlights),
                         self.plights.data_ptr() if self.plights is not None else None, len(s.plights),
@@ -318,6 +330,21 @@ def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERN
     st = TraceStats() if stats else None
     ctx.check(ctx.lib.rt_trace_rays(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(accum.data_ptr()), None,
                                     width, height, mode, C.byref(st) if st is not None else None))
+    if st is None:
+        return None
+    return {n: getattr(st, n) for n, _ in TraceStats._fields_}
+
+
+def trace_rays_batch(ctx, items, mode=MODE_WAVEFRONT, stats=False):
+    """rt_trace_rays_batch over items = [(tlas, buffers, ubo, accum, width, height)]."""
+    keep, binds = [], []
+    arr = (BatchItem * max(1, len(items)))()
+    for k, (tlas, buffers, ubo, accum, w, h) in enumerate(items):
+        b = buffers.bindings(ubo, keep=keep)
+        binds.append(b)
+        arr[k] = BatchItem(tlas.handle, C.addressof(b), accum.data_ptr(), w, h)
+    st = TraceStats() if stats else None
+    ctx.check(ctx.lib.rt_trace_rays_batch(ctx.handle, arr, len(items), mode, C.byref(st) if st is not None else None))
     if st is None:
         return None
     return {n: getattr(st, n) for n, _ in TraceStats._fields_}

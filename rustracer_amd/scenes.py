@@ -440,6 +440,46 @@ def cornell_params(width=512, height=512, spp=16, bounces=4, tex_size=512, seed=
     return sc
 
 
+def batch_scene(seed, spp=16, bounces=6, tri_range=(1_000, 500_000), resolutions=(128, 256, 512, 1024)):
+    """One scene of the many-scene batch: a Cornell-style room (floor, ceiling, back
+    wall white; left red; right green) around a displaced geodesic sphere whose
+    triangle count is log-uniform in tri_range, with a seeded metallic-roughness
+    material, one point light; a square image whose side is drawn from
+    `resolutions`; 16 spp, depth 6."""
+    rng = np.random.default_rng(10_000 + seed)
+    side = int(rng.choice(resolutions))
+    n_tris = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+    freq = max(2, int(round(np.sqrt(n_tris / 20.0))))
+    room = cornell_sphere(width=side, height=side, spp=spp, bounces=bounces, seed=seed)
+    pos, tri = geodesic_sphere(freq)
+    disp = value_noise_3d(pos, seed, octaves=3, base_freq=2.0 + rng.uniform(0, 3))
+    pos = pos * (1.0 + rng.uniform(0.03, 0.15) * disp)[:, None]
+    nrm = vertex_normals(pos, tri)
+    b = Builder()
+    b.materials = list(room.materials)
+    b.vertices = [room.vertices]
+    b.indices = [room.indices]
+    b.nv, b.ni = len(room.vertices), len(room.indices)
+    b.prims = [(int(p["v_offset"]), int(p["i_offset"]), int(p["material_id"]), int(g["prim_count"]))
+               for p, g in zip(room.prim_infos, room.geometries)]
+    m = L.default_material()
+    m["base_color"] = (*rng.uniform(0.3, 0.95, 3), 1.0)
+    m["metallic_factor"] = float(rng.uniform(0, 1) < 0.5)
+    m["roughness_factor"] = rng.uniform(0.2, 0.9)
+    g_obj = b.add_primitive(make_vertices(pos, nrm), tri.reshape(-1).astype(np.uint32), b.add_material(m))
+    # the room's three wall geometries at the identity, the object scaled into the room
+    xf = np.eye(4)
+    xf[:3, :3] *= rng.uniform(0.35, 0.6)
+    xf[:3, 3] = (rng.uniform(-0.3, 0.3), 0.65, rng.uniform(-0.3, 0.3))
+    inst = make_instances([(None, 0), (None, 1), (None, 2), (xf, g_obj)])
+    return b.finish(f"batch_{seed}", inst, room.plights, room.dlights, room.ubo, side, side)
+
+
+def batch_scenes(count=64, spp=16, bounces=6):
+    """The many-scene batch: scenes from seeds 0 .. count-1."""
+    return [batch_scene(i, spp=spp, bounces=bounces) for i in range(count)]
+
+
 def _interp_matrix(size, cells):
     """(size, cells) matrix that resamples a periodic lattice of `cells` values to
     `size` texel centres with smoothstep weights (rows sum to 1)."""

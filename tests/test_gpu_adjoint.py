@@ -104,7 +104,7 @@ def test_autograd_function_and_an_optimisation_step(gpu_ctx):
     gpu_ctx.sync()
     for k, p in zip([1, 2], dscene.params):
         assert torch.allclose(p.grad, ref[k], rtol=1e-5, atol=1e-6 * float(ref[k].abs().max()))
-    losses = [float(loss)]
+    losses = [float(loss.detach())]
     opt = torch.optim.Adam(dscene.params, lr=0.02)
     for _ in range(6):
         opt.zero_grad()
@@ -115,6 +115,6 @@ def test_autograd_function_and_an_optimisation_step(gpu_ctx):
         with torch.no_grad():
             for p in dscene.params:
                 p.clamp_(0.02, 1.0)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     print("losses", losses)
     assert losses[-1] < 0.9 * losses[0]
