@@ -160,14 +160,21 @@ def batch_benchmark(ctx, steps):
     ctx.sync()
     st = rt.trace_rays_batch(ctx, items, stats=True)
     rays = st["rays"]
-    rt.trace_rays_batch(ctx, items)
-    t_batch = []
-    for _ in range(steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
+    t_graph = {}
+    for graph in (0, 1):
+        ctx.set_option(rt.OPTION_GRAPH, graph)
         rt.trace_rays_batch(ctx, items)
-        torch.cuda.synchronize()
-        t_batch.append(time.perf_counter() - t0)
+        ts = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rt.trace_rays_batch(ctx, items)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        t_graph[graph] = float(np.median(ts))
+    ctx.set_option(rt.OPTION_GRAPH, 0)
+    use_graph = int(t_graph[1] < t_graph[0])
+    t_batch = [t_graph[use_graph]]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for tlas, buf, ubo, acc, w, h in items:
@@ -179,6 +186,7 @@ def batch_benchmark(ctx, steps):
     return {"scenes": len(scs), "triangles": int(sum(s.num_triangles for s in scs)),
             "pixels": int(sum(s.width * s.height for s in scs)), "rays": int(rays),
             "batch_ms": round(tb * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
+            "batch_ms_eager": round(t_graph[0] * 1e3, 2), "batch_ms_graph": round(t_graph[1] * 1e3, 2),
             "batch_mrays_per_s": round(rays / tb / 1e6, 1), "sequential_mrays_per_s": round(rays / t_seq / 1e6, 1),
             "scene_gen_s": round(gen_s, 1)}
 
@@ -194,6 +202,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path-tracing frame shards by image tiles, but this build runs "
@@ -221,6 +230,13 @@ def main():
     W, H = sc.width, sc.height
     acc = rt.new_accumulation(W, H)
 
+    tuned = None
+    if not args.no_autotune:
+        # rt_autotune: traversal variant and graph replay for this frame (setup, not timed)
+        v, g, ms = ctx.autotune(tlas, buf, sc.ubo, W, H)
+        tuned = {"traversal_variant": v, "graph": g, "probe_ms_2spp": round(ms, 2)}
+    # the timed frames launch eagerly: per-stage HIP events need the individual launches
+    ctx.set_option(rt.OPTION_GRAPH, 0)
     modes = {"megakernel": rt.MODE_MEGAKERNEL, "wavefront": rt.MODE_WAVEFRONT}
     wanted = list(modes) if args.mode == "best" else [args.mode]
     results = {}
@@ -290,7 +306,7 @@ def main():
                                f"{int(sc.ubo['number_of_bounces'])}, 3 point lights",
                    "mode": best, "rays_per_frame": st["rays"],
                    "modes": {k: round(v["mrays"], 2) for k, v in results.items()},
-                   "build_ms": round(min(build_ms), 2), "scene_gen_s": round(gen_s, 2)},
+                   "build_ms": round(min(build_ms), 2), "scene_gen_s": round(gen_s, 2), "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else int(traffic), "kernel": kernel_name,

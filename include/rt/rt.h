@@ -218,7 +218,12 @@ enum {
      * registers for 5 waves/SIMD; 1: 16 entries, 6 waves; 2: 16 entries, 8 waves;
      * 3: 64 entries (LDS holds 2 waves/SIMD).  Results are identical; speed differs by scene (a
      * ring too shallow for the BVH depth restarts traversals). */
-    RT_OPTION_TRAVERSAL_VARIANT = 3
+    RT_OPTION_TRAVERSAL_VARIANT = 3,
+    /* 1: the wavefront trace captures one chunk of its queue iterations (8 x
+     * extend, shade, shadow, resolve) as a HIP graph and replays it until no path
+     * is alive (one launch per chunk).  Same images.  Ignored while
+     * RT_OPTION_STAGE_TIMING is on.  Default 0. */
+    RT_OPTION_GRAPH = 4
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 
@@ -226,6 +231,15 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * (the analogue of the reference's GPU timestamp query pool,
  * crates/libs/vulkan/src/query.rs:8-73).  The megakernel reports all of its time
  * as extend_ms.  Synchronises the context stream. */
+/* Autotuner: traces the frame (the bindings' UBO with at most 2 samples per pixel)
+ * in wavefront mode with every traversal variant (stack depth, register target)
+ * and with and without graph replay, and keeps the fastest as this context's
+ * RT_OPTION_TRAVERSAL_VARIANT / RT_OPTION_GRAPH.  A variant whose stack ring
+ * gives up on a ray of this frame (RT_ERR_STACK_OVERFLOW) is not a candidate.  Writes the choice and its frame
+ * time to the non-NULL outputs.  Synchronises the stream. */
+int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height,
+                uint32_t* out_variant, uint32_t* out_graph, double* out_ms);
+
 typedef struct rt_stage_times {
     double total_ms;
     double extend_ms;   /* closest-hit traversal */

@@ -104,6 +104,7 @@ struct rt_ctx {
     size_t batch_tex_cap = 0;
     uint32_t grad_cap = 0;
     int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
+    bool use_graph = false;  // RT_OPTION_GRAPH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
 };
@@ -709,6 +710,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
             if (!ctx->wavefront) return fail(ctx, RT_ERR_OUT_OF_MEMORY, "wavefront state");
         }
         std::string err;
+        wavefront_set_graph(ctx->wavefront, ctx->use_graph);
         rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, ctx->variant, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
@@ -786,6 +788,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
                 rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, stats != nullptr);
         if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
         std::string err;
+        wavefront_set_graph(ctx->wavefront, ctx->use_graph);
         rc = launch_wavefront_batch(ctx->wavefront, frames.data(), count, st, any_hit, stats != nullptr, ctx->variant, tm,
                                     &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
@@ -850,6 +853,82 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     return RT_OK;
 }
 
+int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height,
+                uint32_t* out_variant, uint32_t* out_graph, double* out_ms) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    int rc = validate_bindings(ctx, b);
+    if (rc) return rc;
+    if (width == 0 || height == 0 || uint64_t(width) * height > (1ull << 31))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
+    DeviceGuard g(ctx->device);
+    rt_ubo u = *b->ubo;
+    u.number_of_samples = std::min(u.number_of_samples, 2u);
+    u.total_number_of_samples = u.number_of_samples;
+    rt_bindings bb = *b;
+    bb.ubo = &u;
+    hipError_t e;
+    DevPtr scratch(dev_alloc<float4>(size_t(width) * height, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    // start from a clear error flag (a set flag would be blamed on a variant)
+    {
+        uint32_t flags = 0;
+        RT_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        RT_HIP(ctx, hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost), "read error flags");
+        if (flags) return fail(ctx, RT_ERR_STACK_OVERFLOW, "an earlier traversal exceeded its stack restart budget");
+    }
+    const bool timing = ctx->stage_timing;
+    ctx->stage_timing = false;
+    const int keep_variant = ctx->variant;
+    const bool keep_graph = ctx->use_graph;
+    double best = -1.0;
+    int best_v = 0;
+    bool best_g = false;
+    for (int v = 0; v < wavefront_num_variants() && rc == RT_OK; v++) {
+        for (int gr = 0; gr < 2 && rc == RT_OK; gr++) {
+            ctx->variant = v;
+            ctx->use_graph = gr != 0;
+            rc = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
+            if (rc) break;
+            e = hipEventRecord(ctx->ev0, ctx->stream);
+            if (e == hipSuccess)
+                rc = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
+            if (rc) break;
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+            // a variant whose stack ring is too shallow for this BVH gives up on
+            // rays (restart budget): it is not a candidate
+            uint32_t flags = 0;
+            if (e == hipSuccess) e = hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost);
+            if (e == hipSuccess && flags) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
+            if (e != hipSuccess) {
+                rc = hip_fail(ctx, e, "autotune timing");
+                break;
+            }
+            if (flags) continue;
+            if (best < 0.0 || ms < best) {
+                best = ms;
+                best_v = v;
+                best_g = gr != 0;
+            }
+        }
+    }
+    ctx->stage_timing = timing;
+    if (rc == RT_OK && best < 0.0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
+    if (rc) {
+        ctx->variant = keep_variant;
+        ctx->use_graph = keep_graph;
+        return rc;
+    }
+    ctx->variant = best_v;
+    ctx->use_graph = best_g;
+    if (out_variant) *out_variant = uint32_t(best_v);
+    if (out_graph) *out_graph = best_g ? 1u : 0u;
+    if (out_ms) *out_ms = best;
+    return RT_OK;
+}
+
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (option == RT_OPTION_STAGE_TIMING) {
@@ -862,6 +941,11 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
         if (value >= uint64_t(wavefront_num_variants()))
             return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_TRAVERSAL_VARIANT out of range");
         ctx->variant = int(value);
+        return RT_OK;
+    }
+    if (option == RT_OPTION_GRAPH) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_GRAPH takes 0 or 1");
+        ctx->use_graph = value != 0;
         return RT_OK;
     }
     if (option == RT_OPTION_FLATTEN_IDENTITY_INSTANCES) {

@@ -20,6 +20,8 @@ constexpr int kWavefrontUnsupported = -1000;
 int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, int variant,
                      StageTimer* tm, std::string* err);
 int wavefront_num_variants();
+// Replay each chunk of queue iterations as one captured HIP graph.
+void wavefront_set_graph(WavefrontState* s, bool on);
 // Renders `count` frames (host array) in one set of queue kernels: their slots are
 // laid end to end and every kernel serves all of them.  Each frame's image is
 // the one launch_wavefront gives it alone.  Frames with no samples or bounces

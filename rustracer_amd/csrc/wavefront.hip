@@ -393,6 +393,9 @@ struct WavefrontState {
     WfArrays a{};
     char* batch_mem = nullptr;  // frame arguments + slot offsets of a batch
     size_t batch_cap = 0;
+    bool use_graph = false;     // replay captured chunks (RT_OPTION_GRAPH)
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
     int cus = 0;
     std::unordered_map<const void*, uint32_t> grids;  // persistent grid per kernel (occupancy x CUs)
     // grid of a persistent launch of kernel `f`: as many blocks as can be resident
@@ -409,10 +412,14 @@ struct WavefrontState {
 
 WavefrontState* wavefront_create() { return new WavefrontState(); }
 
+void wavefront_set_graph(WavefrontState* s, bool on) { s->use_graph = on; }
+
 void wavefront_destroy(WavefrontState* s) {
     if (!s) return;
     if (s->mem) (void)hipFree(s->mem);
     if (s->batch_mem) (void)hipFree(s->batch_mem);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
     delete s;
 }
 
@@ -472,20 +479,82 @@ int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_
         return int(e);
     }
     const uint64_t max_iter = max_steps + kCheckEvery;
-    for (uint64_t k = 0;; k++) {
+    // one iteration: extend, shade, shadow, resolve, with the path queues of parity k
+    auto iteration = [&](uint64_t k, hipStream_t s, bool marks) {
         uint32_t* q_in = (k & 1) ? a.q1 : a.q0;
         uint32_t* q_out = (k & 1) ? a.q0 : a.q1;
         uint32_t* n_in = a.counts + (k & 1);
         uint32_t* n_out = a.counts + ((k + 1) & 1);
-        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, st, fr, a, q_in, n_in, n_out, a.counts + 2);
-        mark(0);
-        if (stats) hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, st, fr, a, q_in, n_in);
-        else hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, st, fr, a, q_in, n_in);
-        mark(1);
-        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, st, fr, a);
-        mark(2);
-        hipLaunchKernelGGL(k_wf_resolve<Frames>, dim3(g_resolve), dim3(kWfBlock), 0, st, fr, a, q_in, n_in, q_out, n_out);
-        mark(3);
+        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, s, fr, a, q_in, n_in, n_out, a.counts + 2);
+        if (marks) mark(0);
+        if (stats) hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, s, fr, a, q_in, n_in);
+        else hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, s, fr, a, q_in, n_in);
+        if (marks) mark(1);
+        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, s, fr, a);
+        if (marks) mark(2);
+        hipLaunchKernelGGL(k_wf_resolve<Frames>, dim3(g_resolve), dim3(kWfBlock), 0, s, fr, a, q_in, n_in, q_out, n_out);
+        if (marks) mark(3);
+    };
+    // the live-path count after an even number of iterations sits in counts[0]
+    auto live_paths = [&](uint32_t& live) -> hipError_t {
+        hipError_t r = hipMemcpyAsync(&live, a.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (r == hipSuccess) r = hipStreamSynchronize(st);
+        return r;
+    };
+    static_assert(kCheckEvery % 2 == 0, "a chunk ends on queue 0");
+    if (ws->use_graph && !tm) {
+        // the kCheckEvery iterations of a chunk captured once as a HIP graph and
+        // replayed until no path is alive: one launch per chunk instead of 32
+        if (!ws->cap_stream) {
+            e = hipStreamCreateWithFlags(&ws->cap_stream, hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                *err = "capture stream";
+                return int(e);
+            }
+        }
+        e = hipStreamBeginCapture(ws->cap_stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            *err = "hipStreamBeginCapture";
+            return int(e);
+        }
+        for (uint64_t k = 0; k < kCheckEvery; k++) iteration(k, ws->cap_stream, false);
+        hipGraph_t graph = nullptr;
+        hipError_t launch_err = hipGetLastError();
+        e = hipStreamEndCapture(ws->cap_stream, &graph);
+        if (e == hipSuccess) e = launch_err;
+        if (e == hipSuccess && ws->exec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res;
+            if (hipGraphExecUpdate(ws->exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(ws->exec);
+                ws->exec = nullptr;
+            }
+        }
+        if (e == hipSuccess && !ws->exec) e = hipGraphInstantiate(&ws->exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            *err = "capture of the wavefront chunk";
+            return int(e);
+        }
+        for (uint64_t k = 0;; k += kCheckEvery) {
+            e = hipGraphLaunch(ws->exec, st);
+            uint32_t live = 0;
+            if (e == hipSuccess) e = live_paths(live);
+            if (e != hipSuccess) {
+                *err = "replay of the wavefront chunk";
+                return int(e);
+            }
+            if (live == 0) break;
+            if (k > max_iter) {
+                *err = "paths did not terminate";
+                return int(hipErrorUnknown);
+            }
+        }
+        return 0;
+    }
+    for (uint64_t k = 0;; k++) {
+        iteration(k, st, true);
         if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) {
             *err = "launch of a wavefront kernel";
@@ -493,8 +562,7 @@ int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_
         }
         if ((k + 1) % kCheckEvery == 0) {
             uint32_t live = 0;
-            e = hipMemcpyAsync(&live, n_out, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            e = live_paths(live);
             if (e != hipSuccess) {
                 *err = "reading the queue length";
                 return int(e);

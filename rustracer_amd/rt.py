@@ -38,7 +38,7 @@ EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", 
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
            "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint",
-           "rt_trace_rays_batch"]
+           "rt_trace_rays_batch", "rt_autotune"]
 
 
 class RtError(RuntimeError):
@@ -83,6 +83,7 @@ class TraceStats(C.Structure):
 OPTION_STAGE_TIMING = 1
 OPTION_FLATTEN_IDENTITY_INSTANCES = 2
 OPTION_TRAVERSAL_VARIANT = 3
+OPTION_GRAPH = 4
 NUM_TRAVERSAL_VARIANTS = 4
 
 
@@ -122,6 +123,7 @@ def load_library(path=LIB_PATH):
         "rt_tlas_flat_info": (i, [vp, u64, vp]),
         "rt_trace_rays_adjoint": (i, [vp, u64, vp, vp, u, u, vp, vp]),
         "rt_trace_rays_batch": (i, [vp, vp, u, u, vp]),
+        "rt_autotune": (i, [vp, u64, vp, u, u, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -167,6 +169,15 @@ class Context:
 
     def sync(self):
         self.check(self.lib.rt_sync(self.handle))
+
+    def autotune(self, tlas, buffers, ubo, width, height):
+        """rt_autotune: picks the fastest traversal variant and graph replay for this
+        frame and keeps them on the context.  Returns (variant, graph, ms)."""
+        b = buffers.bindings(ubo)
+        v, g, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+        self.check(self.lib.rt_autotune(self.handle, tlas.handle, C.byref(b), width, height, C.byref(v), C.byref(g),
+                                        C.byref(ms)))
+        return v.value, g.value, ms.value
 
     def set_option(self, option, value):
         """rt_set_option (OPTION_STAGE_TIMING: event timing of every pipeline stage)."""

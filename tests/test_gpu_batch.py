@@ -22,8 +22,8 @@ def _scenes():
     return out
 
 
-@pytest.mark.parametrize("mode", [rt.MODE_WAVEFRONT, rt.MODE_MEGAKERNEL])
-def test_batch_frames_equal_single_frames(gpu_ctx, mode):
+@pytest.mark.parametrize("mode,graph", [(rt.MODE_WAVEFRONT, 0), (rt.MODE_WAVEFRONT, 1), (rt.MODE_MEGAKERNEL, 0)])
+def test_batch_frames_equal_single_frames(gpu_ctx, mode, graph):
     dss = [G.device_scene(gpu_ctx, f"batch_case_{k}", (lambda s=s: s)) for k, s in enumerate(_scenes())]
     singles, single_rays = [], 0
     for ds in dss:
@@ -35,7 +35,11 @@ def test_batch_frames_equal_single_frames(gpu_ctx, mode):
         single_rays += st["rays"]
     accs = [rt.new_accumulation(ds.scene.width, ds.scene.height).fill_(7.0) for ds in dss]
     items = [(ds.tlas, ds.buffers, ds.scene.ubo, a, ds.scene.width, ds.scene.height) for ds, a in zip(dss, accs)]
-    st = rt.trace_rays_batch(gpu_ctx, items, mode=mode, stats=True)
+    gpu_ctx.set_option(rt.OPTION_GRAPH, graph)
+    try:
+        st = rt.trace_rays_batch(gpu_ctx, items, mode=mode, stats=True)
+    finally:
+        gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
     for k, (a, ref) in enumerate(zip(accs, singles)):
         assert np.array_equal(a.cpu().numpy(), ref), k
     assert st["rays"] == single_rays

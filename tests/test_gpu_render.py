@@ -199,3 +199,61 @@ def test_traversal_variants_render_the_same_image(gpu_ctx, name):
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
     with pytest.raises(rt.RtError):
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, rt.NUM_TRAVERSAL_VARIANTS)
+
+
+def test_graph_replay_matches_eager(gpu_ctx):
+    """RT_OPTION_GRAPH: the captured chunk is replayed many times per frame, across
+    frames with different UBOs (exec update), with eager work in between; every
+    image equals the eager one."""
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    sc = ds.scene
+    eager = []
+    for fc in range(3):
+        ubo = sc.ubo.copy()
+        ubo["frame_count"] = fc
+        a, _ = _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT)
+        eager.append(a.cpu().numpy())
+    gpu_ctx.set_option(rt.OPTION_GRAPH, 1)
+    try:
+        for rep in range(2):
+            for fc in range(3):
+                ubo = sc.ubo.copy()
+                ubo["frame_count"] = fc
+                a, _ = _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT)
+                assert np.array_equal(a.cpu().numpy(), eager[fc]), (rep, fc)
+                # eager work between replays
+                _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_MEGAKERNEL)
+    finally:
+        gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
+
+
+def test_autotune_keeps_the_image(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    sc = ds.scene
+    ref, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
+    try:
+        v, g, ms = gpu_ctx.autotune(ds.tlas, ds.buffers, sc.ubo, sc.width, sc.height)
+        assert 0 <= v < rt.NUM_TRAVERSAL_VARIANTS and g in (0, 1) and ms > 0
+        a, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
+        assert np.array_equal(a.cpu().numpy(), ref.cpu().numpy())
+    finally:
+        gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
+        gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
+
+
+def test_autotune_on_a_deep_bvh_leaves_no_error(gpu_ctx):
+    """Shallow-stack variants can give up on rays of a deep BVH: the autotuner must
+    drop them and leave the context usable, with the image unchanged."""
+    ds = G.device_scene(gpu_ctx, "displaced_sphere")
+    sc = ds.scene
+    ubo = sc.ubo.copy()
+    ubo["number_of_samples"] = ubo["total_number_of_samples"] = 2
+    ref, _ = _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT)
+    try:
+        v, g, ms = gpu_ctx.autotune(ds.tlas, ds.buffers, ubo, sc.width, sc.height)
+        gpu_ctx.sync()
+        a, st = _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT, stats=True)
+        assert np.array_equal(a.cpu().numpy(), ref.cpu().numpy())
+    finally:
+        gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
+        gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
