@@ -233,8 +233,8 @@ def main():
     tuned = None
     if not args.no_autotune:
         # rt_autotune: traversal variant and graph replay for this frame (setup, not timed)
-        v, g, ms = ctx.autotune(tlas, buf, sc.ubo, W, H)
-        tuned = {"traversal_variant": v, "graph": g, "probe_ms_2spp": round(ms, 2)}
+        tuned = ctx.autotune(tlas, buf, sc.ubo, W, H)
+        tuned["frame_ms"] = round(tuned["frame_ms"], 2)
     # the timed frames launch eagerly: per-stage HIP events need the individual launches
     ctx.set_option(rt.OPTION_GRAPH, 0)
     modes = {"megakernel": rt.MODE_MEGAKERNEL, "wavefront": rt.MODE_WAVEFRONT}

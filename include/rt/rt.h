@@ -223,7 +223,13 @@ enum {
      * extend, shade, shadow, resolve) as a HIP graph and replays it until no path
      * is alive (one launch per chunk).  Same images.  Ignored while
      * RT_OPTION_STAGE_TIMING is on.  Default 0. */
-    RT_OPTION_GRAPH = 4
+    RT_OPTION_GRAPH = 4,
+    /* Wavefront traversal kernels fetch rays dynamically: a wave takes new rays
+     * from the queue once fewer than this many of its 64 lanes are still
+     * traversing (1 = only when all have finished).  Same images.  Defaults:
+     * extend 40, shadow 1. */
+    RT_OPTION_EXTEND_REFILL = 5,
+    RT_OPTION_SHADOW_REFILL = 6
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 
@@ -232,13 +238,19 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * crates/libs/vulkan/src/query.rs:8-73).  The megakernel reports all of its time
  * as extend_ms.  Synchronises the context stream. */
 /* Autotuner: traces the frame (the bindings' UBO with at most 2 samples per pixel)
- * in wavefront mode with every traversal variant (stack depth, register target)
- * and with and without graph replay, and keeps the fastest as this context's
- * RT_OPTION_TRAVERSAL_VARIANT / RT_OPTION_GRAPH.  A variant whose stack ring
- * gives up on a ray of this frame (RT_ERR_STACK_OVERFLOW) is not a candidate.  Writes the choice and its frame
- * time to the non-NULL outputs.  Synchronises the stream. */
-int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height,
-                uint32_t* out_variant, uint32_t* out_graph, double* out_ms);
+ * in wavefront mode and tunes, one after another, the traversal variant (stack
+ * depth, register target), the extend and shadow refill thresholds and graph
+ * replay, keeping the fastest settings on this context (the RT_OPTION_* values).
+ * A variant whose stack ring gives up on a ray of this frame (restart budget) is
+ * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */
+typedef struct rt_tuning {
+    uint32_t traversal_variant;
+    uint32_t graph;
+    uint32_t extend_refill;
+    uint32_t shadow_refill;
+    double frame_ms;  /* probe frame time with the chosen settings */
+} rt_tuning;
+int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height, rt_tuning* out);
 
 typedef struct rt_stage_times {
     double total_ms;

@@ -105,6 +105,7 @@ struct rt_ctx {
     uint32_t grad_cap = 0;
     int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
     bool use_graph = false;  // RT_OPTION_GRAPH
+    uint32_t refill_ext = 40, refill_sh = 1;  // RT_OPTION_EXTEND_REFILL / RT_OPTION_SHADOW_REFILL
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
 };
@@ -711,6 +712,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         }
         std::string err;
         wavefront_set_graph(ctx->wavefront, ctx->use_graph);
+        wavefront_set_refill(ctx->wavefront, ctx->refill_ext, ctx->refill_sh);
         rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, ctx->variant, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
@@ -789,6 +791,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
         if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
         std::string err;
         wavefront_set_graph(ctx->wavefront, ctx->use_graph);
+        wavefront_set_refill(ctx->wavefront, ctx->refill_ext, ctx->refill_sh);
         rc = launch_wavefront_batch(ctx->wavefront, frames.data(), count, st, any_hit, stats != nullptr, ctx->variant, tm,
                                     &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
@@ -853,8 +856,7 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     return RT_OK;
 }
 
-int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height,
-                uint32_t* out_variant, uint32_t* out_graph, double* out_ms) {
+int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height, rt_tuning* out) {
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
     int rc = validate_bindings(ctx, b);
     if (rc) return rc;
@@ -869,7 +871,7 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     hipError_t e;
     DevPtr scratch(dev_alloc<float4>(size_t(width) * height, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-    // start from a clear error flag (a set flag would be blamed on a variant)
+    // start from a clear error flag (a set flag would be blamed on a setting)
     {
         uint32_t flags = 0;
         RT_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -880,52 +882,79 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     ctx->stage_timing = false;
     const int keep_variant = ctx->variant;
     const bool keep_graph = ctx->use_graph;
+    const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
+    // one probe: frame time of the current settings, or < 0 when a traversal gave up
+    auto probe = [&](double& ms_out) -> int {
+        int r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
+        if (r) return r;
+        hipError_t er = hipEventRecord(ctx->ev0, ctx->stream);
+        if (er == hipSuccess)
+            r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
+        if (r) return r;
+        if (er == hipSuccess) er = hipEventRecord(ctx->ev1, ctx->stream);
+        if (er == hipSuccess) er = hipEventSynchronize(ctx->ev1);
+        float ms = 0.f;
+        if (er == hipSuccess) er = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        uint32_t flags = 0;
+        if (er == hipSuccess) er = hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost);
+        if (er == hipSuccess && flags) er = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
+        if (er != hipSuccess) return hip_fail(ctx, er, "autotune probe");
+        ms_out = flags ? -1.0 : double(ms);
+        return RT_OK;
+    };
     double best = -1.0;
-    int best_v = 0;
-    bool best_g = false;
+    auto consider = [&](double ms) { return ms >= 0.0 && (best < 0.0 || ms < best); };
+    // 1. traversal variant (graph off, current refills)
+    ctx->use_graph = false;
+    int best_v = -1;
     for (int v = 0; v < wavefront_num_variants() && rc == RT_OK; v++) {
-        for (int gr = 0; gr < 2 && rc == RT_OK; gr++) {
-            ctx->variant = v;
-            ctx->use_graph = gr != 0;
-            rc = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
+        ctx->variant = v;
+        double ms = -1.0;
+        rc = probe(ms);
+        if (rc == RT_OK && consider(ms)) { best = ms; best_v = v; }
+    }
+    if (rc == RT_OK && best_v < 0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
+    if (rc == RT_OK) ctx->variant = best_v;
+    // 2, 3. refill thresholds of the extend and shadow kernels
+    const uint32_t refills[] = {1u, 16u, 32u, 48u, 64u};
+    for (int which = 0; which < 2 && rc == RT_OK; which++) {
+        uint32_t& knob = which == 0 ? ctx->refill_ext : ctx->refill_sh;
+        uint32_t chosen = knob;
+        for (uint32_t r : refills) {
+            if (r == chosen) continue;
+            knob = r;
+            double ms = -1.0;
+            rc = probe(ms);
             if (rc) break;
-            e = hipEventRecord(ctx->ev0, ctx->stream);
-            if (e == hipSuccess)
-                rc = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
-            if (rc) break;
-            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
-            if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
-            float ms = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-            // a variant whose stack ring is too shallow for this BVH gives up on
-            // rays (restart budget): it is not a candidate
-            uint32_t flags = 0;
-            if (e == hipSuccess) e = hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost);
-            if (e == hipSuccess && flags) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
-            if (e != hipSuccess) {
-                rc = hip_fail(ctx, e, "autotune timing");
-                break;
-            }
-            if (flags) continue;
-            if (best < 0.0 || ms < best) {
-                best = ms;
-                best_v = v;
-                best_g = gr != 0;
-            }
+            if (consider(ms)) { best = ms; chosen = r; }
+        }
+        knob = chosen;
+    }
+    // 4. graph replay
+    if (rc == RT_OK) {
+        ctx->use_graph = true;
+        double ms = -1.0;
+        rc = probe(ms);
+        if (rc == RT_OK) {
+            if (consider(ms)) best = ms;
+            else ctx->use_graph = false;
         }
     }
     ctx->stage_timing = timing;
-    if (rc == RT_OK && best < 0.0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
     if (rc) {
         ctx->variant = keep_variant;
         ctx->use_graph = keep_graph;
+        ctx->refill_ext = keep_re;
+        ctx->refill_sh = keep_rs;
         return rc;
     }
-    ctx->variant = best_v;
-    ctx->use_graph = best_g;
-    if (out_variant) *out_variant = uint32_t(best_v);
-    if (out_graph) *out_graph = best_g ? 1u : 0u;
-    if (out_ms) *out_ms = best;
+    if (out) {
+        out->traversal_variant = uint32_t(ctx->variant);
+        out->graph = ctx->use_graph ? 1u : 0u;
+        out->extend_refill = ctx->refill_ext;
+        out->shadow_refill = ctx->refill_sh;
+        out->frame_ms = best;
+    }
     return RT_OK;
 }
 
@@ -941,6 +970,11 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
         if (value >= uint64_t(wavefront_num_variants()))
             return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_TRAVERSAL_VARIANT out of range");
         ctx->variant = int(value);
+        return RT_OK;
+    }
+    if (option == RT_OPTION_EXTEND_REFILL || option == RT_OPTION_SHADOW_REFILL) {
+        if (value < 1 || value > 64) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "refill thresholds are 1 .. 64");
+        (option == RT_OPTION_EXTEND_REFILL ? ctx->refill_ext : ctx->refill_sh) = uint32_t(value);
         return RT_OK;
     }
     if (option == RT_OPTION_GRAPH) {

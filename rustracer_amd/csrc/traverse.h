@@ -155,48 +155,43 @@ struct RayFromF64 {  // the caller holds the fp64 ray
 
 enum : int { kLevelTlas = 0, kLevelBlas = 1, kLevelFlat = 2 };
 
-// Closest-hit traversal.  `ah.ignore(inst_index, prim, u, v)` is the any-hit
-// shader (true = ignoreIntersectionEXT).  Returns true on a hit.
+// Closest-hit traversal of one ray as a resumable state machine: begin() sets it up,
+// each step() processes one node or leaf and returns true once the traversal has
+// ended (found / hit then hold the result).  `ah.ignore(inst_index, prim, u, v)` is
+// the any-hit shader (true = ignoreIntersectionEXT).
 //
 // The nearest hit child of a node is kept in a register and only the others go to
 // the stack.  The flattened BVH (identity instances) is traversed first, with the
 // TLAS root beneath it on the stack, so its hits bound the TLAS traversal.
-template <bool kAnyHit, bool kStats, int kDepth = kStackDepth, class AnyHit, class RaySrc>
-__device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& rs, float tmin, float tmax,
-                          uint32_t ray_flags, AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit,
-                          TraceStats& st) {
+template <bool kAnyHit, bool kStats, int kDepth = kStackDepth>
+struct Traversal {
     static_assert((kDepth & (kDepth - 1)) == 0, "the stack ring depth is a power of two");
-    const bool terminate = (ray_flags & RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT) != 0;
-    float best_t = tmax;
-    bool found = false;
-    hit.inst = kEmpty;
-    hit.prim = kEmpty;
-    TravRay cur{o, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))};
+    float3 o, d;
+    float tmin, best_t;
+    uint32_t ray_flags;
+    TravRay cur;
     WoopRay wr;
-    const WideNode* nodes = tl.nodes;
-    const float4* tris = tl.flat_tris;
-    uint32_t inst_idx = 0;
-    bool inst_opaque = true;
-    int level = kLevelTlas;
-    uint32_t blas_base = 0;
-    uint32_t sp = 0, base = 0;
-    bool lost = false;
-    uint32_t restarts = 0;
-    auto push = [&](uint32_t e) {
-        lds_stack[(sp & (kDepth - 1)) * stride] = e;
+    const WideNode* nodes;
+    const float4* tris;
+    uint32_t inst_idx;
+    bool inst_opaque, found, lost;
+    int level;
+    uint32_t blas_base, sp, base, restarts, e;
+    HitInfo hit;
+
+    __device__ void push(uint32_t* stack, uint32_t stride, uint32_t v) {
+        stack[(sp & (kDepth - 1)) * stride] = v;
         sp++;
         if (sp - base > uint32_t(kDepth)) {
             base++;
             lost = true;
         }
-    };
-    auto to_world = [&]() {
-        cur = TravRay{o, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))};
-    };
-    auto enter = [&]() -> uint32_t {
+    }
+    __device__ void to_world() { cur = TravRay{o, mk3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z))}; }
+    __device__ uint32_t enter(const TlasView& tl, uint32_t* stack, uint32_t stride) {
         to_world();
         if (tl.flat_nodes) {
-            if (tl.num_tlas) push(tl.root);
+            if (tl.num_tlas) push(stack, stride, tl.root);
             blas_base = sp;
             level = kLevelFlat;
             nodes = tl.flat_nodes;
@@ -207,9 +202,31 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& 
         level = kLevelTlas;
         nodes = tl.nodes;
         return tl.root;
-    };
-    uint32_t e = enter();
-    for (;;) {
+    }
+    __device__ void begin(const TlasView& tl, float3 ro, float3 rd, float t0, float t1, uint32_t flags,
+                          uint32_t* stack, uint32_t stride) {
+        o = ro;
+        d = rd;
+        tmin = t0;
+        best_t = t1;
+        ray_flags = flags;
+        found = false;
+        lost = false;
+        hit.inst = kEmpty;
+        hit.prim = kEmpty;
+        tris = tl.flat_tris;
+        inst_idx = 0;
+        inst_opaque = true;
+        blas_base = 0;
+        sp = 0;
+        base = 0;
+        restarts = 0;
+        e = enter(tl, stack, stride);
+    }
+
+    template <class AnyHit, class RaySrc>
+    __device__ bool step(const TlasView& tl, const RaySrc& rs, AnyHit& ah, uint32_t* stack, uint32_t stride,
+                         TraceStats& st) {
         if (!(e & kLeafBit)) {
             // internal node
             if (kStats) st.nodes++;
@@ -237,9 +254,9 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& 
 #undef RT_CSWAP
 #pragma unroll
                 for (int k = 3; k >= 1; k--)
-                    if (ent[k] != kEmpty) push(ent[k]);
+                    if (ent[k] != kEmpty) push(stack, stride, ent[k]);
                 e = ent[0];
-                continue;
+                return false;
             }
         } else if (level == kLevelTlas) {
             // TLAS leaf: one instance
@@ -272,7 +289,7 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& 
                 level = kLevelBlas;
                 blas_base = sp;
                 e = ir.root;
-                continue;
+                return false;
             }
         } else {
             // triangle leaf of a BLAS or of the flattened BVH
@@ -311,30 +328,43 @@ __device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& 
                 hit.inst = ii;
                 hit.prim = prim;
                 hit.rec = rec;
-                if (terminate) return true;
+                if (ray_flags & RT_RAY_FLAG_TERMINATE_ON_FIRST_HIT) return true;
             }
         }
         // next entry from the stack
         if (sp == base) {
-            if (!lost) break;
+            if (!lost) return true;
             lost = false;
             if (kStats) st.overflows++;
             if (++restarts > kMaxRestarts) {
                 if (tl.error_flags) atomicOr(tl.error_flags, kErrStackOverflow);
-                break;
+                return true;
             }
-            e = enter();
-            continue;
+            e = enter(tl, stack, stride);
+            return false;
         }
         sp--;
-        e = lds_stack[(sp & (kDepth - 1)) * stride];
+        e = stack[(sp & (kDepth - 1)) * stride];
         if (level != kLevelTlas && sp < blas_base) {
             level = kLevelTlas;
             to_world();
             nodes = tl.nodes;
         }
+        return false;
     }
-    return found;
+};
+
+// Runs one traversal to completion.  Returns true on a hit.
+template <bool kAnyHit, bool kStats, int kDepth = kStackDepth, class AnyHit, class RaySrc>
+__device__ bool trace_ray(const TlasView& tl, float3 o, float3 d, const RaySrc& rs, float tmin, float tmax,
+                          uint32_t ray_flags, AnyHit& ah, uint32_t* lds_stack, uint32_t stride, HitInfo& hit,
+                          TraceStats& st) {
+    Traversal<kAnyHit, kStats, kDepth> tr;
+    tr.begin(tl, o, d, tmin, tmax, ray_flags, lds_stack, stride);
+    while (!tr.step(tl, rs, ah, lds_stack, stride, st)) {
+    }
+    hit = tr.hit;
+    return tr.found;
 }
 
 }  // namespace rt

@@ -22,6 +22,9 @@ int launch_wavefront(WavefrontState* s, const FrameArgs& fa, hipStream_t st, boo
 int wavefront_num_variants();
 // Replay each chunk of queue iterations as one captured HIP graph.
 void wavefront_set_graph(WavefrontState* s, bool on);
+// Busy-lane thresholds below which a wave of the extend / shadow kernel takes new
+// rays (1 .. 64).
+void wavefront_set_refill(WavefrontState* s, uint32_t extend, uint32_t shadow);
 // Renders `count` frames (host array) in one set of queue kernels: their slots are
 // laid end to end and every kernel serves all of them.  Each frame's image is
 // the one launch_wavefront gives it alone.  Frames with no samples or bounces

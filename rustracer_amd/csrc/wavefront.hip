@@ -40,7 +40,9 @@ struct WfArrays {
     float4 *sq_o, *sq_d;  // shadow queue: origin + tmax, direction + slot bits
     uint32_t* sq_wb;
     uint32_t *q0, *q1;
-    uint32_t* counts;  // [0], [1]: path queues (ping-pong), [2]: shadow queue
+    uint32_t* counts;  // [0], [1]: path queues (ping-pong), [2]: shadow queue,
+                       // [3]: extend fetch cursor, [4]: shadow fetch cursor
+    uint32_t refill_ext, refill_sh;  // busy lanes below which a wave takes new rays (1..64)
 };
 
 enum : uint32_t { kFlagShadow = 1u, kFlagCont = 2u };
@@ -95,6 +97,25 @@ struct RayFromSlot {
         Dd = d3(a.dx[s], a.dy[s], a.dz[s]);
     }
 };
+
+// Dynamic ray fetch for the traversal kernels: a lane whose ray has ended takes the
+// next queue entry (one atomic per wave); a wave keeps stepping its rays while at
+// least `refill` of its lanes are busy, then refills.  Traversal lengths vary by an
+// order of magnitude between rays, so a wave that only took new rays when all 64
+// had ended (refill 1) would idle many lanes; refilling often costs a ballot and an
+// atomic per refill.  Returns the queue index for the lane, or kEmpty (sets
+// `exhausted`, uniformly over the wave, once the queue has run out).
+__device__ inline uint32_t wave_fetch(bool need, uint32_t* cursor, uint32_t n, bool& exhausted) {
+    uint64_t m = __ballot(need);
+    if (!m) return kEmpty;
+    uint32_t first = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == first) base = atomicAdd(cursor, uint32_t(__popcll(m)));
+    base = __shfl(base, first, 64);
+    if (base + uint32_t(__popcll(m)) >= n) exhausted = true;
+    uint32_t i = base + uint32_t(__popcll(m & lanemask_lt()));
+    return (need && i < n) ? i : kEmpty;
+}
 
 // Appends `item` to queue q (count c) for the lanes with want set: one atomic per wave.
 __device__ inline uint32_t wave_append(bool want, uint32_t* c) {
@@ -181,31 +202,51 @@ __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHi
 void k_wf_extend(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in, uint32_t* zero0, uint32_t* zero1) {
     __shared__ uint32_t s_stack[kDepth * kWfBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *zero0 = 0u;  // next path queue
-        *zero1 = 0u;  // shadow queue
+        *zero0 = 0u;        // next path queue
+        *zero1 = 0u;        // shadow queue
+        a.counts[4] = 0u;   // shadow fetch cursor
     }
     const uint32_t n = *n_in;
+    uint32_t* const stack = s_stack + threadIdx.x;
     PathCounters pc;
-    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
-        uint32_t s = q[i];
-        uint32_t x, y;
-        const FrameArgs& fa = fr.at(s, x, y);
-        const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-        float3 o = make_float3(float(a.ox[s]), float(a.oy[s]), float(a.oz[s]));
-        float3 d = make_float3(float(a.dx[s]), float(a.dy[s]), float(a.dz[s]));
-        if (kStats) { if (a.bounce[s] == 0u) pc.primary++; else pc.extension++; }
-        Rng4 rb{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
-        AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
-        HitInfo h;
-        bool found = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, o, d, RayFromSlot{a, s}, float(kTMin), float(a.tfar[s]),
-                                                        flags, ah, s_stack + threadIdx.x, kWfBlock, h, pc.tr);
-        if (kAnyHit) a.wb[s] = rb.w;
-        a.h_inst[s] = found ? h.inst : kEmpty;
-        a.h_prim[s] = h.prim;
-        a.h_rec[s] = h.rec;
-        a.h_t[s] = h.t;
-        a.h_u[s] = h.u;
-        a.h_v[s] = h.v;
+    Traversal<kAnyHit, kStats, kDepth> tr;
+    uint32_t s = 0, x = 0, y = 0;
+    Rng4 rb{0u, 0u, 0u, 0u};
+    bool active = false, exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            uint32_t i = wave_fetch(!active, a.counts + 3, n, exhausted);
+            if (i != kEmpty) {
+                s = q[i];
+                const FrameArgs& fa = fr.at(s, x, y);
+                const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
+                float3 o = make_float3(float(a.ox[s]), float(a.oy[s]), float(a.oz[s]));
+                float3 d = make_float3(float(a.dx[s]), float(a.dy[s]), float(a.dz[s]));
+                if (kStats) { if (a.bounce[s] == 0u) pc.primary++; else pc.extension++; }
+                rb = Rng4{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
+                tr.begin(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), flags, stack, kWfBlock);
+                active = true;
+            }
+        }
+        if (!__ballot(active)) break;
+        const uint32_t keep = exhausted ? 1u : a.refill_ext;
+        do {
+            if (active) {
+                uint32_t xx, yy;
+                const FrameArgs& fa = fr.at(s, xx, yy);
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
+                if (tr.step(fa.tlas, RayFromSlot{a, s}, ah, stack, kWfBlock, pc.tr)) {
+                    if (kAnyHit) a.wb[s] = rb.w;
+                    a.h_inst[s] = tr.found ? tr.hit.inst : kEmpty;
+                    a.h_prim[s] = tr.hit.prim;
+                    a.h_rec[s] = tr.hit.rec;
+                    a.h_t[s] = tr.hit.t;
+                    a.h_u[s] = tr.hit.u;
+                    a.h_v[s] = tr.hit.v;
+                    active = false;
+                }
+            }
+        } while (uint32_t(__popcll(__ballot(active))) >= keep);
     }
     if (kStats) flush_counters(fr.counters(), pc);
 }
@@ -256,22 +297,42 @@ template <class Frames, bool kAnyHit, bool kStats, int kDepth, int kWaves>
 __global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
 void k_wf_shadow(Frames fr, WfArrays a) {
     __shared__ uint32_t s_stack[kDepth * kWfBlock];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[3] = 0u;  // next extend's fetch cursor
     const uint32_t n = a.counts[2];
+    uint32_t* const stack = s_stack + threadIdx.x;
     PathCounters pc;
-    for (uint32_t i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
-        float4 o = a.sq_o[i], d = a.sq_d[i];
-        uint32_t s = __float_as_uint(d.w);
-        uint32_t x, y;
-        const FrameArgs& fa = fr.at(s, x, y);
-        const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
-        if (kStats) pc.shadow++;
-        Rng4 shadow_rng{x, y, fa.seed_b, a.sq_wb[i]};
-        AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
-        HitInfo sh;
-        const float3 so = make_float3(o.x, o.y, o.z), sd = make_float3(d.x, d.y, d.z);
-        bool hit = trace_ray<kAnyHit, kStats, kDepth>(fa.tlas, so, sd, RayFromF32{so, sd}, float(kShadowTMin), o.w,
-                                                      flags, sah, s_stack + threadIdx.x, kWfBlock, sh, pc.trs);
-        a.unocc[s] = hit ? 0u : 1u;
+    Traversal<kAnyHit, kStats, kDepth> tr;
+    uint32_t s = 0, x = 0, y = 0;
+    Rng4 shadow_rng{0u, 0u, 0u, 0u};
+    bool active = false, exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            uint32_t i = wave_fetch(!active, a.counts + 4, n, exhausted);
+            if (i != kEmpty) {
+                float4 o = a.sq_o[i], d = a.sq_d[i];
+                s = __float_as_uint(d.w);
+                const FrameArgs& fa = fr.at(s, x, y);
+                const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
+                if (kStats) pc.shadow++;
+                shadow_rng = Rng4{x, y, fa.seed_b, a.sq_wb[i]};
+                tr.begin(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), float(kShadowTMin), o.w,
+                         flags, stack, kWfBlock);
+                active = true;
+            }
+        }
+        if (!__ballot(active)) break;
+        const uint32_t keep = exhausted ? 1u : a.refill_sh;
+        do {
+            if (active) {
+                uint32_t xx, yy;
+                const FrameArgs& fa = fr.at(s, xx, yy);
+                AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
+                if (tr.step(fa.tlas, RayFromF32{tr.o, tr.d}, sah, stack, kWfBlock, pc.trs)) {
+                    a.unocc[s] = tr.found ? 0u : 1u;
+                    active = false;
+                }
+            }
+        } while (uint32_t(__popcll(__ballot(active))) >= keep);
     }
     if (kStats) flush_counters(fr.counters(), pc);
 }
@@ -343,12 +404,12 @@ size_t layout(WfArrays& a, char* base, uint32_t n) {
     total += carve(p, a.h_v, n);
     total += carve(p, a.sq_o, n);
     total += carve(p, a.sq_d, n);
-    total += carve(p, a.counts, 4);
+    total += carve(p, a.counts, 8);
     return total;
 }
 
 __global__ void k_zero_counts(uint32_t* c) {
-    if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+    if (threadIdx.x < 8) c[threadIdx.x] = 0u;
 }
 
 template <class Frames> struct Kernels {
@@ -394,6 +455,7 @@ struct WavefrontState {
     char* batch_mem = nullptr;  // frame arguments + slot offsets of a batch
     size_t batch_cap = 0;
     bool use_graph = false;     // replay captured chunks (RT_OPTION_GRAPH)
+    uint32_t refill_ext = 40, refill_sh = 1;
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t exec = nullptr;
     int cus = 0;
@@ -413,6 +475,13 @@ struct WavefrontState {
 WavefrontState* wavefront_create() { return new WavefrontState(); }
 
 void wavefront_set_graph(WavefrontState* s, bool on) { s->use_graph = on; }
+
+void wavefront_set_refill(WavefrontState* s, uint32_t extend, uint32_t shadow) {
+    s->refill_ext = std::min(std::max(extend, 1u), 64u);
+    s->refill_sh = std::min(std::max(shadow, 1u), 64u);
+    s->a.refill_ext = s->refill_ext;
+    s->a.refill_sh = s->refill_sh;
+}
 
 void wavefront_destroy(WavefrontState* s) {
     if (!s) return;
@@ -440,6 +509,8 @@ int ensure_slots(WavefrontState* ws, uint32_t nslots, std::string* err) {
     }
     ws->cap = nslots;
     layout(ws->a, ws->mem, nslots);
+    ws->a.refill_ext = ws->refill_ext;
+    ws->a.refill_sh = ws->refill_sh;
     return 0;
 }
 

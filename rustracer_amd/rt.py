@@ -67,6 +67,11 @@ class Bindings(C.Structure):
                 ("num_indices", C.c_uint32), ("num_prim_infos", C.c_uint32), ("num_materials", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("traversal_variant", C.c_uint32), ("graph", C.c_uint32), ("extend_refill", C.c_uint32),
+                ("shadow_refill", C.c_uint32), ("frame_ms", C.c_double)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("tlas", C.c_uint64), ("bindings", C.c_void_p), ("accum", C.c_void_p), ("width", C.c_uint32),
                 ("height", C.c_uint32)]
@@ -84,6 +89,8 @@ OPTION_STAGE_TIMING = 1
 OPTION_FLATTEN_IDENTITY_INSTANCES = 2
 OPTION_TRAVERSAL_VARIANT = 3
 OPTION_GRAPH = 4
+OPTION_EXTEND_REFILL = 5
+OPTION_SHADOW_REFILL = 6
 NUM_TRAVERSAL_VARIANTS = 4
 
 
@@ -123,7 +130,7 @@ def load_library(path=LIB_PATH):
         "rt_tlas_flat_info": (i, [vp, u64, vp]),
         "rt_trace_rays_adjoint": (i, [vp, u64, vp, vp, u, u, vp, vp]),
         "rt_trace_rays_batch": (i, [vp, vp, u, u, vp]),
-        "rt_autotune": (i, [vp, u64, vp, u, u, vp, vp, vp]),
+        "rt_autotune": (i, [vp, u64, vp, u, u, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -171,13 +178,12 @@ class Context:
         self.check(self.lib.rt_sync(self.handle))
 
     def autotune(self, tlas, buffers, ubo, width, height):
-        """rt_autotune: picks the fastest traversal variant and graph replay for this
-        frame and keeps them on the context.  Returns (variant, graph, ms)."""
+        """rt_autotune: tunes the traversal variant, refill thresholds and graph
+        replay for this frame and keeps them on the context.  Returns a dict."""
         b = buffers.bindings(ubo)
-        v, g, ms = C.c_uint32(), C.c_uint32(), C.c_double()
-        self.check(self.lib.rt_autotune(self.handle, tlas.handle, C.byref(b), width, height, C.byref(v), C.byref(g),
-                                        C.byref(ms)))
-        return v.value, g.value, ms.value
+        t = Tuning()
+        self.check(self.lib.rt_autotune(self.handle, tlas.handle, C.byref(b), width, height, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in Tuning._fields_}
 
     def set_option(self, option, value):
         """rt_set_option (OPTION_STAGE_TIMING: event timing of every pipeline stage)."""

@@ -195,8 +195,16 @@ def test_traversal_variants_render_the_same_image(gpu_ctx, name):
             gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, v)
             img, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
             assert np.array_equal(ref.cpu().numpy(), img.cpu().numpy()), v
+        gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
+        for re, rs in ((1, 1), (64, 64), (17, 33)):
+            gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, re)
+            gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, rs)
+            img, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
+            assert np.array_equal(ref.cpu().numpy(), img.cpu().numpy()), (re, rs)
     finally:
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
+        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
+        gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)
     with pytest.raises(rt.RtError):
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, rt.NUM_TRAVERSAL_VARIANTS)
 
@@ -232,13 +240,16 @@ def test_autotune_keeps_the_image(gpu_ctx):
     sc = ds.scene
     ref, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
     try:
-        v, g, ms = gpu_ctx.autotune(ds.tlas, ds.buffers, sc.ubo, sc.width, sc.height)
-        assert 0 <= v < rt.NUM_TRAVERSAL_VARIANTS and g in (0, 1) and ms > 0
+        t = gpu_ctx.autotune(ds.tlas, ds.buffers, sc.ubo, sc.width, sc.height)
+        assert 0 <= t["traversal_variant"] < rt.NUM_TRAVERSAL_VARIANTS and t["graph"] in (0, 1) and t["frame_ms"] > 0
+        assert 1 <= t["extend_refill"] <= 64 and 1 <= t["shadow_refill"] <= 64
         a, _ = _render(gpu_ctx, ds, mode=rt.MODE_WAVEFRONT)
         assert np.array_equal(a.cpu().numpy(), ref.cpu().numpy())
     finally:
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
         gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
+        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
+        gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)
 
 
 def test_autotune_on_a_deep_bvh_leaves_no_error(gpu_ctx):
@@ -250,10 +261,12 @@ def test_autotune_on_a_deep_bvh_leaves_no_error(gpu_ctx):
     ubo["number_of_samples"] = ubo["total_number_of_samples"] = 2
     ref, _ = _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT)
     try:
-        v, g, ms = gpu_ctx.autotune(ds.tlas, ds.buffers, ubo, sc.width, sc.height)
+        gpu_ctx.autotune(ds.tlas, ds.buffers, ubo, sc.width, sc.height)
         gpu_ctx.sync()
         a, st = _render(gpu_ctx, ds, ubo=ubo, mode=rt.MODE_WAVEFRONT, stats=True)
         assert np.array_equal(a.cpu().numpy(), ref.cpu().numpy())
     finally:
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
         gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
+        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
+        gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)

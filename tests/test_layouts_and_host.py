@@ -20,7 +20,7 @@ PROBE = r"""
 #define Z(T) printf(#T " %zu\n", sizeof(T));
 int main(void) {
   Z(rt_vertex) Z(rt_prim_info) Z(rt_material_raw) Z(rt_light) Z(rt_ubo) Z(rt_instance)
-  Z(rt_blas_input) Z(rt_texture) Z(rt_bindings) Z(rt_trace_stats) Z(rt_ray) Z(rt_hit) Z(rt_bvh_info) Z(rt_stage_times) Z(rt_batch_item)
+  Z(rt_blas_input) Z(rt_texture) Z(rt_bindings) Z(rt_trace_stats) Z(rt_ray) Z(rt_hit) Z(rt_bvh_info) Z(rt_stage_times) Z(rt_batch_item) Z(rt_tuning)
   F(rt_material_raw, metallic_roughness_texture) F(rt_material_raw, transmission_exist)
   F(rt_material_raw, attenuation_distance) F(rt_material_raw, specular_color_factor)
   F(rt_material_raw, sg_specular_glossiness_factor) F(rt_material_raw, sg_specular_glossiness_texture)
@@ -61,7 +61,7 @@ def test_ctypes_structs_match_c_headers(c_layout):
     from rustracer_amd import rt
     for name, st in [("rt_blas_input", rt.BlasInput), ("rt_texture", rt.Texture), ("rt_bindings", rt.Bindings),
                      ("rt_trace_stats", rt.TraceStats), ("rt_bvh_info", rt.BvhInfo), ("rt_stage_times", rt.StageTimes),
-                     ("rt_batch_item", rt.BatchItem)]:
+                     ("rt_batch_item", rt.BatchItem), ("rt_tuning", rt.Tuning)]:
         assert c_layout[name] == C.sizeof(st), name
     assert c_layout["rt_bindings.num_materials"] == rt.Bindings.num_materials.offset
     assert c_layout["rt_trace_stats.kernel_ms"] == rt.TraceStats.kernel_ms.offset
