@@ -319,8 +319,11 @@ int rt_trace_primary(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t w
 
 /* ------------------------------------------------------------- inspection */
 
-/* Layout of a built BLAS, for validity checks.  Node records are 128 bytes
- * (4-wide, fp32 child boxes); triangle records are 48 bytes. */
+/* Layout of a built BLAS, for validity checks.  Node records are 64 bytes (4-wide;
+ * child boxes as 8-bit steps of a per-axis power-of-two grid: origin float[3],
+ * exponent int8[3], pad, planes uint8[24] (lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4]
+ * hi_z[4], plane = origin + q * 2^exponent in fp32), child uint32[4], count uint8[4],
+ * pad); triangle records are 48 bytes. */
 typedef struct rt_bvh_info {
     uint64_t nodes;          /* device pointer to the node array */
     uint64_t tris;           /* device pointer to the triangle/instance records */

@@ -2,8 +2,8 @@
 
 A BLAS is valid when, walking from its root: every child reference is in range,
 no node is reached twice, every triangle record of the BLAS lies in exactly one
-leaf, every child box contains the boxes of its subtree and every leaf box
-contains its triangles (in float, no tolerance)."""
+leaf, and every triangle lies inside every box on its path from the root, as the
+traversal decodes the quantised boxes (in fp32, no tolerance)."""
 import numpy as np
 
 from . import rt
@@ -24,9 +24,9 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
     errors = []
     seen_nodes = np.zeros(len(nodes), bool)
     covered = np.zeros(num_prims, np.int32)
-    stack = [(root, None)]
+    stack = [(root, None)]  # (node, intersection of the decoded boxes on the path)
     while stack:
-        ni, parent_box = stack.pop()
+        ni, path_box = stack.pop()
         if ni >= len(nodes):
             errors.append(f"node index {ni} out of range {len(nodes)}")
             continue
@@ -41,10 +41,11 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
                 continue
             lo = np.array([nd["lo_x"][k], nd["lo_y"][k], nd["lo_z"][k]], np.float32)
             hi = np.array([nd["hi_x"][k], nd["hi_y"][k], nd["hi_z"][k]], np.float32)
-            if parent_box is not None and (np.any(lo < parent_box[0]) or np.any(hi > parent_box[1])):
-                errors.append(f"node {ni} child {k} box not inside parent box")
+            if np.any(lo > hi):
+                errors.append(f"node {ni} child {k} box is inverted")
+            box = (lo, hi) if path_box is None else (np.maximum(lo, path_box[0]), np.minimum(hi, path_box[1]))
             if cnt == 0:
-                stack.append((c, (lo, hi)))
+                stack.append((c, box))
                 continue
             if c < first_prim or c + cnt > first_prim + num_prims:
                 errors.append(f"leaf [{c},{c + cnt}) outside [{first_prim},{first_prim + num_prims})")
@@ -53,8 +54,8 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
             if kind in ("triangle", "flat"):
                 t = tris[c:c + cnt]
                 pts = np.concatenate([t["v0"], t["v1"][:, :3], t["v2"][:, :3]])
-                if np.any(pts < lo) or np.any(pts > hi):
-                    errors.append(f"leaf [{c},{c + cnt}) triangles not inside leaf box")
+                if np.any(pts < box[0]) or np.any(pts > box[1]):
+                    errors.append(f"leaf [{c},{c + cnt}) triangles not inside every box on their path")
     if np.any(covered != 1):
         bad = np.nonzero(covered != 1)[0]
         errors.append(f"{len(bad)} records not in exactly one leaf (first {bad[:5] + first_prim}, counts {covered[bad[:5]]})")

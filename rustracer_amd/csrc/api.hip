@@ -61,13 +61,13 @@ struct TlasEntry {
 
 TlasView tlas_view(const TlasEntry& t, uint32_t* err) {
     TlasView v{};
-    v.nodes = (const WideNode*)t.nodes.get();
+    v.nodes = (const QNode*)t.nodes.get();
     v.inst = (const InstanceRecord*)t.inst.get();
     v.root = 0;
     v.num_inst = t.num_inst;
     v.error_flags = err;
     v.num_tlas = t.num_tlas;
-    v.flat_nodes = t.flat_num_tris ? (const WideNode*)t.flat_nodes.get() : nullptr;
+    v.flat_nodes = t.flat_num_tris ? (const QNode*)t.flat_nodes.get() : nullptr;
     v.flat_tris = (const float4*)t.flat_tris.get();
     v.flat_root = 0;
     return v;
@@ -376,7 +376,7 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
     pool->descs.reset(dev_alloc<BlasDesc>(count, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-    pool->bytes = sizeof(WideNode) * size_t(res.num_nodes) + 48 * size_t(n) + sizeof(BlasDesc) * count;
+    pool->bytes = sizeof(QNode) * size_t(res.num_nodes) + 48 * size_t(n) + sizeof(BlasDesc) * count;
     rc = reorder_triangles(st, (const float4*)d_tri_in.get(), (const uint32_t*)perm.get(), n, (float4*)pool->tris.get());
     if (rc) return hip_fail(ctx, hipError_t(rc), "reorder_triangles");
     std::vector<float> hb(6 * size_t(count));
@@ -388,7 +388,7 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     std::vector<BlasDesc> descs(count);
     for (uint32_t i = 0; i < count; i++) {
         BlasDesc& d = descs[i];
-        d.nodes = (const WideNode*)pool->nodes.get();
+        d.nodes = (const QNode*)pool->nodes.get();
         d.tris = (const float4*)pool->tris.get();
         d.root = i;
         d.num_prims = inputs[i].primitive_count;

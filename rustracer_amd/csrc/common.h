@@ -1,9 +1,10 @@
 // common.h — internal device layouts and small device helpers of the gfx950 library.
 //
 // Acceleration-structure memory (library-owned):
-//   WideNode   128 B: a 4-wide BVH node holding its children's fp32 boxes (SoA per
-//              axis, so one node is one 128-byte HBM transaction pair of dwordx4
-//              loads), child references and leaf counts.
+//   QNode       64 B: a 4-wide BVH node holding its children's boxes quantised to
+//              8-bit steps of a per-axis power-of-two grid (four dwordx4 loads),
+//              child references and leaf counts.  The builder works on 128-B
+//              WideNodes with fp32 boxes and quantises them last.
 //   Triangle    48 B: three float4 {v.xyz, w}; v0.w carries the primitive index
 //              (gl_PrimitiveID) as bits.  Leaves reference contiguous runs.
 //   InstanceRecord 256 B: world->object rows in fp64, BLAS node/triangle pools
@@ -25,6 +26,7 @@ constexpr int kLeafCountShift = 27;          // leaf entry: bits 27..30 = count 
 constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
 constexpr int kMaxLeafPrims = 16;
 
+// The builder's 4-wide node with fp32 child boxes (also the decoded form of QNode).
 struct alignas(128) WideNode {
     float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
     uint32_t child[4];  // internal: node index; leaf: first record; empty: kEmpty
@@ -32,10 +34,35 @@ struct alignas(128) WideNode {
 };
 static_assert(sizeof(WideNode) == 128, "WideNode is 128 bytes");
 
+// The traversal's node, 64 B: the four child boxes as 8-bit steps on a per-axis
+// power-of-two grid anchored at `origin`; a plane is fmaf(q, 2^exp, origin) in
+// fp32.  The builder rounds every plane outward and checks that exact fp32 decode,
+// so a decoded box always contains the child's fp32 box.
+struct alignas(64) QNode {
+    float origin[3];
+    int8_t exp[3];
+    uint8_t _pad0;
+    uint8_t q[24];      // lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4]
+    uint32_t child[4];  // as WideNode
+    uint8_t count[4];   // as WideNode (at most kMaxLeafPrims)
+    uint32_t _pad1;
+};
+static_assert(sizeof(QNode) == 64, "QNode is 64 bytes");
+
+// 2^e as a float, e in [-126, 127]
+__host__ __device__ inline float pow2f(int e) {
+    union {
+        uint32_t u;
+        float f;
+    } v;
+    v.u = uint32_t(e + 127) << 23;
+    return v.f;
+}
+
 enum : uint32_t { kBlasOpaque = 1u };
 
 struct alignas(64) BlasDesc {
-    const WideNode* nodes;  // node pool base (shared by the BLASes of one batch)
+    const QNode* nodes;     // node pool base (shared by the BLASes of one batch)
     const float4* tris;     // triangle pool base, 3 float4 per record
     uint32_t root;          // root node index in the pool
     uint32_t num_prims;
@@ -55,7 +82,7 @@ enum : uint32_t {
 struct alignas(16) InstanceRecord {
     // first 128 bytes: what traversal reads when it enters the instance
     double w2o_d[12];       // world-to-object 3x4 rows in fp64 (traversal, hit refinement)
-    const WideNode* nodes;  // BLAS node pool
+    const QNode* nodes;     // BLAS node pool
     const float4* tris;     // BLAS triangle pool
     uint32_t root;          // BLAS root node
     uint32_t mask;          // 8-bit visibility mask
@@ -78,13 +105,13 @@ enum : uint32_t { kErrStackOverflow = 1u };
 constexpr uint32_t kFlatOpaque = 0x80000000u;
 
 struct TlasView {
-    const WideNode* nodes;
+    const QNode* nodes;
     const InstanceRecord* inst;  // TLAS leaf order, then the flattened instances
     uint32_t root;
     uint32_t num_inst;           // records in inst
     uint32_t* error_flags;       // kErr* bits raised by kernels (may be NULL)
     uint32_t num_tlas;           // instances reachable through the TLAS (0: skip it)
-    const WideNode* flat_nodes;  // flattened BVH (NULL: none)
+    const QNode* flat_nodes;     // flattened BVH (NULL: none)
     const float4* flat_tris;
     uint32_t flat_root;
     uint32_t _pad;

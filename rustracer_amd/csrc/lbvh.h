@@ -30,7 +30,7 @@ struct TriSegment {
 // Scratch and results of one build; device pointers are owned by the caller
 // (api.hip) so that they can be pooled.
 struct LbvhResult {
-    WideNode* nodes = nullptr;      // final node array (exact size)
+    QNode* nodes = nullptr;         // final (quantised) node array, exact size
     uint32_t num_nodes = 0;
     uint32_t* perm = nullptr;       // sorted position -> input primitive index
     float* seg_bounds = nullptr;    // 6 floats per segment (padded root box)

@@ -89,6 +89,77 @@ __global__ void k_reorder_tris(const float4* in, const uint32_t* perm, uint32_t 
     out[3 * size_t(p) + 2] = in[3 * size_t(s) + 2];
 }
 
+// One axis of a node: the grid (origin, exponent) and the 8-bit planes of the
+// non-empty children, each rounded outward so that the fp32 decode
+// fmaf(q, 2^e, origin) traversal evaluates lies on or outside the child's plane.
+__device__ inline void quantize_axis(const float lo[4], const float hi[4], const uint32_t child[4], float& origin,
+                                     int8_t& exp, uint8_t qlo[4], uint8_t qhi[4]) {
+    float o = INFINITY, top = -INFINITY;
+    for (int k = 0; k < 4; k++) {
+        qlo[k] = 0;
+        qhi[k] = 0;
+        if (child[k] == kEmpty) continue;
+        o = fminf(o, lo[k]);
+        top = fmaxf(top, hi[k]);
+    }
+    if (!(o <= top)) {  // no child
+        origin = 0.f;
+        exp = 0;
+        return;
+    }
+    // smallest e with 254 * 2^e >= the extent (one step of room for the outward fixes)
+    const double ext = double(top) - double(o);
+    int e = -126;
+    if (ext > 0.0) {
+        int k = 0;
+        (void)frexp(ext / 254.0, &k);  // ext / 254 = m 2^k, m in [0.5, 1)
+        e = k < -126 ? -126 : (k > 127 ? 127 : k);
+    }
+    for (; e <= 127; e++) {
+        const float s = pow2f(e);
+        bool fits = true;
+        for (int k = 0; k < 4 && fits; k++) {
+            if (child[k] == kEmpty) continue;
+            float ql = floorf((lo[k] - o) / s);
+            if (!(ql >= 0.f)) ql = 0.f;
+            if (ql > 255.f) ql = 255.f;
+            for (int it = 0; it < 4 && ql > 0.f && fmaf(ql, s, o) > lo[k]; it++) ql -= 1.f;
+            if (fmaf(ql, s, o) > lo[k]) ql = 0.f;  // q = 0 decodes to the origin, the smallest lo
+            float qh = ceilf((hi[k] - o) / s);
+            if (!(qh >= ql)) qh = ql;
+            if (qh > 256.f) qh = 256.f;
+            for (int it = 0; it < 4 && fmaf(qh, s, o) < hi[k]; it++) qh += 1.f;
+            if (qh > 255.f || fmaf(qh, s, o) < hi[k]) {
+                fits = false;
+                break;
+            }
+            qlo[k] = uint8_t(ql);
+            qhi[k] = uint8_t(qh);
+        }
+        if (fits || e == 127) break;
+    }
+    if (e > 127) e = 127;
+    origin = o;
+    exp = int8_t(e);
+}
+
+__global__ void k_quantize_nodes(const WideNode* in, uint32_t n, QNode* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const WideNode w = in[i];
+    QNode q;
+    q._pad0 = 0;
+    q._pad1 = 0;
+    quantize_axis(w.lo_x, w.hi_x, w.child, q.origin[0], q.exp[0], q.q + 0, q.q + 4);
+    quantize_axis(w.lo_y, w.hi_y, w.child, q.origin[1], q.exp[1], q.q + 8, q.q + 12);
+    quantize_axis(w.lo_z, w.hi_z, w.child, q.origin[2], q.exp[2], q.q + 16, q.q + 20);
+    for (int k = 0; k < 4; k++) {
+        q.child[k] = w.child[k];
+        q.count[k] = uint8_t(w.count[k]);
+    }
+    out[i] = q;
+}
+
 __global__ void k_gather_flat(const FlatSource* src, uint32_t nsrc, uint32_t n, float4* tri, float4* lo, float4* hi) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -805,12 +876,13 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
         return fail(int(hipErrorUnknown), "read node count");
     if (hc[3] != 0u || hc[0] > max_nodes) return fail(int(hipErrorUnknown), "node capacity exceeded during collapse");
     uint32_t num_nodes = hc[0];
-    WideNode* nodes = nullptr;
-    if (hipMalloc(&nodes, sizeof(WideNode) * size_t(num_nodes)) != hipSuccess) return fail(int(hipErrorOutOfMemory), "out of device memory");
-    if (hipMemcpyAsync(nodes, nodes_tmp, sizeof(WideNode) * size_t(num_nodes), hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+    QNode* nodes = nullptr;
+    if (hipMalloc(&nodes, sizeof(QNode) * size_t(num_nodes)) != hipSuccess) return fail(int(hipErrorOutOfMemory), "out of device memory");
+    hipLaunchKernelGGL(k_quantize_nodes, dim3(blocks_for(num_nodes, kBlock)), dim3(kBlock), 0, st,
+                       (const WideNode*)nodes_tmp, num_nodes, nodes);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
         (void)hipFree(nodes);
-        return fail(int(hipErrorUnknown), "copy nodes");
+        return fail(int(hipErrorUnknown), "quantise nodes");
     }
     out->nodes = nodes;
     out->num_nodes = num_nodes;

@@ -120,12 +120,30 @@ __device__ inline uint32_t node_hits(const WideNode& n, const TravRay& r, float 
     return mask;
 }
 
-__device__ inline WideNode load_node(const WideNode* nodes, uint32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+__device__ inline float qbyte(uint32_t w, int k) { return float((w >> (8 * k)) & 255u); }
+
+// Four dwordx4 loads and the decode of the child planes (fmaf(q, 2^e, origin)).
+__device__ inline WideNode load_node(const QNode* nodes, uint32_t i) {
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const float sx = pow2f(int(int8_t(w0.w & 255u))), sy = pow2f(int(int8_t((w0.w >> 8) & 255u)));
+    const float sz = pow2f(int(int8_t((w0.w >> 16) & 255u)));
     WideNode n;
-    float4* q = reinterpret_cast<float4*>(&n);
 #pragma unroll
-    for (int k = 0; k < 8; k++) q[k] = p[k];
+    for (int k = 0; k < 4; k++) {
+        n.lo_x[k] = fmaf(qbyte(w1.x, k), sx, ox);
+        n.hi_x[k] = fmaf(qbyte(w1.y, k), sx, ox);
+        n.lo_y[k] = fmaf(qbyte(w1.z, k), sy, oy);
+        n.hi_y[k] = fmaf(qbyte(w1.w, k), sy, oy);
+        n.lo_z[k] = fmaf(qbyte(w2.x, k), sz, oz);
+        n.hi_z[k] = fmaf(qbyte(w2.y, k), sz, oz);
+        n.count[k] = (w3.z >> (8 * k)) & 255u;
+    }
+    n.child[0] = w2.z;
+    n.child[1] = w2.w;
+    n.child[2] = w3.x;
+    n.child[3] = w3.y;
     return n;
 }
 
@@ -171,7 +189,7 @@ struct Traversal {
     uint32_t ray_flags;
     TravRay cur;
     WoopRay wr;
-    const WideNode* nodes;
+    const QNode* nodes;
     const float4* tris;
     uint32_t inst_idx;
     bool inst_opaque, found, lost;

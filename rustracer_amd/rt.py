@@ -408,9 +408,32 @@ def trace_primary(ctx, tlas, buffers, ubo, width, height):
 
 
 def read_nodes(ctx, info):
-    """Copies a BVH's node array (128-byte nodes) to the host as structured records."""
+    """Copies a BVH's node array (64-byte quantised nodes) to the host and decodes it
+    into NODE records (fp32 child boxes exactly as traversal decodes them)."""
     ctx.sync()
-    return np.frombuffer(read_device(info.nodes, 128 * info.num_nodes), NODE)
+    raw = np.frombuffer(read_device(info.nodes, QNODE.itemsize * info.num_nodes), QNODE)
+    return decode_nodes(raw)
+
+
+def decode_nodes(raw):
+    """QNode -> NODE: plane = origin + q * 2^exp in fp32 (q * 2^exp is exact, so the
+    single rounding matches the device's fmaf)."""
+    out = np.zeros(len(raw), NODE)
+    scale = np.ldexp(np.float32(1.0), raw["exp"].astype(np.int32)).astype(np.float32)  # (n, 3)
+    q = raw["q"].astype(np.float32).reshape(-1, 6, 4)
+    for a, (lo, hi) in enumerate((("lo_x", "hi_x"), ("lo_y", "hi_y"), ("lo_z", "hi_z"))):
+        o = raw["origin"][:, a:a + 1].astype(np.float32)
+        s = scale[:, a:a + 1]
+        out[lo] = (o + q[:, 2 * a] * s).astype(np.float32)
+        out[hi] = (o + q[:, 2 * a + 1] * s).astype(np.float32)
+    out["child"] = raw["child"]
+    out["count"] = raw["count"].astype(np.uint32)
+    return out
+
+
+QNODE = np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("_pad0", "u1"), ("q", "u1", 24), ("child", "<u4", 4),
+                  ("count", "u1", 4), ("_pad1", "<u4")])
+assert QNODE.itemsize == 64
 
 
 NODE = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<f4", 4), ("hi_y", "<f4", 4),
