@@ -229,7 +229,11 @@ enum {
      * traversing (1 = only when all have finished).  Same images.  Defaults:
      * extend 40, shadow 1. */
     RT_OPTION_EXTEND_REFILL = 5,
-    RT_OPTION_SHADOW_REFILL = 6
+    RT_OPTION_SHADOW_REFILL = 6,
+    /* Byte order of rt_trace_rays' display image: 0 (default) R, G, B, A; 1: B, G,
+     * R, A, the reference's storage image (the swapchain's B8G8R8A8_UNORM,
+     * crates/libs/vulkan/src/swapchain.rs:39-48, crates/libs/app/src/lib.rs:305-312). */
+    RT_OPTION_OUTPUT_BGRA = 7
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 
@@ -269,7 +273,13 @@ int rt_get_stage_times(rt_ctx* ctx, rt_stage_times* out);
  * per pixel, ubo->number_of_samples samples of up to ubo->number_of_bounces traces,
  * accumulated into `accum` (rgba32f, row-major, width*height*4 floats):
  *   accum = (number_of_samples != total_number_of_samples ? accum : 0) + sum(radiance), alpha = 0.
- * `out_rgba8` (nullable) receives the tonemapped display image.
+ * `out_rgba8` (nullable, 4-byte aligned, width*height*4 bytes) receives the display
+ * image of RayTracing.rgen:132-166: accum / total_number_of_samples tone-mapped by
+ * ubo->tone_mapping_mode (0 default, 1 Uncharted 2, 2 Hejl-Richard, 3 ACES, else
+ * gamma 2.2 only), replaced by the DISTANCE mapping's grey level or, in the
+ * megakernel, the HEAT mapping's colour ramp of the pixel's shader-clock ticks
+ * (RT_ERR_UNSUPPORTED in the wavefront mode), stored as 8-bit UNORM with alpha 255
+ * (byte order: RT_OPTION_OUTPUT_BGRA).
  * `mode` selects the plain megakernel or the wavefront pipeline; both produce the
  * same image bit for bit. */
 int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,

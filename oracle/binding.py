@@ -92,6 +92,8 @@ def lib():
             "orc_render_pixels": (i, [vp, vp, u, u, vp, u, i, i, vp, vp]),
             "orc_trace_primary": (i, [vp, vp, u, u, vp, vp, i]),
             "orc_render": (i, [vp, vp, u, u, u, u, u, u, i, i, vp, vp]),
+            "orc_tone_map": (C.c_float, [C.c_float, u]), "orc_heatmap": (None, [C.c_float, vp]),
+            "orc_output_image": (None, [vp, u, vp, vp, vp, i, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -241,3 +243,27 @@ class OracleScene:
         stats = {n: getattr(st, n) for n, _ in OrcStats._fields_}
         stats["rays"] = stats["primary_rays"] + stats["extension_rays"] + stats["shadow_rays"]
         return accum, stats
+
+
+def tone_map(c, mode):
+    """orc_tone_map: one fp32 channel through tone-mapping mode `mode`."""
+    return float(lib().orc_tone_map(float(c), int(mode)))
+
+
+def heatmap(t):
+    out = np.zeros(3, np.float32)
+    lib().orc_heatmap(float(t), out.ctypes.data)
+    return out
+
+
+def output_image(accum, ubo, last_t=None, ticks=None, bgra=False):
+    """orc_output_image: (h, w, 4) float32 accumulation -> (h, w, 4) uint8 display image."""
+    acc = np.ascontiguousarray(accum, np.float32)
+    n = acc.size // 4
+    ubo = np.ascontiguousarray(ubo)
+    lt = None if last_t is None else np.ascontiguousarray(last_t, np.float32).reshape(-1)
+    tk = None if ticks is None else np.ascontiguousarray(ticks, np.uint64).reshape(-1)
+    out = np.zeros(n, np.uint32)
+    lib().orc_output_image(acc.ctypes.data, n, lt.ctypes.data if lt is not None else None,
+                           tk.ctypes.data if tk is not None else None, ubo.ctypes.data, int(bool(bgra)), out.ctypes.data)
+    return out.view(np.uint8).reshape(acc.shape[:-1] + (4,))

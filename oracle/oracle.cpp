@@ -896,6 +896,7 @@ template <class R> struct Tracer {
     const Scene& s;
     const rt_ubo& ubo;
     orc_stats* st;
+    double last_t = 0.0;  // Ray.t after the pixel's last trace (read by the DISTANCE mapping, rgen:160)
 
     HitRec cast(V3<R> o, V3<R> d, R tmin, R tmax, uint32_t flags, U4* rng) {
         double od[3] = {double(o.x), double(o.y), double(o.z)};
@@ -1250,6 +1251,7 @@ template <class R> struct Tracer {
                 d = ray.scatter_direction;
             }
         }
+        last_t = double(ray.t);
         return radiance;
     }
 };
@@ -1413,12 +1415,13 @@ int orc_render_pixels(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, const
                 Tracer<float> tr{s, *ubo, stats ? &local : nullptr};
                 V3<float> r = tr.pixel(x, y, W, H);
                 o[0] = r.x; o[1] = r.y; o[2] = r.z;
+                o[3] = tr.last_t;
             } else {
                 Tracer<double> tr{s, *ubo, stats ? &local : nullptr};
                 V3<double> r = tr.pixel(x, y, W, H);
                 o[0] = r.x; o[1] = r.y; o[2] = r.z;
+                o[3] = tr.last_t;
             }
-            o[3] = 0.0;
         }
         if (stats) {
 #pragma omp critical
@@ -1516,6 +1519,85 @@ int orc_render(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, uint32_t x0,
     }
     if (stats) *stats = total;
     return 0;
+}
+
+// ------------------------------------------------------------ display image
+// RayTracing.rgen:132-166 after accumulation: tone mapping (shaders/lib/Tonemapping.glsl),
+// the HEAT / DISTANCE mappings (shaders/lib/Heatmap.glsl, rgen:151-163) and the
+// 8-bit UNORM store.  fp32 like the shaders; this file builds with contraction off.
+static float gamma22(float c) { return std::pow(c, 1.0f / 2.2f); }  // LINEARtoSRGB, Tonemapping.glsl:9-11
+
+static float uncharted2_impl(float x) {  // Tonemapping.glsl:15-23
+    const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+    float num = x * (A * x + C * B) + D * E;
+    float den = x * (A * x + B) + D * F;
+    return num / den - E / F;
+}
+
+float orc_tone_map(float c, uint32_t mode) {
+    if (mode == 0) return gamma22(c / (c + 1.0f));  // defaultToneMap, :49-52
+    if (mode == 1) {                                // toneMapUncharted, :25-30
+        float white = 1.0f / uncharted2_impl(11.2f);
+        return gamma22(uncharted2_impl(c * 2.0f) * white);
+    }
+    if (mode == 2) {  // toneMapHejlRichard, :34-37
+        float x = std::max(0.0f, c - 0.004f);
+        return (x * (6.2f * x + .5f)) / (x * (6.2f * x + 1.7f) + 0.06f);
+    }
+    if (mode == 3) {  // toneMapACES, :41-47
+        float v = (c * (2.51f * c + 0.03f)) / (c * (2.43f * c + 0.59f) + 0.14f);
+        return gamma22(std::min(std::max(v, 0.0f), 1.0f));
+    }
+    return gamma22(c);
+}
+
+static float smooth_step(float a, float b, float x) {  // GLSL smoothstep
+    float t = std::min(std::max((x - a) / (b - a), 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+void orc_heatmap(float t, float out[3]) {  // Heatmap.glsl:4-33
+    static const float c[10][3] = {{0, 2, 91},    {0, 108, 251}, {0, 221, 221}, {51, 221, 0}, {255, 252, 0},
+                                   {255, 180, 0}, {255, 104, 0}, {226, 22, 0},  {191, 0, 83}, {145, 0, 65}};
+    float s = t * 10.0f;
+    int cur = int(s) <= 9 ? int(s) : 9;
+    int prv = cur >= 1 ? cur - 1 : 0;
+    int nxt = cur < 9 ? cur + 1 : 9;
+    const float blur = 0.8f;
+    float wc = smooth_step(float(cur) - blur, float(cur) + blur, s) *
+               (1.0f - smooth_step(float(cur + 1) - blur, float(cur + 1) + blur, s));
+    float wp = 1.0f - smooth_step(float(cur) - blur, float(cur) + blur, s);
+    float wn = smooth_step(float(cur + 1) - blur, float(cur + 1) + blur, s);
+    for (int k = 0; k < 3; k++) {
+        float r = wc * (c[cur][k] / 255.0f) + wp * (c[prv][k] / 255.0f) + wn * (c[nxt][k] / 255.0f);
+        out[k] = std::min(std::max(r, 0.0f), 1.0f);
+    }
+}
+
+static uint32_t to_unorm8(float v) {  // clamp, scale by 255, round to nearest even
+    if (!(v > 0.0f)) return 0u;
+    if (v > 1.0f) v = 1.0f;
+    return uint32_t(std::nearbyint(v * 255.0f));
+}
+
+void orc_output_image(const float* accum, uint32_t n, const float* last_t, const uint64_t* ticks, const rt_ubo* ubo,
+                      int bgra, uint32_t* out) {
+    const float total = float(ubo->total_number_of_samples);
+    for (uint32_t i = 0; i < n; i++) {
+        float col[3];
+        for (int k = 0; k < 3; k++) col[k] = orc_tone_map(accum[4 * size_t(i) + k] / total, ubo->tone_mapping_mode);
+        if (ubo->mapping == RT_MAPPING_HEAT) {
+            float scale = 1000000.0f * ubo->heatmap_scale * ubo->heatmap_scale;
+            float t = ticks ? float(ticks[i]) / scale : 0.0f;
+            orc_heatmap(std::min(std::max(t, 0.0f), 1.0f), col);
+        } else if (ubo->mapping == RT_MAPPING_DISTANCE) {
+            float t = last_t ? last_t[i] : 0.0f;
+            float g = std::min((ubo->heatmap_scale - std::max(t, 0.001f)) / ubo->heatmap_scale, 1.0f);
+            col[0] = col[1] = col[2] = g;
+        }
+        uint32_t r = to_unorm8(col[0]), g = to_unorm8(col[1]), b = to_unorm8(col[2]);
+        out[i] = bgra ? (b | g << 8 | r << 16 | 255u << 24) : (r | g << 8 | b << 16 | 255u << 24);
+    }
 }
 
 }  // extern "C"

@@ -118,7 +118,7 @@ void orc_scene_set_eval_textures(void* scene, const rt_texture* tex, uint32_t n)
 int orc_trace_rays(void* scene, const float* rays, uint32_t n, uint32_t ray_flags, uint32_t seed,
                    rt_hit* hits, double* hits_tuv, int brute, int threads);
 /* Renders the n pixels xy[2i], xy[2i+1] of a width x height launch; out receives
- * n x 4 doubles (summed radiance, alpha 0). */
+ * n x 4 doubles: summed radiance and the payload's t after the pixel's last trace. */
 int orc_render_pixels(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
                       const uint32_t* xy, uint32_t n, int precision, int threads, double* out,
                       orc_stats* stats);
@@ -131,6 +131,15 @@ int orc_trace_primary(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t h
 int orc_render(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
                uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, int precision,
                int threads, double* accum, orc_stats* stats);
+
+/* --- display image (RayTracing.rgen:132-166), fp32 --- */
+float orc_tone_map(float c, uint32_t mode);
+void orc_heatmap(float t, float out[3]);
+/* n pixels of an rgba32f accumulation image -> packed 8-bit pixels (R,G,B,A bytes,
+ * or B,G,R,A when bgra).  last_t: payload t per pixel (DISTANCE); ticks: shader
+ * clock ticks per pixel (HEAT); either may be NULL (read as 0). */
+void orc_output_image(const float* accum, uint32_t n, const float* last_t, const uint64_t* ticks, const rt_ubo* ubo,
+                      int bgra, uint32_t* out);
 
 #ifdef __cplusplus
 }

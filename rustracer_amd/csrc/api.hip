@@ -106,6 +106,7 @@ struct rt_ctx {
     int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
     bool use_graph = false;  // RT_OPTION_GRAPH
     uint32_t refill_ext = 40, refill_sh = 1;  // RT_OPTION_EXTEND_REFILL / RT_OPTION_SHADOW_REFILL
+    bool out_bgra = false;   // RT_OPTION_OUTPUT_BGRA
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
 };
@@ -236,6 +237,12 @@ int make_frame(rt_ctx* ctx, const TlasEntry& tl, const rt_bindings* b, float* ac
     fa->mapping = u.mapping;
     fa->frame_count = u.frame_count;
     fa->fully_opaque = u.fully_opaque;
+    fa->o.out = nullptr;
+    fa->o.heat_scale = u.heatmap_scale;
+    fa->o.tone_mode = u.tone_mapping_mode;
+    fa->o.mapping = u.mapping;
+    fa->o.total = u.total_number_of_samples;
+    fa->o.bgra = 0u;
     return RT_OK;
 }
 
@@ -686,7 +693,12 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     if (width == 0 || height == 0 || uint64_t(width) * height > (1ull << 31))
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
     if (mode != RT_MODE_MEGAKERNEL && mode != RT_MODE_WAVEFRONT) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown mode");
-    if (out_rgba8) return fail(ctx, RT_ERR_UNSUPPORTED, "the rgba8 display image is not produced by this build");
+    if (out_rgba8 && (reinterpret_cast<uintptr_t>(out_rgba8) & 3u))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out_rgba8 must be 4-byte aligned");
+    if (out_rgba8 && b->ubo->mapping == RT_MAPPING_HEAT && mode == RT_MODE_WAVEFRONT)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "the HEAT mapping (per-pixel shader clock) needs the megakernel mode");
+    if (out_rgba8 && b->ubo->total_number_of_samples == 0)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "the display image needs total_number_of_samples > 0");
     auto it = ctx->tlas.find(tlas);
     if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
     if (b->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, "HasSky (skybox miss) is not supported");
@@ -695,6 +707,8 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     if (rc) return rc;
     FrameArgs fa;
     make_frame(ctx, it->second, b, accum, width, height, &fa);
+    fa.o.out = reinterpret_cast<uint32_t*>(out_rgba8);
+    fa.o.bgra = ctx->out_bgra ? 1u : 0u;
     const bool any_hit = !fa.fully_opaque;
     hipStream_t st = ctx->stream;
     if (stats) {
@@ -980,6 +994,11 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (option == RT_OPTION_GRAPH) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_GRAPH takes 0 or 1");
         ctx->use_graph = value != 0;
+        return RT_OK;
+    }
+    if (option == RT_OPTION_OUTPUT_BGRA) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_OUTPUT_BGRA takes 0 or 1");
+        ctx->out_bgra = value != 0;
         return RT_OK;
     }
     if (option == RT_OPTION_FLATTEN_IDENTITY_INSTANCES) {

@@ -458,6 +458,7 @@ struct PathState {
     double volume_dis;  // Ray.volume_dis
     double tfar;
     uint32_t wa, wb, lcg, sample, bounce;
+    float last_t;       // payload t after the last trace (Ray.t, read by the DISTANCE mapping)
 };
 
 __device__ inline void init_path(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p) {
@@ -468,6 +469,7 @@ __device__ inline void init_path(const FrameArgs& fa, uint32_t x, uint32_t y, Pa
     p.hit_point = d3(0.0);
     p.volume_dis = -1.0;
     p.sample = 0u;
+    p.last_t = 0.f;
 }
 
 // Start of one sample, rgen:52-80.
@@ -478,6 +480,7 @@ __device__ inline void begin_sample(const FrameArgs& fa, uint32_t x, uint32_t y,
     p.thr = d3(1.0);
     p.volume_dis = -1.0;
     p.bounce = 0u;
+    p.last_t = 0.f;  // Ray.t = 0, rgen:79
 }
 
 // What one traced bounce contributes, and the shadow ray it needs.
@@ -544,6 +547,7 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
     p.hit_point = ray.hit_point;
     p.volume_dis = ray.volume_dis;
     p.wb = ray.rng.w;
+    p.last_t = float(ray.t);
     r.cont = false;
     if (p.bounce + 1u >= fa.bounces) return;
     double rr = 1.0;
@@ -590,10 +594,11 @@ __device__ inline bool shadow_unoccluded(const FrameArgs& fa, uint32_t x, uint32
                                        sh, pc.trs);
 }
 
-// RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance.
+// RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance
+// and the payload's t after the last trace.
 template <bool kAnyHit, bool kStats>
 __device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, uint32_t stride,
-                                  PathCounters& pc) {
+                                  PathCounters& pc, float& last_t) {
     PathState p;
     init_path(fa, x, y, p);
     const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
@@ -615,6 +620,7 @@ __device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, u
             if (!r.cont) break;
         }
     }
+    last_t = p.last_t;
     return p.rad;
 }
 

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "output.h"
 #include "shade.h"
 #include "traverse.h"
 
@@ -21,6 +22,7 @@ struct FrameArgs {
     float mvi[16], pin[16];        // ModelViewInverse, ProjectionInverse (column-major)
     float aperture, focus_distance, ortho, exposure;
     uint32_t samples, bounces, antialiasing, mapping, frame_count, fully_opaque;
+    OutputArgs o;                  // display image (o.out NULL: none)
 };
 
 enum Counter : int {

@@ -34,10 +34,16 @@ __global__ __launch_bounds__(kMegaBlock) void k_megakernel(FrameArgs fa) {
     const uint32_t y = ty * 8 + (lane >> 3);
     PathCounters pc;
     if (x < fa.width && y < fa.height) {
-        D3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, kMegaBlock, pc);
+        const bool heat = fa.o.out && fa.o.mapping == RT_MAPPING_HEAT;
+        const uint64_t t0 = heat ? uint64_t(clock64()) : 0u;  // clockARB(), rgen:27
+        float last_t;
+        D3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, kMegaBlock, pc, last_t);
         float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
         float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
-        *acc = make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
+        const float4 sum =
+            make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
+        *acc = sum;
+        if (fa.o.out) write_output(fa.o, fa.width, x, y, sum, last_t, heat ? uint64_t(clock64()) - t0 : 0u);
     }
     if (kStats) flush_counters(fa.counters, pc);
 }

@@ -35,6 +35,7 @@ struct WfArrays {
     uint32_t *wa, *wb, *lcg, *sample, *bounce;
     uint32_t *h_inst, *h_prim, *h_rec;
     float *h_t, *h_u, *h_v;
+    float* last_t;     // payload t after the last trace (DISTANCE mapping only)
     uint32_t *flags;   // kFlagShadow | kFlagCont
     uint32_t *unocc;   // shadow result
     float4 *sq_o, *sq_d;  // shadow queue: origin + tmax, direction + slot bits
@@ -158,10 +159,12 @@ struct ManyFrames {
     __device__ unsigned long long* counters() const { return cnt; }
 };
 
-__device__ inline void write_accum(const FrameArgs& fa, uint32_t x, uint32_t y, D3 rad) {
+__device__ inline void write_accum(const FrameArgs& fa, uint32_t x, uint32_t y, D3 rad, float last_t) {
     float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
     float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
-    *acc = make_float4(prev.x + float(rad.x), prev.y + float(rad.y), prev.z + float(rad.z), 0.f);
+    const float4 sum = make_float4(prev.x + float(rad.x), prev.y + float(rad.y), prev.z + float(rad.z), 0.f);
+    *acc = sum;
+    if (fa.o.out) write_output(fa.o, fa.width, x, y, sum, last_t, 0u);
 }
 
 template <class Frames>
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, co
             h.v = a.h_v[s];
             shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
             store_full(a, s, p);
+            if (fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE) a.last_t[s] = p.last_t;
             a.pwx[s] = r.pend_with.x; a.pwy[s] = r.pend_with.y; a.pwz[s] = r.pend_with.z;
             a.pox[s] = r.pend_without.x; a.poy[s] = r.pend_without.y; a.poz[s] = r.pend_without.z;
             a.flags[s] = (r.shadow ? kFlagShadow : 0u) | (r.cont ? kFlagCont : 0u);
@@ -374,7 +378,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Frames fr, WfArrays a, 
                     a.bounce[s] = 0u;
                     alive = true;
                 } else {
-                    write_accum(fa, x, y, rad);
+                    const bool dist = fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE;
+                    write_accum(fa, x, y, rad, dist ? a.last_t[s] : 0.f);
                 }
             }
         }
@@ -402,6 +407,7 @@ size_t layout(WfArrays& a, char* base, uint32_t n) {
     total += carve(p, a.h_t, n);
     total += carve(p, a.h_u, n);
     total += carve(p, a.h_v, n);
+    total += carve(p, a.last_t, n);
     total += carve(p, a.sq_o, n);
     total += carve(p, a.sq_d, n);
     total += carve(p, a.counts, 8);

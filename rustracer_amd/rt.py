@@ -91,6 +91,8 @@ OPTION_TRAVERSAL_VARIANT = 3
 OPTION_GRAPH = 4
 OPTION_EXTEND_REFILL = 5
 OPTION_SHADOW_REFILL = 6
+OPTION_OUTPUT_BGRA = 7
+TONE_MAP_DEFAULT, TONE_MAP_UNCHARTED, TONE_MAP_HEJL_RICHARD, TONE_MAP_ACES, TONE_MAP_SRGB = 0, 1, 2, 3, 4
 NUM_TRAVERSAL_VARIANTS = 4
 
 
@@ -339,13 +341,18 @@ def new_accumulation(width, height, device="cuda"):
     return torch.zeros((height, width, 4), dtype=torch.float32, device=device)
 
 
-def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERNEL, stats=False):
+def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERNEL, stats=False, out=None):
     """record_raytracing_commands + trace_rays (main.rs:244-270): runs the pipeline for
-    a width x height launch into `accum` (torch float32 (h, w, 4) on the device).
-    Returns a dict of counters when stats=True (the counting kernel variant)."""
+    a width x height launch into `accum` (torch float32 (h, w, 4) on the device) and,
+    when `out` (torch uint8 (h, w, 4) on the device) is given, the tone-mapped display
+    image (RayTracing.rgen:132-166).  Returns a dict of counters when stats=True (the
+    counting kernel variant)."""
     b = buffers.bindings(ubo)
     st = TraceStats() if stats else None
-    ctx.check(ctx.lib.rt_trace_rays(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(accum.data_ptr()), None,
+    if out is not None and (out.dtype != _torch().uint8 or not out.is_contiguous() or out.numel() != width * height * 4):
+        raise ValueError("out must be a contiguous uint8 tensor of width * height * 4 elements")
+    ctx.check(ctx.lib.rt_trace_rays(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(accum.data_ptr()),
+                                    C.c_void_p(out.data_ptr()) if out is not None else None,
                                     width, height, mode, C.byref(st) if st is not None else None))
     if st is None:
         return None
