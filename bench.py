@@ -305,6 +305,9 @@ def main():
                                f"{W}x{H}, {int(sc.ubo['number_of_samples'])} spp, depth "
                                f"{int(sc.ubo['number_of_bounces'])}, 3 point lights",
                    "mode": best, "rays_per_frame": st["rays"],
+                   "counts": {k: int(st[k]) for k in ("primary_rays", "extension_rays", "shadow_rays", "node_visits",
+                                                       "shadow_node_visits", "tri_tests", "shadow_tri_tests",
+                                                       "shade_events")},
                    "modes": {k: round(v["mrays"], 2) for k, v in results.items()},
                    "build_ms": round(min(build_ms), 2), "scene_gen_s": round(gen_s, 2), "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

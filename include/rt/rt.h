@@ -233,7 +233,20 @@ enum {
     /* Byte order of rt_trace_rays' display image: 0 (default) R, G, B, A; 1: B, G,
      * R, A, the reference's storage image (the swapchain's B8G8R8A8_UNORM,
      * crates/libs/vulkan/src/swapchain.rs:39-48, crates/libs/app/src/lib.rs:305-312). */
-    RT_OPTION_OUTPUT_BGRA = 7
+    RT_OPTION_OUTPUT_BGRA = 7,
+    /* The wavefront trace deals the frame's 8x8 pixel tiles (a batch: its frames) to
+     * this many slot sets (1 .. 4), each running its own chain of queue kernels on
+     * its own stream, so one set's kernels fill the CUs that another set's kernel
+     * tails leave idle.  Same images.  RT_OPTION_STAGE_TIMING traces one set.
+     * Default 2. */
+    RT_OPTION_WAVEFRONT_STREAMS = 8,
+    /* Once fewer paths than this are alive in a slot set, the wavefront trace runs
+     * each of them to the end of its pixel's samples in one thread of one launch
+     * (the megakernel's loop from the path's state) instead of further queue
+     * iterations, whose launches would last as long as their slowest ray while most
+     * of the GPU idles.  0: never.  Same images.  Ignored while
+     * RT_OPTION_STAGE_TIMING is on.  Default 65536. */
+    RT_OPTION_WAVEFRONT_FINISH = 9
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 
@@ -243,8 +256,10 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * as extend_ms.  Synchronises the context stream. */
 /* Autotuner: traces the frame (the bindings' UBO with at most 2 samples per pixel)
  * in wavefront mode and tunes, one after another, the traversal variant (stack
- * depth, register target), the extend and shadow refill thresholds and graph
- * replay, keeping the fastest settings on this context (the RT_OPTION_* values).
+ * depth, register target), the number of concurrent slot sets, the live-path
+ * count that hands the tail to one finishing launch, the extend and shadow refill
+ * thresholds and graph replay, keeping the fastest settings on this context (the
+ * RT_OPTION_* values).
  * A variant whose stack ring gives up on a ray of this frame (restart budget) is
  * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */
 typedef struct rt_tuning {
@@ -252,6 +267,8 @@ typedef struct rt_tuning {
     uint32_t graph;
     uint32_t extend_refill;
     uint32_t shadow_refill;
+    uint32_t wavefront_streams;
+    uint32_t wavefront_finish;
     double frame_ms;  /* probe frame time with the chosen settings */
 } rt_tuning;
 int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width, uint32_t height, rt_tuning* out);

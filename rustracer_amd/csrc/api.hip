@@ -107,6 +107,8 @@ struct rt_ctx {
     bool use_graph = false;  // RT_OPTION_GRAPH
     uint32_t refill_ext = 40, refill_sh = 1;  // RT_OPTION_EXTEND_REFILL / RT_OPTION_SHADOW_REFILL
     bool out_bgra = false;   // RT_OPTION_OUTPUT_BGRA
+    int streams = 2;         // RT_OPTION_WAVEFRONT_STREAMS
+    uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
 };
@@ -727,6 +729,8 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         std::string err;
         wavefront_set_graph(ctx->wavefront, ctx->use_graph);
         wavefront_set_refill(ctx->wavefront, ctx->refill_ext, ctx->refill_sh);
+        wavefront_set_streams(ctx->wavefront, ctx->streams);
+        wavefront_set_finish(ctx->wavefront, ctx->finish);
         rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, ctx->variant, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
@@ -806,6 +810,8 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
         std::string err;
         wavefront_set_graph(ctx->wavefront, ctx->use_graph);
         wavefront_set_refill(ctx->wavefront, ctx->refill_ext, ctx->refill_sh);
+        wavefront_set_streams(ctx->wavefront, ctx->streams);
+        wavefront_set_finish(ctx->wavefront, ctx->finish);
         rc = launch_wavefront_batch(ctx->wavefront, frames.data(), count, st, any_hit, stats != nullptr, ctx->variant, tm,
                                     &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
@@ -897,6 +903,8 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     const int keep_variant = ctx->variant;
     const bool keep_graph = ctx->use_graph;
     const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
+    const int keep_streams = ctx->streams;
+    const uint32_t keep_finish = ctx->finish;
     // one probe: frame time of the current settings, or < 0 when a traversal gave up
     auto probe = [&](double& ms_out) -> int {
         int r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
@@ -929,7 +937,33 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     }
     if (rc == RT_OK && best_v < 0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
     if (rc == RT_OK) ctx->variant = best_v;
-    // 2, 3. refill thresholds of the extend and shadow kernels
+    // 2. concurrent slot sets
+    if (rc == RT_OK) {
+        int chosen = ctx->streams;
+        for (int n = 1; n <= wavefront_max_streams() && rc == RT_OK; n++) {
+            if (n == chosen) continue;
+            ctx->streams = n;
+            double ms = -1.0;
+            rc = probe(ms);
+            if (rc == RT_OK && consider(ms)) { best = ms; chosen = n; }
+        }
+        ctx->streams = chosen;
+    }
+    // 3. live-path count that hands the tail to the finishing launch
+    if (rc == RT_OK) {
+        const uint32_t finish[] = {0u, 1u << 14, 1u << 16, 1u << 18};
+        uint32_t chosen = ctx->finish;
+        for (uint32_t f : finish) {
+            if (f == chosen) continue;
+            ctx->finish = f;
+            double ms = -1.0;
+            rc = probe(ms);
+            if (rc) break;
+            if (consider(ms)) { best = ms; chosen = f; }
+        }
+        ctx->finish = chosen;
+    }
+    // 4, 5. refill thresholds of the extend and shadow kernels
     const uint32_t refills[] = {1u, 16u, 32u, 48u, 64u};
     for (int which = 0; which < 2 && rc == RT_OK; which++) {
         uint32_t& knob = which == 0 ? ctx->refill_ext : ctx->refill_sh;
@@ -944,7 +978,7 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         }
         knob = chosen;
     }
-    // 4. graph replay
+    // 6. graph replay
     if (rc == RT_OK) {
         ctx->use_graph = true;
         double ms = -1.0;
@@ -960,6 +994,8 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         ctx->use_graph = keep_graph;
         ctx->refill_ext = keep_re;
         ctx->refill_sh = keep_rs;
+        ctx->streams = keep_streams;
+        ctx->finish = keep_finish;
         return rc;
     }
     if (out) {
@@ -967,6 +1003,8 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         out->graph = ctx->use_graph ? 1u : 0u;
         out->extend_refill = ctx->refill_ext;
         out->shadow_refill = ctx->refill_sh;
+        out->wavefront_streams = uint32_t(ctx->streams);
+        out->wavefront_finish = ctx->finish;
         out->frame_ms = best;
     }
     return RT_OK;
@@ -994,6 +1032,17 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (option == RT_OPTION_GRAPH) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_GRAPH takes 0 or 1");
         ctx->use_graph = value != 0;
+        return RT_OK;
+    }
+    if (option == RT_OPTION_WAVEFRONT_FINISH) {
+        if (value > 0xFFFFFFFFull) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_WAVEFRONT_FINISH is a 32-bit count");
+        ctx->finish = uint32_t(value);
+        return RT_OK;
+    }
+    if (option == RT_OPTION_WAVEFRONT_STREAMS) {
+        if (value < 1 || value > uint64_t(wavefront_max_streams()))
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_WAVEFRONT_STREAMS takes 1 .. 4");
+        ctx->streams = int(value);
         return RT_OK;
     }
     if (option == RT_OPTION_OUTPUT_BGRA) {

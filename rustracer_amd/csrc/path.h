@@ -594,16 +594,17 @@ __device__ inline bool shadow_unoccluded(const FrameArgs& fa, uint32_t x, uint32
                                        sh, pc.trs);
 }
 
-// RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance
-// and the payload's t after the last trace.
+// The sample loop of RayTracing.rgen main (:49-130) for one pixel from path state p:
+// with `resume`, p's current sample has begun and its ray p.o, p.d at bounce
+// p.bounce is the next to trace (the state the wavefront queues hold between
+// iterations); otherwise the loop begins sample p.sample.  Runs to the last sample.
 template <bool kAnyHit, bool kStats>
-__device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, uint32_t stride,
-                                  PathCounters& pc, float& last_t) {
-    PathState p;
-    init_path(fa, x, y, p);
+__device__ inline void run_samples(const FrameArgs& fa, uint32_t x, uint32_t y, PathState& p, bool resume,
+                                   uint32_t* stack, uint32_t stride, PathCounters& pc) {
     const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
     for (; p.sample < fa.samples; p.sample++) {
-        begin_sample(fa, x, y, p);
+        if (!resume) begin_sample(fa, x, y, p);
+        resume = false;
         if (fa.bounces == 0u) continue;
         for (;;) {
             if (kStats) { if (p.bounce == 0u) pc.primary++; else pc.extension++; }
@@ -620,6 +621,16 @@ __device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, u
             if (!r.cont) break;
         }
     }
+}
+
+// RayTracing.rgen main for one pixel (megakernel form); returns the summed radiance
+// and the payload's t after the last trace.
+template <bool kAnyHit, bool kStats>
+__device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, uint32_t stride,
+                                  PathCounters& pc, float& last_t) {
+    PathState p;
+    init_path(fa, x, y, p);
+    run_samples<kAnyHit, kStats>(fa, x, y, p, false, stack, stride, pc);
     last_t = p.last_t;
     return p.rad;
 }

@@ -25,6 +25,13 @@ void wavefront_set_graph(WavefrontState* s, bool on);
 // Busy-lane thresholds below which a wave of the extend / shadow kernel takes new
 // rays (1 .. 64).
 void wavefront_set_refill(WavefrontState* s, uint32_t extend, uint32_t shadow);
+// Number of slot sets (1 .. wavefront_max_streams()) traced concurrently on their own
+// streams; stage timing uses one.
+void wavefront_set_streams(WavefrontState* s, int n);
+int wavefront_max_streams();
+// Live-path count below which the rest of a slot set's paths are run to completion by
+// one launch of megakernel-style threads (0: never).  Same images.
+void wavefront_set_finish(WavefrontState* s, uint32_t live_paths);
 // Renders `count` frames (host array) in one set of queue kernels: their slots are
 // laid end to end and every kernel serves all of them.  Each frame's image is
 // the one launch_wavefront gives it alone.  Frames with no samples or bounces

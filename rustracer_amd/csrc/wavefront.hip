@@ -26,6 +26,7 @@ namespace {
 
 constexpr int kWfBlock = 256;
 constexpr uint32_t kCheckEvery = 8;
+constexpr int kMaxStreams = 4;  // slot sets traced concurrently (RT_OPTION_WAVEFRONT_STREAMS)
 
 struct WfArrays {
     double *ox, *oy, *oz, *dx, *dy, *dz;  // ray
@@ -42,16 +43,18 @@ struct WfArrays {
     uint32_t* sq_wb;
     uint32_t *q0, *q1;
     uint32_t* counts;  // [0], [1]: path queues (ping-pong), [2]: shadow queue,
-                       // [3]: extend fetch cursor, [4]: shadow fetch cursor
+                       // [3]: extend fetch cursor, [4]: shadow fetch cursor (8 per slot set)
     uint32_t refill_ext, refill_sh;  // busy lanes below which a wave takes new rays (1..64)
 };
 
 enum : uint32_t { kFlagShadow = 1u, kFlagCont = 2u };
 
-__device__ inline void slot_pixel(const FrameArgs& fa, uint32_t s, uint32_t& x, uint32_t& y) {
-    // 8x8 pixel tiles: consecutive slots are neighbouring pixels
+// Slot -> pixel: slots come in 8x8 pixel tiles (consecutive slots are neighbouring
+// pixels); a set of slots traced on one stream holds every `step`-th tile from `first`.
+__device__ inline void slot_pixel(const FrameArgs& fa, uint32_t s, uint32_t step, uint32_t first, uint32_t& x,
+                                  uint32_t& y) {
     uint32_t tiles_x = (fa.width + 7) / 8;
-    uint32_t tile = s >> 6, within = s & 63u;
+    uint32_t tile = (s >> 6) * step + first, within = s & 63u;
     x = (tile % tiles_x) * 8 + (within & 7u);
     y = (tile / tiles_x) * 8 + (within >> 3);
 }
@@ -135,8 +138,9 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* c) {
 // [base[i], base[i + 1]) (rt_trace_rays_batch).
 struct OneFrame {
     FrameArgs fa;
+    uint32_t tile_step = 1, tile_first = 0;  // the frame's tiles this slot set holds
     __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
-        slot_pixel(fa, s, x, y);
+        slot_pixel(fa, s, tile_step, tile_first, x, y);
         return fa;
     }
     __device__ unsigned long long* counters() const { return fa.counters; }
@@ -153,7 +157,7 @@ struct ManyFrames {
             if (base[m] <= s) lo = m; else hi = m;
         }
         const FrameArgs& fa = fas[lo];
-        slot_pixel(fa, s - base[lo], x, y);
+        slot_pixel(fa, s - base[lo], 1u, 0u, x, y);
         return fa;
     }
     __device__ unsigned long long* counters() const { return cnt; }
@@ -388,6 +392,30 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Frames fr, WfArrays a, 
     }
 }
 
+// The tail of a trace: once few paths are alive, each iteration's launches last as
+// long as their slowest ray while most of the GPU idles.  This kernel takes the live
+// paths of queue 0 and runs each one, in one thread, through the rest of its pixel's
+// samples (the megakernel's loop from the slot's state), then writes the pixel.
+constexpr int kFinishBlock = 128;
+template <class Frames, bool kAnyHit, bool kStats>
+__global__ __launch_bounds__(kFinishBlock) void k_wf_finish(Frames fr, WfArrays a, const uint32_t* q,
+                                                            const uint32_t* n_in) {
+    __shared__ uint32_t s_stack[kStackDepth * kFinishBlock];
+    const uint32_t n = *n_in;
+    PathCounters pc;
+    for (uint32_t i = blockIdx.x * kFinishBlock + threadIdx.x; i < n; i += gridDim.x * kFinishBlock) {
+        const uint32_t s = q[i];
+        uint32_t x, y;
+        const FrameArgs& fa = fr.at(s, x, y);
+        PathState p;
+        load_full(a, s, p);
+        p.last_t = 0.f;
+        run_samples<kAnyHit, kStats>(fa, x, y, p, true, s_stack + threadIdx.x, kFinishBlock, pc);
+        write_accum(fa, x, y, p.rad, p.last_t);
+    }
+    if (kStats) flush_counters(fr.counters(), pc);
+}
+
 template <class T> size_t carve(char*& p, T*& out, size_t n) {
     size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
     out = reinterpret_cast<T*>(p);
@@ -410,8 +438,28 @@ size_t layout(WfArrays& a, char* base, uint32_t n) {
     total += carve(p, a.last_t, n);
     total += carve(p, a.sq_o, n);
     total += carve(p, a.sq_d, n);
-    total += carve(p, a.counts, 8);
+    total += carve(p, a.counts, 8 * kMaxStreams);
     return total;
+}
+
+// The arrays of the slots [base, base + n) of a slot set with its own counters: the
+// per-slot arrays and both queues are offset, so the set's kernels index them from 0.
+WfArrays slice(const WfArrays& a, uint32_t base, uint32_t* counts) {
+    WfArrays r = a;
+    double** d[] = {&r.ox, &r.oy, &r.oz, &r.dx, &r.dy, &r.dz, &r.tx, &r.ty, &r.tz, &r.rx, &r.ry, &r.rz,
+                    &r.hx, &r.hy, &r.hz, &r.vol, &r.tfar, &r.pwx, &r.pwy, &r.pwz, &r.pox, &r.poy, &r.poz};
+    for (double** x : d) *x += base;
+    uint32_t** u[] = {&r.wa, &r.wb, &r.lcg, &r.sample, &r.bounce, &r.h_inst, &r.h_prim, &r.h_rec, &r.flags,
+                      &r.unocc, &r.sq_wb, &r.q0, &r.q1};
+    for (uint32_t** x : u) *x += base;
+    r.h_t += base;
+    r.h_u += base;
+    r.h_v += base;
+    r.last_t += base;
+    r.sq_o += base;
+    r.sq_d += base;
+    r.counts = counts;
+    return r;
 }
 
 __global__ void k_zero_counts(uint32_t* c) {
@@ -420,6 +468,11 @@ __global__ void k_zero_counts(uint32_t* c) {
 
 template <class Frames> struct Kernels {
     using ExtendFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
+    using FinishFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*);
+    static FinishFn finish(bool ah, bool st) {
+        return ah ? (st ? k_wf_finish<Frames, true, true> : k_wf_finish<Frames, true, false>)
+                  : (st ? k_wf_finish<Frames, false, true> : k_wf_finish<Frames, false, false>);
+    }
     using ShadowFn = void (*)(Frames, WfArrays);
     template <int V> static ExtendFn extend_v(bool ah, bool st) {
         constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
@@ -462,8 +515,13 @@ struct WavefrontState {
     size_t batch_cap = 0;
     bool use_graph = false;     // replay captured chunks (RT_OPTION_GRAPH)
     uint32_t refill_ext = 40, refill_sh = 1;
+    int streams = 2;            // slot sets traced concurrently (RT_OPTION_WAVEFRONT_STREAMS)
+    uint32_t finish_below = 0;  // live paths below which k_wf_finish ends a set (RT_OPTION_WAVEFRONT_FINISH)
     hipStream_t cap_stream = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec[kMaxStreams] = {};
+    hipStream_t set_stream[kMaxStreams] = {};  // streams of slot sets 1.. (set 0 uses the caller's)
+    hipEvent_t fork = nullptr, join[kMaxStreams] = {}, counted[kMaxStreams][2] = {};
+    uint32_t* host_live = nullptr;  // pinned: [set][parity] live-path counts read back
     int cus = 0;
     std::unordered_map<const void*, uint32_t> grids;  // persistent grid per kernel (occupancy x CUs)
     // grid of a persistent launch of kernel `f`: as many blocks as can be resident
@@ -489,11 +547,25 @@ void wavefront_set_refill(WavefrontState* s, uint32_t extend, uint32_t shadow) {
     s->a.refill_sh = s->refill_sh;
 }
 
+void wavefront_set_streams(WavefrontState* s, int n) { s->streams = std::min(std::max(n, 1), kMaxStreams); }
+
+void wavefront_set_finish(WavefrontState* s, uint32_t live_paths) { s->finish_below = live_paths; }
+
+int wavefront_max_streams() { return kMaxStreams; }
+
 void wavefront_destroy(WavefrontState* s) {
     if (!s) return;
     if (s->mem) (void)hipFree(s->mem);
     if (s->batch_mem) (void)hipFree(s->batch_mem);
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->host_live) (void)hipHostFree(s->host_live);
+    for (int g = 0; g < kMaxStreams; g++) {
+        if (s->exec[g]) (void)hipGraphExecDestroy(s->exec[g]);
+        if (s->set_stream[g]) (void)hipStreamDestroy(s->set_stream[g]);
+        if (s->join[g]) (void)hipEventDestroy(s->join[g]);
+        for (int k = 0; k < 2; k++)
+            if (s->counted[g][k]) (void)hipEventDestroy(s->counted[g][k]);
+    }
+    if (s->fork) (void)hipEventDestroy(s->fork);
     if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
     delete s;
 }
@@ -520,10 +592,36 @@ int ensure_slots(WavefrontState* ws, uint32_t nslots, std::string* err) {
     return 0;
 }
 
-// The queue loop over `nslots` slots whose frames `fr` gives; every path ends within
-// max_samples * max_bounces bounces.
+// Streams, events and pinned read-back words for `n` slot sets.
+int ensure_streams(WavefrontState* ws, int n, std::string* err) {
+    hipError_t e = hipSuccess;
+    if (!ws->host_live) e = hipHostMalloc((void**)&ws->host_live, sizeof(uint32_t) * 2 * kMaxStreams, hipHostMallocDefault);
+    if (e == hipSuccess && !ws->fork) e = hipEventCreateWithFlags(&ws->fork, hipEventDisableTiming);
+    for (int g = 0; g < n && e == hipSuccess; g++) {
+        if (g > 0 && !ws->set_stream[g]) e = hipStreamCreateWithFlags(&ws->set_stream[g], hipStreamNonBlocking);
+        if (e == hipSuccess && !ws->join[g]) e = hipEventCreateWithFlags(&ws->join[g], hipEventDisableTiming);
+        for (int k = 0; k < 2 && e == hipSuccess; k++)
+            if (!ws->counted[g][k]) e = hipEventCreateWithFlags(&ws->counted[g][k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) *err = "cannot create the slot-set streams";
+    return int(e);
+}
+
+// One set of slots traced by its own chain of queue kernels.
+template <class Frames> struct SlotSet {
+    Frames fr;
+    WfArrays a;       // sliced to the set's slots, with its own counters
+    uint32_t nslots;
+};
+
+// The queue loop.  Each slot set runs init, then chunks of kCheckEvery iterations
+// (extend, shade, shadow, resolve) on its own stream, so one set's kernels fill the
+// CUs another set's kernel tails leave idle; every set's stream waits for `st` at
+// the start and `st` waits for every set at the end.  The host reads each set's
+// live-path count one chunk behind the chunk it enqueues, so the device never waits
+// for the host.  Every path ends within max_steps bounces.
 template <class Frames>
-int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_t max_steps, hipStream_t st,
+int run_wavefront(WavefrontState* ws, const std::vector<SlotSet<Frames>>& sets, uint64_t max_steps, hipStream_t st,
                   bool any_hit, bool stats, int variant, StageTimer* tm, std::string* err) {
     if (ws->cus == 0) {
         int dev = 0, cus = 0;
@@ -531,7 +629,9 @@ int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         ws->cus = std::max(cus, 1);
     }
-    const WfArrays& a = ws->a;
+    const int nsets = int(sets.size());
+    int rc = ensure_streams(ws, nsets, err);
+    if (rc) return rc;
     using K = Kernels<Frames>;
     const typename K::ExtendFn k_extend = K::extend(variant, any_hit, stats);
     const typename K::ShadowFn k_shadow = K::shadow(variant, any_hit, stats);
@@ -541,47 +641,34 @@ int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_
     const uint32_t g_shade = ws->grid_for(k_shade);
     const uint32_t g_resolve = ws->grid_for((const void*)k_wf_resolve<Frames>);
     hipError_t e = hipSuccess;
-    // stage marks: 0 extend, 1 shade, 2 shadow, 3 resolve (init counts as resolve)
+    auto stream_of = [&](int g) { return g == 0 ? st : ws->set_stream[g]; };
+    // stage marks (one slot set only): 0 extend, 1 shade, 2 shadow, 3 resolve (init counts as resolve)
     auto mark = [&](int stage) {
         if (tm && e == hipSuccess) e = tm->mark(st, stage);
     };
-    mark(-1);
-    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, a.counts);
-    hipLaunchKernelGGL(k_wf_init<Frames>, dim3(std::min(g_resolve, (nslots + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock),
-                       0, st, fr, a, nslots);
-    mark(3);
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e != hipSuccess) {
-        *err = "launch k_wf_init";
-        return int(e);
-    }
-    const uint64_t max_iter = max_steps + kCheckEvery;
-    // one iteration: extend, shade, shadow, resolve, with the path queues of parity k
-    auto iteration = [&](uint64_t k, hipStream_t s, bool marks) {
+    // one iteration of set g: extend, shade, shadow, resolve, with the path queues of parity k
+    auto iteration = [&](const SlotSet<Frames>& set, uint64_t k, hipStream_t s, bool marks) {
+        const WfArrays& a = set.a;
         uint32_t* q_in = (k & 1) ? a.q1 : a.q0;
         uint32_t* q_out = (k & 1) ? a.q0 : a.q1;
         uint32_t* n_in = a.counts + (k & 1);
         uint32_t* n_out = a.counts + ((k + 1) & 1);
-        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, s, fr, a, q_in, n_in, n_out, a.counts + 2);
+        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in, n_out, a.counts + 2);
         if (marks) mark(0);
-        if (stats) hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, s, fr, a, q_in, n_in);
-        else hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, s, fr, a, q_in, n_in);
+        if (stats) hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in);
+        else hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in);
         if (marks) mark(1);
-        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, s, fr, a);
+        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, s, set.fr, a);
         if (marks) mark(2);
-        hipLaunchKernelGGL(k_wf_resolve<Frames>, dim3(g_resolve), dim3(kWfBlock), 0, s, fr, a, q_in, n_in, q_out, n_out);
+        hipLaunchKernelGGL(k_wf_resolve<Frames>, dim3(g_resolve), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in, q_out,
+                           n_out);
         if (marks) mark(3);
     };
-    // the live-path count after an even number of iterations sits in counts[0]
-    auto live_paths = [&](uint32_t& live) -> hipError_t {
-        hipError_t r = hipMemcpyAsync(&live, a.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-        if (r == hipSuccess) r = hipStreamSynchronize(st);
-        return r;
-    };
     static_assert(kCheckEvery % 2 == 0, "a chunk ends on queue 0");
-    if (ws->use_graph && !tm) {
-        // the kCheckEvery iterations of a chunk captured once as a HIP graph and
-        // replayed until no path is alive: one launch per chunk instead of 32
+    const bool graph = ws->use_graph && !tm;
+    if (graph) {
+        // the kCheckEvery iterations of a chunk of each set captured once as a HIP
+        // graph (one launch per chunk instead of 32)
         if (!ws->cap_stream) {
             e = hipStreamCreateWithFlags(&ws->cap_stream, hipStreamNonBlocking);
             if (e != hipSuccess) {
@@ -589,72 +676,128 @@ int run_wavefront(WavefrontState* ws, const Frames& fr, uint32_t nslots, uint64_
                 return int(e);
             }
         }
-        e = hipStreamBeginCapture(ws->cap_stream, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) {
-            *err = "hipStreamBeginCapture";
-            return int(e);
-        }
-        for (uint64_t k = 0; k < kCheckEvery; k++) iteration(k, ws->cap_stream, false);
-        hipGraph_t graph = nullptr;
-        hipError_t launch_err = hipGetLastError();
-        e = hipStreamEndCapture(ws->cap_stream, &graph);
-        if (e == hipSuccess) e = launch_err;
-        if (e == hipSuccess && ws->exec) {
-            hipGraphNode_t bad = nullptr;
-            hipGraphExecUpdateResult res;
-            if (hipGraphExecUpdate(ws->exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
-                (void)hipGetLastError();
-                (void)hipGraphExecDestroy(ws->exec);
-                ws->exec = nullptr;
-            }
-        }
-        if (e == hipSuccess && !ws->exec) e = hipGraphInstantiate(&ws->exec, graph, nullptr, nullptr, 0);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            *err = "capture of the wavefront chunk";
-            return int(e);
-        }
-        for (uint64_t k = 0;; k += kCheckEvery) {
-            e = hipGraphLaunch(ws->exec, st);
-            uint32_t live = 0;
-            if (e == hipSuccess) e = live_paths(live);
+        for (int g = 0; g < nsets; g++) {
+            e = hipStreamBeginCapture(ws->cap_stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) {
-                *err = "replay of the wavefront chunk";
+                *err = "hipStreamBeginCapture";
                 return int(e);
             }
-            if (live == 0) break;
-            if (k > max_iter) {
-                *err = "paths did not terminate";
-                return int(hipErrorUnknown);
+            for (uint64_t k = 0; k < kCheckEvery; k++) iteration(sets[g], k, ws->cap_stream, false);
+            hipGraph_t cg = nullptr;
+            hipError_t launch_err = hipGetLastError();
+            e = hipStreamEndCapture(ws->cap_stream, &cg);
+            if (e == hipSuccess) e = launch_err;
+            if (e == hipSuccess && ws->exec[g]) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res;
+                if (hipGraphExecUpdate(ws->exec[g], cg, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(ws->exec[g]);
+                    ws->exec[g] = nullptr;
+                }
+            }
+            if (e == hipSuccess && !ws->exec[g]) e = hipGraphInstantiate(&ws->exec[g], cg, nullptr, nullptr, 0);
+            if (cg) (void)hipGraphDestroy(cg);
+            if (e != hipSuccess) {
+                *err = "capture of the wavefront chunk";
+                return int(e);
             }
         }
-        return 0;
     }
-    for (uint64_t k = 0;; k++) {
-        iteration(k, st, true);
-        if (e == hipSuccess) e = hipGetLastError();
+    // fork: the sets' streams start after everything already enqueued on st
+    if (nsets > 1) {
+        e = hipEventRecord(ws->fork, st);
+        for (int g = 1; g < nsets && e == hipSuccess; g++) e = hipStreamWaitEvent(ws->set_stream[g], ws->fork, 0);
         if (e != hipSuccess) {
-            *err = "launch of a wavefront kernel";
+            *err = "fork of the slot-set streams";
             return int(e);
         }
-        if ((k + 1) % kCheckEvery == 0) {
-            uint32_t live = 0;
-            e = live_paths(live);
+    }
+    mark(-1);
+    for (int g = 0; g < nsets; g++) {
+        const SlotSet<Frames>& set = sets[g];
+        hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, stream_of(g), set.a.counts);
+        hipLaunchKernelGGL(k_wf_init<Frames>, dim3(std::min(g_resolve, (set.nslots + kWfBlock - 1) / kWfBlock)),
+                           dim3(kWfBlock), 0, stream_of(g), set.fr, set.a, set.nslots);
+    }
+    mark(3);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        *err = "launch k_wf_init";
+        rc = int(e);
+    }
+    const uint64_t max_chunks = max_steps / kCheckEvery + 2;
+    bool alive[kMaxStreams];
+    for (int g = 0; g < kMaxStreams; g++) alive[g] = g < nsets;
+    for (uint64_t c = 0; rc == 0; c++) {
+        // enqueue chunk c of every live set, then its live-path count read-back
+        for (int g = 0; g < nsets && rc == 0; g++) {
+            if (!alive[g]) continue;
+            hipStream_t s = stream_of(g);
+            if (graph) {
+                e = hipGraphLaunch(ws->exec[g], s);
+            } else {
+                for (uint64_t k = 0; k < kCheckEvery; k++) iteration(sets[g], k, s, true);
+                if (e == hipSuccess) e = hipGetLastError();
+            }
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(ws->host_live + 2 * g + (c & 1), sets[g].a.counts, sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipEventRecord(ws->counted[g][c & 1], s);
+            if (e != hipSuccess) {
+                *err = "launch of a wavefront chunk";
+                rc = int(e);
+            }
+        }
+        if (rc || c == 0) continue;
+        // sets whose count after chunk c - 1 was zero are done (their chunk c found empty queues)
+        bool any = false;
+        for (int g = 0; g < nsets && rc == 0; g++) {
+            if (!alive[g]) continue;
+            e = hipEventSynchronize(ws->counted[g][(c - 1) & 1]);
             if (e != hipSuccess) {
                 *err = "reading the queue length";
-                return int(e);
+                rc = int(e);
+                break;
             }
-            if (live == 0) break;
+            const uint32_t live = ws->host_live[2 * g + ((c - 1) & 1)];
+            if (live == 0u) {
+                alive[g] = false;
+            } else if (live < ws->finish_below && !tm) {
+                // the paths left after chunk c (at most `live`) end in one launch
+                const typename K::FinishFn k_finish = K::finish(any_hit, stats);
+                const uint32_t blocks = std::min((live + kFinishBlock - 1) / kFinishBlock, 65535u);
+                hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kFinishBlock), 0, stream_of(g), sets[g].fr, sets[g].a,
+                                   sets[g].a.q0, sets[g].a.counts);
+                e = hipGetLastError();
+                if (e != hipSuccess) {
+                    *err = "launch k_wf_finish";
+                    rc = int(e);
+                    break;
+                }
+                alive[g] = false;
+            }
+            any = any || alive[g];
         }
-        if (k > max_iter) {
+        if (rc == 0 && !any) break;
+        if (rc == 0 && c > max_chunks) {
             *err = "paths did not terminate";
-            return int(hipErrorUnknown);
+            rc = int(hipErrorUnknown);
         }
     }
-    return 0;
+    // join: st continues after every set's work
+    for (int g = 1; g < nsets; g++) {
+        hipError_t j = hipEventRecord(ws->join[g], ws->set_stream[g]);
+        if (j == hipSuccess) j = hipStreamWaitEvent(st, ws->join[g], 0);
+        if (j != hipSuccess && rc == 0) {
+            *err = "join of the slot-set streams";
+            rc = int(j);
+        }
+    }
+    return rc;
 }
 
-uint32_t frame_slots(const FrameArgs& fa) { return ((fa.width + 7) / 8) * ((fa.height + 7) / 8) * 64u; }
+uint32_t frame_tiles(const FrameArgs& fa) { return ((fa.width + 7) / 8) * ((fa.height + 7) / 8); }
 
 }  // namespace
 
@@ -665,11 +808,19 @@ int launch_wavefront(WavefrontState* ws, const FrameArgs& fa, hipStream_t st, bo
         return kWavefrontUnsupported;
     }
     if (fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, stats, tm);
-    const uint32_t nslots = frame_slots(fa);
-    int rc = ensure_slots(ws, nslots, err);
+    const uint32_t tiles = frame_tiles(fa);
+    int rc = ensure_slots(ws, tiles * 64u, err);
     if (rc) return rc;
-    return run_wavefront(ws, OneFrame{fa}, nslots, uint64_t(fa.samples) * fa.bounces, st, any_hit, stats, variant, tm,
-                         err);
+    // the frame's tiles dealt round-robin to the slot sets (stage timing: one set)
+    const uint32_t nsets = tm ? 1u : std::min(uint32_t(ws->streams), tiles);
+    std::vector<SlotSet<OneFrame>> sets;
+    uint32_t base = 0;
+    for (uint32_t g = 0; g < nsets; g++) {
+        const uint32_t n = ((tiles - g + nsets - 1) / nsets) * 64u;
+        sets.push_back(SlotSet<OneFrame>{OneFrame{fa, nsets, g}, slice(ws->a, base, ws->a.counts + 8 * g), n});
+        base += n;
+    }
+    return run_wavefront(ws, sets, uint64_t(fa.samples) * fa.bounces, st, any_hit, stats, variant, tm, err);
 }
 
 int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t count, hipStream_t st, bool any_hit,
@@ -682,7 +833,7 @@ int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t
     uint64_t total = 0, steps = 0;
     for (uint32_t i = 0; i < count; i++) {
         base[i] = uint32_t(total);
-        if (frames[i].samples && frames[i].bounces) total += frame_slots(frames[i]);
+        if (frames[i].samples && frames[i].bounces) total += frame_tiles(frames[i]) * 64u;
         steps = std::max(steps, uint64_t(frames[i].samples) * frames[i].bounces);
         if (total >= (1ull << 31)) {
             *err = "the batch exceeds 2^31 path slots";
@@ -693,8 +844,22 @@ int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t
     if (total == 0) return 0;
     int rc = ensure_slots(ws, uint32_t(total), err);
     if (rc) return rc;
-    // frame arguments and slot offsets in device memory
-    const size_t fbytes = sizeof(FrameArgs) * count, bbytes = sizeof(uint32_t) * (count + 1);
+    // slot sets: runs of whole frames with about total / nsets slots each
+    const uint32_t want = tm ? 1u : uint32_t(ws->streams);
+    std::vector<uint32_t> cut{0u};  // first frame of each set, then count
+    for (uint32_t i = 1; i < count && cut.size() < want; i++)
+        if (uint64_t(base[i]) * want >= uint64_t(total) * cut.size() && base[i] > base[cut.back()] && base[i] < total)
+            cut.push_back(i);  // every set owns at least one slot
+    cut.push_back(count);
+    const uint32_t nsets = uint32_t(cut.size()) - 1u;
+    // frame arguments and per-set slot offsets (relative to the set) in device memory
+    std::vector<uint32_t> set_base;
+    std::vector<size_t> set_off;
+    for (uint32_t g = 0; g < nsets; g++) {
+        set_off.push_back(set_base.size());
+        for (uint32_t i = cut[g]; i <= cut[g + 1]; i++) set_base.push_back(base[i] - base[cut[g]]);
+    }
+    const size_t fbytes = sizeof(FrameArgs) * count, bbytes = sizeof(uint32_t) * set_base.size();
     if (fbytes + bbytes > ws->batch_cap) {
         if (ws->batch_mem) (void)hipFree(ws->batch_mem);
         ws->batch_mem = nullptr;
@@ -710,13 +875,19 @@ int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t
     uint32_t* d_base = reinterpret_cast<uint32_t*>(ws->batch_mem + ((fbytes + 255) & ~size_t(255)));
     // frames with no samples or bounces own no slots: the image reset still applies
     hipError_t e = hipMemcpyAsync(d_frames, frames, fbytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_base, base.data(), bbytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_base, set_base.data(), bbytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) {
         *err = "copy of the batch table";
         return int(e);
     }
-    ManyFrames fr{d_frames, d_base, count, frames[0].counters};
-    rc = run_wavefront(ws, fr, uint32_t(total), steps, st, any_hit, stats, variant, tm, err);
+    std::vector<SlotSet<ManyFrames>> sets;
+    for (uint32_t g = 0; g < nsets; g++) {
+        const uint32_t f0 = cut[g], nf = cut[g + 1] - cut[g];
+        const uint32_t first = base[f0], n = base[cut[g + 1]] - first;
+        sets.push_back(SlotSet<ManyFrames>{ManyFrames{d_frames + f0, d_base + set_off[g], nf, frames[0].counters},
+                                           slice(ws->a, first, ws->a.counts + 8 * g), n});
+    }
+    rc = run_wavefront(ws, sets, steps, st, any_hit, stats, variant, tm, err);
     // the host arrays the copies read must outlive them
     if (rc == 0) {
         e = hipStreamSynchronize(st);

@@ -69,7 +69,8 @@ class Bindings(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("traversal_variant", C.c_uint32), ("graph", C.c_uint32), ("extend_refill", C.c_uint32),
-                ("shadow_refill", C.c_uint32), ("frame_ms", C.c_double)]
+                ("shadow_refill", C.c_uint32), ("wavefront_streams", C.c_uint32), ("wavefront_finish", C.c_uint32),
+                ("frame_ms", C.c_double)]
 
 
 class BatchItem(C.Structure):
@@ -92,6 +93,8 @@ OPTION_GRAPH = 4
 OPTION_EXTEND_REFILL = 5
 OPTION_SHADOW_REFILL = 6
 OPTION_OUTPUT_BGRA = 7
+OPTION_WAVEFRONT_STREAMS = 8
+OPTION_WAVEFRONT_FINISH = 9
 TONE_MAP_DEFAULT, TONE_MAP_UNCHARTED, TONE_MAP_HEJL_RICHARD, TONE_MAP_ACES, TONE_MAP_SRGB = 0, 1, 2, 3, 4
 NUM_TRAVERSAL_VARIANTS = 4
 

@@ -26,7 +26,7 @@ int main(void) {
   F(rt_material_raw, sg_specular_glossiness_factor) F(rt_material_raw, sg_specular_glossiness_texture)
   F(rt_ubo, projection_inverse) F(rt_ubo, exposure) F(rt_bindings, num_materials)
   F(rt_trace_stats, kernel_ms) F(rt_trace_stats, shadow_instance_visits) F(rt_hit, custom_index)
-  F(rt_stage_times, iterations)
+  F(rt_stage_times, iterations) F(rt_tuning, frame_ms)
   return 0;
 }
 """
@@ -67,6 +67,7 @@ def test_ctypes_structs_match_c_headers(c_layout):
     assert c_layout["rt_trace_stats.kernel_ms"] == rt.TraceStats.kernel_ms.offset
     assert c_layout["rt_trace_stats.shadow_instance_visits"] == rt.TraceStats.shadow_instance_visits.offset
     assert c_layout["rt_stage_times.iterations"] == rt.StageTimes.iterations.offset
+    assert c_layout["rt_tuning.frame_ms"] == rt.Tuning.frame_ms.offset
 
 
 def test_reference_layout_sizes(c_layout):
