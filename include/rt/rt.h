@@ -246,7 +246,11 @@ enum {
      * iterations, whose launches would last as long as their slowest ray while most
      * of the GPU idles.  0: never.  Same images.  Ignored while
      * RT_OPTION_STAGE_TIMING is on.  Default 65536. */
-    RT_OPTION_WAVEFRONT_FINISH = 9
+    RT_OPTION_WAVEFRONT_FINISH = 9,
+    /* Most triangles per BVH leaf (1 .. 16) of BLASes and flattened BVHs built
+     * afterwards: a subtree with at most this many triangles becomes one leaf.
+     * Same hits.  Default 2. */
+    RT_OPTION_BLAS_LEAF_SIZE = 10
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 

@@ -13,52 +13,95 @@ EMPTY = 0xFFFFFFFF
 
 
 def check_blas(ctx, blas, expect_prims=None):
-    info = blas.info()
-    nodes = rt.read_nodes(ctx, info)
-    n_total = info.first_prim + info.num_prims
-    tris = np.frombuffer(rt.read_device(info.tris, 48 * max(1, n_total)), TRI)[:n_total] if n_total else np.zeros(0, TRI)
-    return _check(nodes, info.root, info.first_prim, info.num_prims, tris, "triangle", expect_prims)
+    """Errors of one BLAS (empty list: valid)."""
+    return check_blases(ctx, [blas], None if expect_prims is None else [expect_prims])[0]
+
+
+def check_blases(ctx, blases, expect_prims=None):
+    """Errors of each BLAS.  The BLASes of one batched build share a node pool and a
+    triangle pool, which are read and decoded once."""
+    infos = [b.info() for b in blases]
+    pools = {}
+    for k, info in enumerate(infos):
+        pools.setdefault((info.nodes, info.num_nodes, info.tris), []).append(k)
+    out = [None] * len(blases)
+    for members in pools.values():
+        info0 = infos[members[0]]
+        nodes = rt.read_nodes(ctx, info0)
+        n_total = max(infos[k].first_prim + infos[k].num_prims for k in members)
+        tris = np.frombuffer(rt.read_device(info0.tris, 48 * max(1, n_total)), TRI)[:n_total] if n_total else np.zeros(0, TRI)
+        for k in members:
+            info = infos[k]
+            out[k] = _check(nodes, info.root, info.first_prim, info.num_prims, tris, "triangle",
+                            None if expect_prims is None else expect_prims[k])
+    return out
 
 
 def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
+    """Level-by-level walk from `root` over all nodes of a level at once."""
     errors = []
-    seen_nodes = np.zeros(len(nodes), bool)
-    covered = np.zeros(num_prims, np.int32)
-    stack = [(root, None)]  # (node, intersection of the decoded boxes on the path)
-    while stack:
-        ni, path_box = stack.pop()
-        if ni >= len(nodes):
-            errors.append(f"node index {ni} out of range {len(nodes)}")
-            continue
-        if seen_nodes[ni]:
-            errors.append(f"node {ni} reached twice")
-            continue
-        seen_nodes[ni] = True
-        nd = nodes[ni]
-        for k in range(4):
-            c, cnt = int(nd["child"][k]), int(nd["count"][k])
-            if c == EMPTY:
-                continue
-            lo = np.array([nd["lo_x"][k], nd["lo_y"][k], nd["lo_z"][k]], np.float32)
-            hi = np.array([nd["hi_x"][k], nd["hi_y"][k], nd["hi_z"][k]], np.float32)
-            if np.any(lo > hi):
-                errors.append(f"node {ni} child {k} box is inverted")
-            box = (lo, hi) if path_box is None else (np.maximum(lo, path_box[0]), np.minimum(hi, path_box[1]))
-            if cnt == 0:
-                stack.append((c, box))
-                continue
-            if c < first_prim or c + cnt > first_prim + num_prims:
-                errors.append(f"leaf [{c},{c + cnt}) outside [{first_prim},{first_prim + num_prims})")
-                continue
-            covered[c - first_prim:c - first_prim + cnt] += 1
-            if kind in ("triangle", "flat"):
-                t = tris[c:c + cnt]
-                pts = np.concatenate([t["v0"], t["v1"][:, :3], t["v2"][:, :3]])
-                if np.any(pts < box[0]) or np.any(pts > box[1]):
-                    errors.append(f"leaf [{c},{c + cnt}) triangles not inside every box on their path")
+    n = len(nodes)
+    if root >= n:
+        return [f"node index {root} out of range {n}"]
+    child = nodes["child"].astype(np.int64)
+    count = nodes["count"].astype(np.int64)
+    lo = np.stack([nodes["lo_x"], nodes["lo_y"], nodes["lo_z"]], -1)  # (n, 4, 3)
+    hi = np.stack([nodes["hi_x"], nodes["hi_y"], nodes["hi_z"]], -1)
+    visits = np.zeros(n, np.int64)
+    visits[root] = 1
+    # intersection of the decoded boxes on the path to each reached node
+    path_lo = np.full((n, 3), -np.inf, np.float32)
+    path_hi = np.full((n, 3), np.inf, np.float32)
+    frontier = np.array([root], np.int64)
+    leaf_first, leaf_cnt, leaf_lo, leaf_hi = [], [], [], []
+    while frontier.size:
+        c, cnt = child[frontier], count[frontier]
+        valid = c != EMPTY
+        if np.any(valid & np.any(lo[frontier] > hi[frontier], -1)):
+            errors.append("a child box is inverted")
+        blo = np.maximum(lo[frontier], path_lo[frontier][:, None, :])
+        bhi = np.minimum(hi[frontier], path_hi[frontier][:, None, :])
+        internal = valid & (cnt == 0)
+        ci = c[internal]
+        if np.any(ci >= n):
+            errors.append(f"node index {int(ci.max())} out of range {n}")
+            keep = ci < n
+            ci, ilo, ihi = ci[keep], blo[internal][keep], bhi[internal][keep]
+        else:
+            ilo, ihi = blo[internal], bhi[internal]
+        np.add.at(visits, ci, 1)
+        if np.any(visits[ci] > 1):
+            errors.append(f"{int(np.sum(visits[ci] > 1))} nodes reached twice")
+        first_time = visits[ci] == 1
+        ci, ilo, ihi = ci[first_time], ilo[first_time], ihi[first_time]
+        path_lo[ci] = ilo
+        path_hi[ci] = ihi
+        leaf = valid & (cnt > 0)
+        leaf_first.append(c[leaf])
+        leaf_cnt.append(cnt[leaf])
+        leaf_lo.append(blo[leaf])
+        leaf_hi.append(bhi[leaf])
+        frontier = np.unique(ci)
+    lf, lc = np.concatenate(leaf_first), np.concatenate(leaf_cnt)
+    llo, lhi = np.concatenate(leaf_lo), np.concatenate(leaf_hi)
+    bad = (lf < first_prim) | (lf + lc > first_prim + num_prims)
+    if np.any(bad):
+        errors.append(f"{int(bad.sum())} leaves outside [{first_prim},{first_prim + num_prims})")
+        lf, lc, llo, lhi = lf[~bad], lc[~bad], llo[~bad], lhi[~bad]
+    # every record of every leaf, with its leaf's path box
+    rec = np.repeat(lf, lc) + (np.arange(int(lc.sum())) - np.repeat(np.cumsum(lc) - lc, lc))
+    covered = np.bincount(rec - first_prim, minlength=num_prims)[:num_prims] if num_prims else np.zeros(0, np.int64)
     if np.any(covered != 1):
-        bad = np.nonzero(covered != 1)[0]
-        errors.append(f"{len(bad)} records not in exactly one leaf (first {bad[:5] + first_prim}, counts {covered[bad[:5]]})")
+        badr = np.nonzero(covered != 1)[0]
+        errors.append(f"{len(badr)} records not in exactly one leaf (first {badr[:5] + first_prim}, counts {covered[badr[:5]]})")
+    if kind in ("triangle", "flat") and len(rec):
+        rlo, rhi = np.repeat(llo, lc, axis=0), np.repeat(lhi, lc, axis=0)
+        t = tris[rec]
+        for v in (t["v0"], t["v1"][:, :3], t["v2"][:, :3]):
+            out = np.any(v < rlo, -1) | np.any(v > rhi, -1)
+            if np.any(out):
+                errors.append(f"{int(out.sum())} triangles not inside every box on their path")
+                break
     if kind == "triangle" and num_prims:
         ids = np.sort(tris["prim"][first_prim:first_prim + num_prims])
         if not np.array_equal(ids, np.arange(num_prims, dtype=np.uint32)):

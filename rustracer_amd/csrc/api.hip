@@ -23,7 +23,7 @@ using namespace rt;
 
 namespace {
 
-constexpr uint32_t kBlasLeafMax = 4;
+constexpr uint32_t kBlasLeafMax = 2;  // default triangles per leaf (RT_OPTION_BLAS_LEAF_SIZE)
 constexpr uint32_t kTlasLeafMax = 1;
 
 std::string g_create_error;
@@ -108,6 +108,7 @@ struct rt_ctx {
     uint32_t refill_ext = 40, refill_sh = 1;  // RT_OPTION_EXTEND_REFILL / RT_OPTION_SHADOW_REFILL
     bool out_bgra = false;   // RT_OPTION_OUTPUT_BGRA
     int streams = 2;         // RT_OPTION_WAVEFRONT_STREAMS
+    uint32_t leaf_max = kBlasLeafMax;  // RT_OPTION_BLAS_LEAF_SIZE
     uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
@@ -376,7 +377,7 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     LbvhResult res;
     std::string err;
     rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), n, (const uint32_t*)d_seg_start.get(),
-                    seg_start, kBlasLeafMax, &res, &err);
+                    seg_start, ctx->leaf_max, &res, &err);
     if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "BLAS build: " + err);
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
     auto pool = std::make_shared<BlasPool>();
@@ -592,7 +593,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
         if (rc) return hip_fail(ctx, hipError_t(rc), "gather_flat");
         LbvhResult fres;
         rc = lbvh_build(st, (const float4*)d_flo.get(), (const float4*)d_fhi.get(), n, (const uint32_t*)d_fseg.get(),
-                        fseg, kBlasLeafMax, &fres, &err);
+                        fseg, ctx->leaf_max, &fres, &err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "flattened BVH build: " + err);
         DevPtr fperm(fres.perm), fseg_bounds(fres.seg_bounds);
         te.flat_nodes.reset(fres.nodes);
@@ -1032,6 +1033,12 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (option == RT_OPTION_GRAPH) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_GRAPH takes 0 or 1");
         ctx->use_graph = value != 0;
+        return RT_OK;
+    }
+    if (option == RT_OPTION_BLAS_LEAF_SIZE) {
+        if (value < 1 || value > uint64_t(kMaxLeafPrims))
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_BLAS_LEAF_SIZE takes 1 .. 16");
+        ctx->leaf_max = uint32_t(value);
         return RT_OK;
     }
     if (option == RT_OPTION_WAVEFRONT_FINISH) {

@@ -313,9 +313,16 @@ struct Traversal {
             // triangle leaf of a BLAS or of the flattened BVH
             const uint32_t first = e & kLeafFirstMask;
             const uint32_t count = ((e >> kLeafCountShift) & 15u) + 1u;
+            const float4* tp = tris + 3 * size_t(first);
+            float4 na = tp[0], nb = tp[1], nc = tp[2];
             for (uint32_t k = 0; k < count; k++) {
-                const float4* tp = tris + 3 * size_t(first + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                // the next record's loads are in flight while this one is tested
+                const float4 a = na, b = nb, c = nc;
+                if (k + 1 < count) {
+                    na = tp[3 * (k + 1)];
+                    nb = tp[3 * (k + 1) + 1];
+                    nc = tp[3 * (k + 1) + 2];
+                }
                 if (kStats) st.tris++;
                 float t, bu, bv;
                 if (!woop_hit(wr, cur.o, xyz(a), xyz(b), xyz(c), tmin, best_t, t, bu, bv)) continue;

@@ -16,8 +16,10 @@ pytestmark = pytest.mark.gpu
 
 def _check_all(ctx, ds, every=1):
     sc = ds.scene
-    for g in range(0, len(ds.blases), every):
-        errs = bvhcheck.check_blas(ctx, ds.blases[g], int(sc.geometries["prim_count"][g]))
+    idx = list(range(0, len(ds.blases), every))
+    all_errs = bvhcheck.check_blases(ctx, [ds.blases[g] for g in idx],
+                                     [int(sc.geometries["prim_count"][g]) for g in idx])
+    for g, errs in zip(idx, all_errs):
         assert not errs, (g, errs[:3])
     errs = bvhcheck.check_tlas(ctx, ds.tlas, len(sc.instances))
     assert not errs, errs[:3]
