@@ -224,10 +224,11 @@ enum {
      * is alive (one launch per chunk).  Same images.  Ignored while
      * RT_OPTION_STAGE_TIMING is on.  Default 0. */
     RT_OPTION_GRAPH = 4,
-    /* Wavefront traversal kernels fetch rays dynamically: a wave takes new rays
-     * from the queue once fewer than this many of its 64 lanes are still
-     * traversing (1 = only when all have finished).  Same images.  Defaults:
-     * extend 40, shadow 1. */
+    /* The wavefront traversal kernel fetches rays dynamically: a wave takes new
+     * rays from the queue once fewer than this many of its 64 lanes are still
+     * traversing (1 = only when all have finished).  Same images.  Default 40.
+     * RT_OPTION_SHADOW_REFILL (1 .. 64) is accepted and has no effect: shadow rays
+     * are fetched with the extension rays. */
     RT_OPTION_EXTEND_REFILL = 5,
     RT_OPTION_SHADOW_REFILL = 6,
     /* Byte order of rt_trace_rays' display image: 0 (default) R, G, B, A; 1: B, G,
@@ -258,12 +259,12 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * (the analogue of the reference's GPU timestamp query pool,
  * crates/libs/vulkan/src/query.rs:8-73).  The megakernel reports all of its time
  * as extend_ms.  Synchronises the context stream. */
-/* Autotuner: traces the frame (the bindings' UBO with at most 2 samples per pixel)
+/* Autotuner: traces the frame (the bindings' UBO with at most 8 samples per pixel)
  * in wavefront mode and tunes, one after another, the traversal variant (stack
  * depth, register target), the number of concurrent slot sets, the live-path
- * count that hands the tail to one finishing launch, the extend and shadow refill
- * thresholds and graph replay, keeping the fastest settings on this context (the
- * RT_OPTION_* values).
+ * count that hands the tail to one finishing launch, the refill threshold and
+ * graph replay, keeping the fastest settings on this context (the RT_OPTION_*
+ * values).
  * A variant whose stack ring gives up on a ray of this frame (restart budget) is
  * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */
 typedef struct rt_tuning {
@@ -279,10 +280,12 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
 
 typedef struct rt_stage_times {
     double total_ms;
-    double extend_ms;   /* closest-hit traversal */
-    double shade_ms;    /* closest-hit / miss shaders */
-    double shadow_ms;   /* shadow-ray traversal */
-    double resolve_ms;  /* radiance update, regeneration, compaction */
+    double extend_ms;   /* traversal: each iteration's extension rays together with the
+                           previous bounce's shadow rays (one kernel) */
+    double shade_ms;    /* NEE radiance, closest-hit / miss shaders, next rays,
+                           regeneration, compaction */
+    double shadow_ms;   /* 0: shadow rays are traced with the extension rays */
+    double resolve_ms;  /* path-state initialisation */
     uint32_t iterations;
     uint32_t _pad;
 } rt_stage_times;

@@ -23,6 +23,7 @@ using namespace rt;
 
 namespace {
 
+constexpr uint32_t kAutotuneSamples = 8;  // samples per pixel of the autotuner's probe frames
 constexpr uint32_t kBlasLeafMax = 2;  // default triangles per leaf (RT_OPTION_BLAS_LEAF_SIZE)
 constexpr uint32_t kTlasLeafMax = 1;
 
@@ -885,7 +886,7 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
     DeviceGuard g(ctx->device);
     rt_ubo u = *b->ubo;
-    u.number_of_samples = std::min(u.number_of_samples, 2u);
+    u.number_of_samples = std::min(u.number_of_samples, kAutotuneSamples);
     u.total_number_of_samples = u.number_of_samples;
     rt_bindings bb = *b;
     bb.ubo = &u;
@@ -964,22 +965,21 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         }
         ctx->finish = chosen;
     }
-    // 4, 5. refill thresholds of the extend and shadow kernels
-    const uint32_t refills[] = {1u, 16u, 32u, 48u, 64u};
-    for (int which = 0; which < 2 && rc == RT_OK; which++) {
-        uint32_t& knob = which == 0 ? ctx->refill_ext : ctx->refill_sh;
-        uint32_t chosen = knob;
+    // 4. refill threshold of the traversal kernel
+    if (rc == RT_OK) {
+        const uint32_t refills[] = {1u, 16u, 32u, 48u, 64u};
+        uint32_t chosen = ctx->refill_ext;
         for (uint32_t r : refills) {
             if (r == chosen) continue;
-            knob = r;
+            ctx->refill_ext = r;
             double ms = -1.0;
             rc = probe(ms);
             if (rc) break;
             if (consider(ms)) { best = ms; chosen = r; }
         }
-        knob = chosen;
+        ctx->refill_ext = chosen;
     }
-    // 6. graph replay
+    // 5. graph replay
     if (rc == RT_OK) {
         ctx->use_graph = true;
         double ms = -1.0;

@@ -2,17 +2,21 @@
 //
 // The raygen loop of the reference (crates/examples/gltf_viewer/shaders/RayTracing.rgen)
 // is split into queue kernels that each do one stage for every live path:
-//   init     path state of every pixel, camera ray of sample 0           -> queue
-//   extend   closest-hit traversal (fp32, small register footprint)
-//   shade    closest-hit / miss shaders in fp64, NEE set-up, next ray     -> shadow queue
-//   shadow   shadow-ray traversal with terminate-on-first-hit
-//   resolve  radiance update; a path that ended starts its pixel's next
-//            sample or retires the pixel into the accumulation image       -> next queue
-// One slot per pixel keeps each pixel's RNG streams sequential, so the image is
-// the megakernel's bit for bit (both call the same device functions in path.h).
-// Queues are compacted with wave ballots (one atomic per wave).  Kernels are
-// persistent grid-stride loops that read their queue length from device memory,
-// so the host only checks for completion every few iterations.
+//   init   path state of every pixel, camera ray of sample 0            -> path queue
+//   trace  traversal of the queue's extension rays (closest hit) and of the
+//          previous bounce's shadow rays (terminate on first hit), served by one
+//          persistent grid from one fetch cursor
+//   shade  the previous bounce's NEE radiance (its shadow ray's result), then the
+//          closest-hit / miss shaders in fp64, NEE set-up (shadow ray), next ray;
+//          a path that ended starts its pixel's next sample or retires the pixel
+//          into the accumulation image                                   -> path queue
+// Adding a bounce's NEE term one iteration late (once its shadow ray is traced)
+// changes nothing in the order of the radiance sums or of the RNG draws, so the image
+// is the megakernel's bit for bit (both call the same device functions in path.h).
+// One slot per pixel keeps each pixel's RNG streams sequential.  Queues are
+// compacted with wave ballots (one atomic per wave).  Kernels are persistent
+// grid-stride loops that read their queue lengths from device memory, so the host
+// only checks for completion every few iterations.
 #include <algorithm>
 #include <unordered_map>
 #include <vector>
@@ -37,17 +41,26 @@ struct WfArrays {
     uint32_t *h_inst, *h_prim, *h_rec;
     float *h_t, *h_u, *h_v;
     float* last_t;     // payload t after the last trace (DISTANCE mapping only)
-    uint32_t *flags;   // kFlagShadow | kFlagCont
+    uint32_t *flags;   // kFlagShadow
     uint32_t *unocc;   // shadow result
-    float4 *sq_o, *sq_d;  // shadow queue: origin + tmax, direction + slot bits
-    uint32_t* sq_wb;
-    uint32_t *q0, *q1;
-    uint32_t* counts;  // [0], [1]: path queues (ping-pong), [2]: shadow queue,
-                       // [3]: extend fetch cursor, [4]: shadow fetch cursor (8 per slot set)
+    float4 *sq_o, *sq_d;  // pending shadow ray per slot: origin + tmax, direction
+    uint32_t* sq_wb;      // its stream-B counter
+    uint32_t *q0, *q1;    // path queues (ping-pong)
+    uint32_t *qs0, *qs1;  // shadow-ray queues (ping-pong): slot | kRetireBit
+    uint32_t* counts;  // [0], [1]: path queue lengths, [2], [3]: shadow queue lengths
+                       // (ping-pong), [4]: trace fetch cursor (8 per slot set)
     uint32_t refill_ext, refill_sh;  // busy lanes below which a wave takes new rays (1..64)
 };
 
-enum : uint32_t { kFlagShadow = 1u, kFlagCont = 2u };
+// kFlagShadow: the slot's last shaded bounce has a shadow ray whose result (unocc)
+// decides its NEE radiance (pw* or po*), added by the next shade.  A shadow queue
+// entry with kRetireBit: that bounce ended the pixel's last sample, and the slot
+// has no path queue entry; the next shade adds the radiance and writes the pixel.
+enum : uint32_t { kFlagShadow = 1u };
+constexpr uint32_t kRetireBit = 0x80000000u;
+constexpr int kCursor = 4;  // counts[kCursor]: the trace kernel's fetch cursor
+
+__host__ __device__ inline uint32_t* shadow_count(const WfArrays& a, uint64_t parity) { return a.counts + 2 + (parity & 1); }
 
 // Slot -> pixel: slots come in 8x8 pixel tiles (consecutive slots are neighbouring
 // pixels); a set of slots traced on one stream holds every `step`-th tile from `first`.
@@ -188,50 +201,87 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_init(Frames fr, WfArrays a, uin
             init_path(fa, x, y, p);
             begin_sample(fa, x, y, p);
             store_full(a, s, p);
+            a.flags[s] = 0u;
         }
         uint32_t idx = wave_append(valid, &a.counts[0]);
         if (valid) q[idx] = s;
     }
 }
 
-// Traversal variants of the extend and shadow kernels: LDS stack depth and the
-// waves per SIMD the register allocation targets (the compiler may spill above
-// its natural occupancy; the any-hit kernels keep their natural allocation).
+// Traversal variants of the trace kernel: LDS stack depth and the waves per SIMD the
+// register allocation targets (the compiler may spill above its natural occupancy).
 // Selected per context (RT_OPTION_TRAVERSAL_VARIANT).
 struct TravVariant {
     int depth, waves;
 };
 constexpr TravVariant kTravVariants[] = {{32, 5}, {16, 6}, {16, 8}, {64, 2}};
 constexpr int kNumTravVariants = int(sizeof(kTravVariants) / sizeof(kTravVariants[0]));
+// The any-hit kernels (alpha-tested geometry) hold more state: a variant's target
+// maps to 3 waves (no spills), 4 for the 16-deep variants (spills), or 2.
+constexpr int anyhit_waves(int w) { return w >= 6 ? 4 : (w < 3 ? w : 3); }
 
+// The ray a trace-kernel lane works on, in fp64 for instance entry, re-read from the
+// slot (keeps it out of the traversal's registers): an extension ray from the fp64
+// path state, a shadow ray from its fp32 record.
+struct RayFromSlotOrShadow {
+    const WfArrays& a;
+    uint32_t s;
+    bool shadow;
+    __device__ void get(D3& O, D3& Dd) const {
+        if (shadow) {
+            const float4 o = a.sq_o[s], d = a.sq_d[s];
+            O = d3(make_float3(o.x, o.y, o.z));
+            Dd = d3(make_float3(d.x, d.y, d.z));
+        } else {
+            O = d3(a.ox[s], a.oy[s], a.oz[s]);
+            Dd = d3(a.dx[s], a.dy[s], a.dz[s]);
+        }
+    }
+};
+
+// Iteration k's traversal: the extension rays of path queue q (n_e entries) then the
+// shadow rays of queue qs (n_s entries, from the previous shade), as one index
+// range served by one fetch cursor.  Zeroes the queue lengths the next shade
+// appends to.
 template <class Frames, bool kAnyHit, bool kStats, int kDepth, int kWaves>
-__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
-void k_wf_extend(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in, uint32_t* zero0, uint32_t* zero1) {
+__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? anyhit_waves(kWaves) : kWaves)))
+void k_wf_trace(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_e_in, const uint32_t* qs,
+                const uint32_t* n_s_in, uint32_t* zero_e, uint32_t* zero_s) {
     __shared__ uint32_t s_stack[kDepth * kWfBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *zero0 = 0u;        // next path queue
-        *zero1 = 0u;        // shadow queue
-        a.counts[4] = 0u;   // shadow fetch cursor
+        *zero_e = 0u;  // next path queue
+        *zero_s = 0u;  // next shadow queue
     }
-    const uint32_t n = *n_in;
+    const uint32_t n_e = *n_e_in, n = n_e + *n_s_in;
     uint32_t* const stack = s_stack + threadIdx.x;
     PathCounters pc;
+    TraceStats ts;
     Traversal<kAnyHit, kStats, kDepth> tr;
     uint32_t s = 0, x = 0, y = 0;
-    Rng4 rb{0u, 0u, 0u, 0u};
-    bool active = false, exhausted = false;
+    Rng4 rng{0u, 0u, 0u, 0u};
+    bool active = false, exhausted = false, shadow = false;
     for (;;) {
         if (!exhausted) {
-            uint32_t i = wave_fetch(!active, a.counts + 3, n, exhausted);
+            uint32_t i = wave_fetch(!active, a.counts + kCursor, n, exhausted);
             if (i != kEmpty) {
-                s = q[i];
+                shadow = i >= n_e;
+                s = shadow ? (qs[i - n_e] & ~kRetireBit) : q[i];
                 const FrameArgs& fa = fr.at(s, x, y);
-                const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-                float3 o = make_float3(float(a.ox[s]), float(a.oy[s]), float(a.oz[s]));
-                float3 d = make_float3(float(a.dx[s]), float(a.dy[s]), float(a.dz[s]));
-                if (kStats) { if (a.bounce[s] == 0u) pc.primary++; else pc.extension++; }
-                rb = Rng4{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
-                tr.begin(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), flags, stack, kWfBlock);
+                const uint32_t opq = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
+                if (shadow) {
+                    // castShadowRay, rchit:35-59: tMin 0.1, tMax the light distance
+                    const float4 o = a.sq_o[s], d = a.sq_d[s];
+                    if (kStats) pc.shadow++;
+                    rng = Rng4{x, y, fa.seed_b, a.sq_wb[s]};
+                    tr.begin(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), float(kShadowTMin), o.w,
+                             kShadowFlags | opq, stack, kWfBlock);
+                } else {
+                    float3 o = make_float3(float(a.ox[s]), float(a.oy[s]), float(a.oz[s]));
+                    float3 d = make_float3(float(a.dx[s]), float(a.dy[s]), float(a.dz[s]));
+                    if (kStats) { if (a.bounce[s] == 0u) pc.primary++; else pc.extension++; }
+                    rng = Rng4{x, y, fa.seed_b, kAnyHit ? a.wb[s] : 0u};
+                    tr.begin(fa.tlas, o, d, float(kTMin), float(a.tfar[s]), opq, stack, kWfBlock);
+                }
                 active = true;
             }
         }
@@ -241,15 +291,28 @@ void k_wf_extend(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in,
             if (active) {
                 uint32_t xx, yy;
                 const FrameArgs& fa = fr.at(s, xx, yy);
-                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
-                if (tr.step(fa.tlas, RayFromSlot{a, s}, ah, stack, kWfBlock, pc.tr)) {
-                    if (kAnyHit) a.wb[s] = rb.w;
-                    a.h_inst[s] = tr.found ? tr.hit.inst : kEmpty;
-                    a.h_prim[s] = tr.hit.prim;
-                    a.h_rec[s] = tr.hit.rec;
-                    a.h_t[s] = tr.hit.t;
-                    a.h_u[s] = tr.hit.u;
-                    a.h_v[s] = tr.hit.v;
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rng};
+                if (tr.step(fa.tlas, RayFromSlotOrShadow{a, s, shadow}, ah, stack, kWfBlock, ts)) {
+                    if (shadow) {
+                        a.unocc[s] = tr.found ? 0u : 1u;
+                    } else {
+                        if (kAnyHit) a.wb[s] = rng.w;
+                        a.h_inst[s] = tr.found ? tr.hit.inst : kEmpty;
+                        a.h_prim[s] = tr.hit.prim;
+                        a.h_rec[s] = tr.hit.rec;
+                        a.h_t[s] = tr.hit.t;
+                        a.h_u[s] = tr.hit.u;
+                        a.h_v[s] = tr.hit.v;
+                    }
+                    if (kStats) {
+                        TraceStats& to = shadow ? pc.trs : pc.tr;
+                        to.nodes += ts.nodes;
+                        to.tris += ts.tris;
+                        to.instances += ts.instances;
+                        to.anyhits += ts.anyhits;
+                        to.overflows += ts.overflows;
+                        ts = TraceStats{};
+                    }
                     active = false;
                 }
             }
@@ -258,22 +321,49 @@ void k_wf_extend(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in,
     if (kStats) flush_counters(fr.counters(), pc);
 }
 
+// Adds the NEE radiance of the slot's last shaded bounce, now that its shadow ray
+// has been traced (the radiance update of rgen:98-103 for that bounce).
+__device__ inline void add_pending(const WfArrays& a, uint32_t s, uint32_t f, D3& rad) {
+    if (f & kFlagShadow) {
+        rad = rad + (a.unocc[s] ? d3(a.pwx[s], a.pwy[s], a.pwz[s]) : d3(a.pox[s], a.poy[s], a.poz[s]));
+    }
+}
+
+// Iteration k's shading of path queue q: pending radiance, then the shaders for the
+// slot's traced ray; appends the next path queue and the next shadow-ray queue.
+// The retiring entries of the shadow queue the trace just served (qs_in) get their
+// last radiance and are written.
 template <class Frames, bool kStats>
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in) {
-    const uint32_t n = *n_in;
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in,
+                                                        const uint32_t* qs_in, const uint32_t* ns_in, uint32_t* q_out,
+                                                        uint32_t* n_out, uint32_t* qs_out, uint32_t* ns_out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[kCursor] = 0u;  // next trace's fetch cursor
+    const uint32_t n_e = *n_in, n = n_e + *ns_in;
     PathCounters pc;
     for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
         uint32_t i = base + threadIdx.x;
-        bool active = i < n;
-        BounceOut r;
-        r.shadow = false;
+        bool next = false, want_shadow = false, retire = false;
         uint32_t s = 0;
-        if (active) {
+        if (i >= n_e && i < n) {
+            const uint32_t entry = qs_in[i - n_e];
+            if (entry & kRetireBit) {
+                // the pixel's last bounce: its radiance, then the pixel
+                s = entry & ~kRetireBit;
+                uint32_t x, y;
+                const FrameArgs& fa = fr.at(s, x, y);
+                D3 rad = d3(a.rx[s], a.ry[s], a.rz[s]);
+                add_pending(a, s, kFlagShadow, rad);
+                const bool dist = fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE;
+                write_accum(fa, x, y, rad, dist ? a.last_t[s] : 0.f);
+            }
+        } else if (i < n_e) {
             s = q[i];
             uint32_t x, y;
             const FrameArgs& fa = fr.at(s, x, y);
+            const uint32_t f = a.flags[s];
             PathState p;
             load_full(a, s, p);
+            add_pending(a, s, f, p.rad);
             HitInfo h;
             h.inst = a.h_inst[s];
             bool found = h.inst != kEmpty;
@@ -282,135 +372,82 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, co
             h.t = a.h_t[s];
             h.u = a.h_u[s];
             h.v = a.h_v[s];
+            BounceOut r;
             shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
-            store_full(a, s, p);
-            if (fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE) a.last_t[s] = p.last_t;
-            a.pwx[s] = r.pend_with.x; a.pwy[s] = r.pend_with.y; a.pwz[s] = r.pend_with.z;
-            a.pox[s] = r.pend_without.x; a.poy[s] = r.pend_without.y; a.poz[s] = r.pend_without.z;
-            a.flags[s] = (r.shadow ? kFlagShadow : 0u) | (r.cont ? kFlagCont : 0u);
-        }
-        bool want = active && r.shadow;
-        uint32_t idx = wave_append(want, &a.counts[2]);
-        if (want) {
-            float3 so = to_f3(r.so), sd = to_f3(r.sd);
-            a.sq_o[idx] = make_float4(so.x, so.y, so.z, float(r.sdist));
-            a.sq_d[idx] = make_float4(sd.x, sd.y, sd.z, __uint_as_float(s));
-            a.sq_wb[idx] = r.shadow_wb;
-        }
-    }
-    if (kStats) flush_counters(fr.counters(), pc);
-}
-
-template <class Frames, bool kAnyHit, bool kStats, int kDepth, int kWaves>
-__global__ __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(kAnyHit ? 1 : kWaves)))
-void k_wf_shadow(Frames fr, WfArrays a) {
-    __shared__ uint32_t s_stack[kDepth * kWfBlock];
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[3] = 0u;  // next extend's fetch cursor
-    const uint32_t n = a.counts[2];
-    uint32_t* const stack = s_stack + threadIdx.x;
-    PathCounters pc;
-    Traversal<kAnyHit, kStats, kDepth> tr;
-    uint32_t s = 0, x = 0, y = 0;
-    Rng4 shadow_rng{0u, 0u, 0u, 0u};
-    bool active = false, exhausted = false;
-    for (;;) {
-        if (!exhausted) {
-            uint32_t i = wave_fetch(!active, a.counts + 4, n, exhausted);
-            if (i != kEmpty) {
-                float4 o = a.sq_o[i], d = a.sq_d[i];
-                s = __float_as_uint(d.w);
-                const FrameArgs& fa = fr.at(s, x, y);
-                const uint32_t flags = kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u);
-                if (kStats) pc.shadow++;
-                shadow_rng = Rng4{x, y, fa.seed_b, a.sq_wb[i]};
-                tr.begin(fa.tlas, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), float(kShadowTMin), o.w,
-                         flags, stack, kWfBlock);
-                active = true;
-            }
-        }
-        if (!__ballot(active)) break;
-        const uint32_t keep = exhausted ? 1u : a.refill_sh;
-        do {
-            if (active) {
-                uint32_t xx, yy;
-                const FrameArgs& fa = fr.at(s, xx, yy);
-                AlphaAnyHit sah{&fa.sv, &fa.tlas, &shadow_rng};
-                if (tr.step(fa.tlas, RayFromF32{tr.o, tr.d}, sah, stack, kWfBlock, pc.trs)) {
-                    a.unocc[s] = tr.found ? 0u : 1u;
-                    active = false;
-                }
-            }
-        } while (uint32_t(__popcll(__ballot(active))) >= keep);
-    }
-    if (kStats) flush_counters(fr.counters(), pc);
-}
-
-template <class Frames>
-__global__ __launch_bounds__(kWfBlock) void k_wf_resolve(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_in,
-                                                          uint32_t* q_out, uint32_t* n_out) {
-    const uint32_t n = *n_in;
-    for (uint32_t base = blockIdx.x * kWfBlock; base < n; base += gridDim.x * kWfBlock) {
-        uint32_t i = base + threadIdx.x;
-        bool alive = false;
-        uint32_t s = 0;
-        if (i < n) {
-            s = q[i];
-            uint32_t x, y;
-            const FrameArgs& fa = fr.at(s, x, y);
-            uint32_t f = a.flags[s];
-            bool unocc = (f & kFlagShadow) && a.unocc[s];
-            D3 rad = d3(a.rx[s], a.ry[s], a.rz[s]);
-            D3 pend = unocc ? d3(a.pwx[s], a.pwy[s], a.pwz[s]) : d3(a.pox[s], a.poy[s], a.poz[s]);
-            rad = rad + pend;
-            a.rx[s] = rad.x; a.ry[s] = rad.y; a.rz[s] = rad.z;
-            if (f & kFlagCont) {
-                alive = true;
+            uint32_t nf = 0u;
+            if (r.shadow) {
+                // its radiance waits for the shadow ray (next trace, next shade)
+                a.pwx[s] = r.pend_with.x; a.pwy[s] = r.pend_with.y; a.pwz[s] = r.pend_with.z;
+                a.pox[s] = r.pend_without.x; a.poy[s] = r.pend_without.y; a.poz[s] = r.pend_without.z;
+                const float3 so = to_f3(r.so), sd = to_f3(r.sd);
+                a.sq_o[s] = make_float4(so.x, so.y, so.z, float(r.sdist));
+                a.sq_d[s] = make_float4(sd.x, sd.y, sd.z, 0.f);
+                a.sq_wb[s] = r.shadow_wb;
+                nf = kFlagShadow;
+                want_shadow = true;
             } else {
-                uint32_t sample = a.sample[s] + 1u;
-                a.sample[s] = sample;
-                if (sample < fa.samples) {
-                    PathState p;
-                    p.wa = a.wa[s];
-                    p.lcg = a.lcg[s];
-                    begin_sample(fa, x, y, p);
-                    store_ray(a, s, p);
-                    a.wa[s] = p.wa;
-                    a.lcg[s] = p.lcg;
-                    a.tx[s] = p.thr.x; a.ty[s] = p.thr.y; a.tz[s] = p.thr.z;
-                    a.vol[s] = p.volume_dis;
-                    a.tfar[s] = p.tfar;
-                    a.bounce[s] = 0u;
-                    alive = true;
-                } else {
-                    const bool dist = fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE;
-                    write_accum(fa, x, y, rad, dist ? a.last_t[s] : 0.f);
-                }
+                p.rad = p.rad + r.pend_without;
             }
+            if (fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE) a.last_t[s] = p.last_t;
+            if (r.cont) {
+                next = true;
+            } else if (++p.sample < fa.samples) {
+                begin_sample(fa, x, y, p);  // the pixel's next sample
+                next = true;
+            } else if (nf) {
+                retire = true;  // written once its shadow ray is traced
+            } else {
+                write_accum(fa, x, y, p.rad, p.last_t);
+            }
+            store_full(a, s, p);
+            a.flags[s] = nf;
         }
-        uint32_t idx = wave_append(alive, n_out);
-        if (alive) q_out[idx] = s;
+        uint32_t idx = wave_append(next, n_out);
+        if (next) q_out[idx] = s;
+        uint32_t sidx = wave_append(want_shadow, ns_out);
+        if (want_shadow) qs_out[sidx] = s | (retire ? kRetireBit : 0u);
     }
+    if (kStats) flush_counters(fr.counters(), pc);
 }
 
 // The tail of a trace: once few paths are alive, each iteration's launches last as
-// long as their slowest ray while most of the GPU idles.  This kernel takes the live
-// paths of queue 0 and runs each one, in one thread, through the rest of its pixel's
-// samples (the megakernel's loop from the slot's state), then writes the pixel.
+// long as their slowest ray while most of the GPU idles.  This kernel takes, after a
+// shade, the paths of path queue q and the retiring slots of shadow queue qs, and
+// runs each one, in one thread, through its pending shadow ray and the rest of its
+// pixel's samples (the megakernel's loop from the slot's state), then writes the
+// pixel.
 constexpr int kFinishBlock = 128;
 template <class Frames, bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kFinishBlock) void k_wf_finish(Frames fr, WfArrays a, const uint32_t* q,
-                                                            const uint32_t* n_in) {
+                                                            const uint32_t* n_e_in, const uint32_t* qs,
+                                                            const uint32_t* n_s_in) {
     __shared__ uint32_t s_stack[kStackDepth * kFinishBlock];
-    const uint32_t n = *n_in;
+    const uint32_t n_e = *n_e_in, n = n_e + *n_s_in;
     PathCounters pc;
     for (uint32_t i = blockIdx.x * kFinishBlock + threadIdx.x; i < n; i += gridDim.x * kFinishBlock) {
-        const uint32_t s = q[i];
+        const uint32_t entry = i < n_e ? q[i] : qs[i - n_e];
+        if (i >= n_e && !(entry & kRetireBit)) continue;  // a path queue entry as well
+        const bool retire = i >= n_e;
+        const uint32_t s = entry & ~kRetireBit;
+        const uint32_t f = a.flags[s];
         uint32_t x, y;
         const FrameArgs& fa = fr.at(s, x, y);
         PathState p;
         load_full(a, s, p);
-        p.last_t = 0.f;
-        run_samples<kAnyHit, kStats>(fa, x, y, p, true, s_stack + threadIdx.x, kFinishBlock, pc);
+        if (f & kFlagShadow) {
+            // the last shaded bounce's shadow ray, as the trace kernel would take it
+            const float4 o = a.sq_o[s], d = a.sq_d[s];
+            BounceOut r;
+            r.so = d3(make_float3(o.x, o.y, o.z));
+            r.sd = d3(make_float3(d.x, d.y, d.z));
+            r.sdist = double(o.w);
+            r.shadow_wb = a.sq_wb[s];
+            const bool unocc = shadow_unoccluded<kAnyHit, kStats>(fa, x, y, r, s_stack + threadIdx.x, kFinishBlock, pc);
+            p.rad = p.rad + (unocc ? d3(a.pwx[s], a.pwy[s], a.pwz[s]) : d3(a.pox[s], a.poy[s], a.poz[s]));
+        }
+        const bool dist = fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE;
+        p.last_t = dist ? a.last_t[s] : 0.f;
+        if (!retire) run_samples<kAnyHit, kStats>(fa, x, y, p, true, s_stack + threadIdx.x, kFinishBlock, pc);
         write_accum(fa, x, y, p.rad, p.last_t);
     }
     if (kStats) flush_counters(fr.counters(), pc);
@@ -430,7 +467,7 @@ size_t layout(WfArrays& a, char* base, uint32_t n) {
                         &a.hx, &a.hy, &a.hz, &a.vol, &a.tfar, &a.pwx, &a.pwy, &a.pwz, &a.pox, &a.poy, &a.poz};
     for (double** d : dptrs) total += carve(p, *d, n);
     uint32_t** uptrs[] = {&a.wa, &a.wb, &a.lcg, &a.sample, &a.bounce, &a.h_inst, &a.h_prim, &a.h_rec, &a.flags,
-                          &a.unocc, &a.sq_wb, &a.q0, &a.q1};
+                          &a.unocc, &a.sq_wb, &a.q0, &a.q1, &a.qs0, &a.qs1};
     for (uint32_t** u : uptrs) total += carve(p, *u, n);
     total += carve(p, a.h_t, n);
     total += carve(p, a.h_u, n);
@@ -450,7 +487,7 @@ WfArrays slice(const WfArrays& a, uint32_t base, uint32_t* counts) {
                     &r.hx, &r.hy, &r.hz, &r.vol, &r.tfar, &r.pwx, &r.pwy, &r.pwz, &r.pox, &r.poy, &r.poz};
     for (double** x : d) *x += base;
     uint32_t** u[] = {&r.wa, &r.wb, &r.lcg, &r.sample, &r.bounce, &r.h_inst, &r.h_prim, &r.h_rec, &r.flags,
-                      &r.unocc, &r.sq_wb, &r.q0, &r.q1};
+                      &r.unocc, &r.sq_wb, &r.q0, &r.q1, &r.qs0, &r.qs1};
     for (uint32_t** x : u) *x += base;
     r.h_t += base;
     r.h_u += base;
@@ -467,41 +504,28 @@ __global__ void k_zero_counts(uint32_t* c) {
 }
 
 template <class Frames> struct Kernels {
-    using ExtendFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*, uint32_t*, uint32_t*);
-    using FinishFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*);
+    using TraceFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
+                             uint32_t*, uint32_t*);
+    using FinishFn = void (*)(Frames, WfArrays, const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*);
     static FinishFn finish(bool ah, bool st) {
         return ah ? (st ? k_wf_finish<Frames, true, true> : k_wf_finish<Frames, true, false>)
                   : (st ? k_wf_finish<Frames, false, true> : k_wf_finish<Frames, false, false>);
     }
-    using ShadowFn = void (*)(Frames, WfArrays);
-    template <int V> static ExtendFn extend_v(bool ah, bool st) {
+    template <int V> static TraceFn trace_v(bool ah, bool st) {
         constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
-        return ah ? (st ? k_wf_extend<Frames, true, true, D, W> : k_wf_extend<Frames, true, false, D, W>)
-                  : (st ? k_wf_extend<Frames, false, true, D, W> : k_wf_extend<Frames, false, false, D, W>);
+        return ah ? (st ? k_wf_trace<Frames, true, true, D, W> : k_wf_trace<Frames, true, false, D, W>)
+                  : (st ? k_wf_trace<Frames, false, true, D, W> : k_wf_trace<Frames, false, false, D, W>);
     }
-    template <int V> static ShadowFn shadow_v(bool ah, bool st) {
-        constexpr int D = kTravVariants[V].depth, W = kTravVariants[V].waves;
-        return ah ? (st ? k_wf_shadow<Frames, true, true, D, W> : k_wf_shadow<Frames, true, false, D, W>)
-                  : (st ? k_wf_shadow<Frames, false, true, D, W> : k_wf_shadow<Frames, false, false, D, W>);
-    }
-    static ExtendFn extend(int v, bool ah, bool st) {
+    static TraceFn trace(int v, bool ah, bool st) {
         switch (v) {
-            case 1: return extend_v<1>(ah, st);
-            case 2: return extend_v<2>(ah, st);
-            case 3: return extend_v<3>(ah, st);
-            default: return extend_v<0>(ah, st);
-        }
-    }
-    static ShadowFn shadow(int v, bool ah, bool st) {
-        switch (v) {
-            case 1: return shadow_v<1>(ah, st);
-            case 2: return shadow_v<2>(ah, st);
-            case 3: return shadow_v<3>(ah, st);
-            default: return shadow_v<0>(ah, st);
+            case 1: return trace_v<1>(ah, st);
+            case 2: return trace_v<2>(ah, st);
+            case 3: return trace_v<3>(ah, st);
+            default: return trace_v<0>(ah, st);
         }
     }
 };
-static_assert(kNumTravVariants == 4, "Kernels::extend / shadow list every variant");
+static_assert(kNumTravVariants == 4, "Kernels::trace lists every variant");
 
 }  // namespace
 
@@ -521,7 +545,7 @@ struct WavefrontState {
     hipGraphExec_t exec[kMaxStreams] = {};
     hipStream_t set_stream[kMaxStreams] = {};  // streams of slot sets 1.. (set 0 uses the caller's)
     hipEvent_t fork = nullptr, join[kMaxStreams] = {}, counted[kMaxStreams][2] = {};
-    uint32_t* host_live = nullptr;  // pinned: [set][parity] live-path counts read back
+    uint32_t* host_live = nullptr;  // pinned: [set][parity][4] queue lengths read back
     int cus = 0;
     std::unordered_map<const void*, uint32_t> grids;  // persistent grid per kernel (occupancy x CUs)
     // grid of a persistent launch of kernel `f`: as many blocks as can be resident
@@ -595,7 +619,7 @@ int ensure_slots(WavefrontState* ws, uint32_t nslots, std::string* err) {
 // Streams, events and pinned read-back words for `n` slot sets.
 int ensure_streams(WavefrontState* ws, int n, std::string* err) {
     hipError_t e = hipSuccess;
-    if (!ws->host_live) e = hipHostMalloc((void**)&ws->host_live, sizeof(uint32_t) * 2 * kMaxStreams, hipHostMallocDefault);
+    if (!ws->host_live) e = hipHostMalloc((void**)&ws->host_live, sizeof(uint32_t) * 8 * kMaxStreams, hipHostMallocDefault);
     if (e == hipSuccess && !ws->fork) e = hipEventCreateWithFlags(&ws->fork, hipEventDisableTiming);
     for (int g = 0; g < n && e == hipSuccess; g++) {
         if (g > 0 && !ws->set_stream[g]) e = hipStreamCreateWithFlags(&ws->set_stream[g], hipStreamNonBlocking);
@@ -633,42 +657,44 @@ int run_wavefront(WavefrontState* ws, const std::vector<SlotSet<Frames>>& sets, 
     int rc = ensure_streams(ws, nsets, err);
     if (rc) return rc;
     using K = Kernels<Frames>;
-    const typename K::ExtendFn k_extend = K::extend(variant, any_hit, stats);
-    const typename K::ShadowFn k_shadow = K::shadow(variant, any_hit, stats);
+    const typename K::TraceFn k_trace = K::trace(variant, any_hit, stats);
     const void* k_shade = stats ? (const void*)k_wf_shade<Frames, true> : (const void*)k_wf_shade<Frames, false>;
-    const uint32_t g_extend = ws->grid_for((const void*)k_extend);
-    const uint32_t g_shadow = ws->grid_for((const void*)k_shadow);
+    const uint32_t g_trace = ws->grid_for((const void*)k_trace);
     const uint32_t g_shade = ws->grid_for(k_shade);
-    const uint32_t g_resolve = ws->grid_for((const void*)k_wf_resolve<Frames>);
+    const uint32_t g_init = ws->grid_for((const void*)k_wf_init<Frames>);
     hipError_t e = hipSuccess;
     auto stream_of = [&](int g) { return g == 0 ? st : ws->set_stream[g]; };
-    // stage marks (one slot set only): 0 extend, 1 shade, 2 shadow, 3 resolve (init counts as resolve)
+    // stage marks (one slot set only): 0 trace, 1 shade, 3 init
     auto mark = [&](int stage) {
         if (tm && e == hipSuccess) e = tm->mark(st, stage);
     };
-    // one iteration of set g: extend, shade, shadow, resolve, with the path queues of parity k
+    // iteration k of set g: trace, shade, with the path queues of parity k; the
+    // shadow queue the trace reads is the one the previous shade wrote
     auto iteration = [&](const SlotSet<Frames>& set, uint64_t k, hipStream_t s, bool marks) {
         const WfArrays& a = set.a;
         uint32_t* q_in = (k & 1) ? a.q1 : a.q0;
         uint32_t* q_out = (k & 1) ? a.q0 : a.q1;
         uint32_t* n_in = a.counts + (k & 1);
         uint32_t* n_out = a.counts + ((k + 1) & 1);
-        hipLaunchKernelGGL(k_extend, dim3(g_extend), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in, n_out, a.counts + 2);
+        uint32_t* qs_in = (k & 1) ? a.qs0 : a.qs1;  // parity of k - 1
+        uint32_t* qs_out = (k & 1) ? a.qs1 : a.qs0;
+        uint32_t* s_in = shadow_count(a, k + 1);
+        uint32_t* s_out = shadow_count(a, k);
+        hipLaunchKernelGGL(k_trace, dim3(g_trace), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in, qs_in, s_in, n_out, s_out);
         if (marks) mark(0);
-        if (stats) hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in);
-        else hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in);
+        if (stats)
+            hipLaunchKernelGGL((k_wf_shade<Frames, true>), dim3(g_shade), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in,
+                               qs_in, s_in, q_out, n_out, qs_out, s_out);
+        else
+            hipLaunchKernelGGL((k_wf_shade<Frames, false>), dim3(g_shade), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in,
+                               qs_in, s_in, q_out, n_out, qs_out, s_out);
         if (marks) mark(1);
-        hipLaunchKernelGGL(k_shadow, dim3(g_shadow), dim3(kWfBlock), 0, s, set.fr, a);
-        if (marks) mark(2);
-        hipLaunchKernelGGL(k_wf_resolve<Frames>, dim3(g_resolve), dim3(kWfBlock), 0, s, set.fr, a, q_in, n_in, q_out,
-                           n_out);
-        if (marks) mark(3);
     };
     static_assert(kCheckEvery % 2 == 0, "a chunk ends on queue 0");
     const bool graph = ws->use_graph && !tm;
     if (graph) {
         // the kCheckEvery iterations of a chunk of each set captured once as a HIP
-        // graph (one launch per chunk instead of 32)
+        // graph (one launch per chunk instead of 16)
         if (!ws->cap_stream) {
             e = hipStreamCreateWithFlags(&ws->cap_stream, hipStreamNonBlocking);
             if (e != hipSuccess) {
@@ -717,7 +743,7 @@ int run_wavefront(WavefrontState* ws, const std::vector<SlotSet<Frames>>& sets, 
     for (int g = 0; g < nsets; g++) {
         const SlotSet<Frames>& set = sets[g];
         hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, stream_of(g), set.a.counts);
-        hipLaunchKernelGGL(k_wf_init<Frames>, dim3(std::min(g_resolve, (set.nslots + kWfBlock - 1) / kWfBlock)),
+        hipLaunchKernelGGL(k_wf_init<Frames>, dim3(std::min(g_init, (set.nslots + kWfBlock - 1) / kWfBlock)),
                            dim3(kWfBlock), 0, stream_of(g), set.fr, set.a, set.nslots);
     }
     mark(3);
@@ -741,7 +767,7 @@ int run_wavefront(WavefrontState* ws, const std::vector<SlotSet<Frames>>& sets, 
                 if (e == hipSuccess) e = hipGetLastError();
             }
             if (e == hipSuccess)
-                e = hipMemcpyAsync(ws->host_live + 2 * g + (c & 1), sets[g].a.counts, sizeof(uint32_t),
+                e = hipMemcpyAsync(ws->host_live + 8 * g + 4 * (c & 1), sets[g].a.counts, 4 * sizeof(uint32_t),
                                    hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipEventRecord(ws->counted[g][c & 1], s);
             if (e != hipSuccess) {
@@ -760,7 +786,11 @@ int run_wavefront(WavefrontState* ws, const std::vector<SlotSet<Frames>>& sets, 
                 rc = int(e);
                 break;
             }
-            const uint32_t live = ws->host_live[2 * g + ((c - 1) & 1)];
+            // after an even number of iterations: the path queue (counts[0]) and the
+            // shadow queue of the last shade (odd parity), whose retiring entries
+            // are pixels still to be written
+            const uint32_t* hl = ws->host_live + 8 * g + 4 * ((c - 1) & 1);
+            const uint32_t live = hl[0] + hl[3];
             if (live == 0u) {
                 alive[g] = false;
             } else if (live < ws->finish_below && !tm) {
@@ -768,7 +798,7 @@ int run_wavefront(WavefrontState* ws, const std::vector<SlotSet<Frames>>& sets, 
                 const typename K::FinishFn k_finish = K::finish(any_hit, stats);
                 const uint32_t blocks = std::min((live + kFinishBlock - 1) / kFinishBlock, 65535u);
                 hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kFinishBlock), 0, stream_of(g), sets[g].fr, sets[g].a,
-                                   sets[g].a.q0, sets[g].a.counts);
+                                   sets[g].a.q0, sets[g].a.counts, sets[g].a.qs1, shadow_count(sets[g].a, 1));
                 e = hipGetLastError();
                 if (e != hipSuccess) {
                     *err = "launch k_wf_finish";
