@@ -26,10 +26,12 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {"node": 64, "tri": 48, "instance": 128, "shade": 668, "anyhit": 668, "tex": 16, "light": 48, "pixel": 32}
 
 
-# per closest-hit ray in the extend kernel: queue index (4 B), fp64 origin +
-# direction + tmax (56 B), hit record written (instance, primitive, record, t, u, v:
-# 24 B)
+# own record of each ray the trace kernel serves.  Extension ray: queue index (4 B),
+# fp64 origin + direction + tmax read (56 B), hit record written (instance,
+# primitive, record, t, u, v: 24 B).  Shadow ray: queue entry (4 B), fp32 origin +
+# tmax and direction (32 B) and stream-B counter (4 B) read, result written (4 B).
 EXTEND_RAY_BYTES = 84
+SHADOW_RAY_BYTES = 44
 
 
 def algorithmic_bytes(st, pixels):
@@ -40,15 +42,14 @@ def algorithmic_bytes(st, pixels):
             st["light_candidates"] * BYTES["light"] + pixels * BYTES["pixel"])
 
 
-def extend_bytes(st):
-    """Bytes of the closest-hit traversal of one frame (all k_wf_extend launches):
-    the nodes, triangles and instances its rays visit plus each ray's own record.
-    Any-hit work is absent from this workload (every material is opaque)."""
-    rays = st["primary_rays"] + st["extension_rays"]
-    return ((st["node_visits"] - st["shadow_node_visits"]) * BYTES["node"] +
-            (st["tri_tests"] - st["shadow_tri_tests"]) * BYTES["tri"] +
-            (st["instance_visits"] - st["shadow_instance_visits"]) * BYTES["instance"] +
-            rays * EXTEND_RAY_BYTES)
+def trace_bytes(st):
+    """Bytes of the traversal of one frame (all k_wf_trace launches): the nodes,
+    triangles and instances its extension and shadow rays visit plus each ray's
+    own record.  Any-hit work is absent from this workload (every material is
+    opaque)."""
+    return (st["node_visits"] * BYTES["node"] + st["tri_tests"] * BYTES["tri"] +
+            st["instance_visits"] * BYTES["instance"] +
+            (st["primary_rays"] + st["extension_rays"]) * EXTEND_RAY_BYTES + st["shadow_rays"] * SHADOW_RAY_BYTES)
 
 
 def measured_traffic(kernel):
@@ -265,7 +266,7 @@ def main():
         ms_per_step = wall * 1e3 / args.steps
         # stage split, after the timed frames: HIP events around every stage kernel on
         # the stream the kernels run on (one slot set and no finishing launch, so all
-        # closest-hit traversal runs in k_wf_extend launches)
+        # traversal runs in k_wf_trace launches)
         ctx.set_option(rt.OPTION_STAGE_TIMING, 1)
         stages = []
         for k in range(max(1, min(args.steps, 2))):
@@ -281,10 +282,11 @@ def main():
     st = r["st"]
     stage = r["stage"]
     if best == "wavefront":
-        # dominant kernel: the closest-hit traversal, launched once per iteration
-        kernel_name = "k_wf_extend"
+        # dominant kernel: the traversal (extension and shadow rays), launched once
+        # per iteration
+        kernel_name = "k_wf_trace"
         launches = max(1, int(stage["iterations"]))
-        bytes_per_launch = extend_bytes(st) / launches
+        bytes_per_launch = trace_bytes(st) / launches
         launch_ms = stage["extend_ms"] / launches
     else:
         kernel_name = "k_megakernel"
