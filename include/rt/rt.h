@@ -94,6 +94,33 @@ int rt_build_blas(rt_ctx* ctx, const rt_blas_input* input, rt_blas* out);
 int rt_blas_device_address(rt_ctx* ctx, rt_blas blas, uint64_t* out);
 void rt_destroy_blas(rt_ctx* ctx, rt_blas blas);
 
+/* ------------------------------------------------------ animated scenes */
+
+/* Skinning of the vertex buffer for an animation step: the compute pass the
+ * reference dispatches before rebuilding its acceleration structures
+ * (crates/examples/gltf_viewer/shaders/AnimationCompute.comp:14-38, run by
+ * crates/examples/gltf_viewer/src/main.rs:383-386).  d_out[i] = d_in[i], and for
+ * skin_index >= 0: transform = sum of weights[k] * bones[joints[k]] of that skin;
+ * pos = transform * (pos, 1); normal = normalize(transform * (normal, 0));
+ * tangent = normalize(transform * (tangent.xyz, 0)) with tangent.w kept.  d_skins
+ * holds num_skins * 256 column-major mat4 (the reference's Skin array).  d_in may be
+ * d_out; both 16-byte aligned device buffers of `count` vertices.  Enqueued on the
+ * context stream; a skin index or joint out of range leaves that vertex unskinned
+ * and makes the next rt_sync return RT_ERR_INVALID_ARGUMENT. */
+int rt_skin_vertices(rt_ctx* ctx, const rt_vertex* d_in, rt_vertex* d_out, uint32_t count,
+                     const float* d_skins, uint32_t num_skins);
+
+/* Refit of the BLASes of one rt_build_blas_batch call (all its handles, in its
+ * order) to new vertex data with the same triangles: the reference rebuilds its
+ * BLASes from the skinned vertex buffer every animation step (main.rs:388-396 ->
+ * create_as); a refit keeps the tree and recomputes its boxes bottom-up, which is
+ * faster and gives the same hits (a looser tree if the mesh deforms far).
+ * inputs[i] must have the build's primitive_count and describe the same triangles
+ * (vertex_data may be another buffer, e.g. the skinned one).  TLASes built over
+ * these BLASes must be rebuilt afterwards (rt_build_tlas), as the reference rebuilds
+ * its TLAS.  Synchronises the context stream. */
+int rt_refit_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count, const rt_blas* blases);
+
 /* --------------------------------------------------- top-level (instances) */
 
 /* Builds a TLAS over `count` instances held in device memory in the 64-byte

@@ -94,6 +94,7 @@ def lib():
             "orc_render": (i, [vp, vp, u, u, u, u, u, u, i, i, vp, vp]),
             "orc_tone_map": (C.c_float, [C.c_float, u]), "orc_heatmap": (None, [C.c_float, vp]),
             "orc_output_image": (None, [vp, u, vp, vp, vp, i, vp]),
+            "orc_skin_vertices": (u, [vp, vp, u, vp, u]),
         }
         for name, (res, args) in sig.items():
             f = getattr(_lib, name)
@@ -267,3 +268,15 @@ def output_image(accum, ubo, last_t=None, ticks=None, bgra=False):
     lib().orc_output_image(acc.ctypes.data, n, lt.ctypes.data if lt is not None else None,
                            tk.ctypes.data if tk is not None else None, ubo.ctypes.data, int(bool(bgra)), out.ctypes.data)
     return out.view(np.uint8).reshape(acc.shape[:-1] + (4,))
+
+
+def skin_vertices(vertices, skins):
+    """orc_skin_vertices: (skinned copy of a VERTEX array, count left unskinned).
+    skins: float32 (num_skins, 256, 4, 4), skins[s, j, c] = column c of mat4 j."""
+    from rustracer_amd import layouts as L
+    v = np.ascontiguousarray(vertices, L.VERTEX)
+    out = np.zeros_like(v)
+    sk = None if skins is None else np.ascontiguousarray(skins, np.float32)
+    bad = lib().orc_skin_vertices(v.ctypes.data, out.ctypes.data, len(v), sk.ctypes.data if sk is not None else None,
+                                  0 if sk is None else sk.shape[0])
+    return out, int(bad)

@@ -1600,4 +1600,47 @@ void orc_output_image(const float* accum, uint32_t n, const float* last_t, const
     }
 }
 
+// ------------------------------------------------------------ skinning
+// AnimationCompute.comp:14-38 in fp64: transform = sum_k weights[k] * bones[joints[k]]
+// (column-major mat4), pos = T (pos, 1), normal = normalize(T (normal, 0)).xyz,
+// tangent = normalize(T (tangent.xyz, 0)) with tangent.w kept.  Vertices with
+// skin_index < 0 are copied; returns the number of vertices whose skin index or
+// joints are out of range (copied unchanged).
+uint32_t orc_skin_vertices(const rt_vertex* in, rt_vertex* out, uint32_t n, const float* skins, uint32_t num_skins) {
+    uint32_t bad = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        rt_vertex v = in[i];
+        if (v.skin_index >= 0) {
+            bool ok = uint32_t(v.skin_index) < num_skins;
+            for (int k = 0; k < 4; k++) ok = ok && v.joints[k] < 256u;
+            if (!ok) {
+                bad++;
+            } else {
+                double T[16] = {0};
+                for (int k = 0; k < 4; k++) {
+                    const float* B = skins + (size_t(v.skin_index) * 256 + v.joints[k]) * 16;
+                    for (int e = 0; e < 16; e++) T[e] += double(v.weights[k]) * double(B[e]);
+                }
+                auto apply = [&](const float* x, double w, double r[4]) {
+                    for (int row = 0; row < 4; row++)
+                        r[row] = T[row] * x[0] + T[4 + row] * x[1] + T[8 + row] * x[2] + T[12 + row] * w;
+                };
+                double p[4], nn[4], t[4];
+                apply(v.pos, 1.0, p);
+                apply(v.normal, 0.0, nn);
+                apply(v.tangent, 0.0, t);
+                const double nl = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+                const double tl = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + t[3] * t[3]);
+                for (int k = 0; k < 3; k++) {
+                    v.pos[k] = float(p[k]);
+                    v.normal[k] = float(nn[k] / nl);
+                    v.tangent[k] = float(t[k] / tl);
+                }
+            }
+        }
+        out[i] = v;
+    }
+    return bad;
+}
+
 }  // extern "C"

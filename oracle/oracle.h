@@ -132,6 +132,10 @@ int orc_render(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
                uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, int precision,
                int threads, double* accum, orc_stats* stats);
 
+/* --- skinning (AnimationCompute.comp:14-38), fp64; returns the count of vertices
+ * left unskinned because their skin index or a joint is out of range --- */
+uint32_t orc_skin_vertices(const rt_vertex* in, rt_vertex* out, uint32_t n, const float* skins, uint32_t num_skins);
+
 /* --- display image (RayTracing.rgen:132-166), fp32 --- */
 float orc_tone_map(float c, uint32_t mode);
 void orc_heatmap(float t, float out[3]);

@@ -13,8 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librt_gfx950.so")
-SOURCES = ["api.hip", "lbvh.hip", "render.hip", "wavefront.hip", "adjoint.hip"]
-HEADERS = ["common.h", "lbvh.h", "traverse.h", "shade.h", "render.h", "path.h", "wavefront.h"]
+SOURCES = ["api.hip", "lbvh.hip", "render.hip", "wavefront.hip", "adjoint.hip", "skin.hip"]
+HEADERS = ["common.h", "lbvh.h", "traverse.h", "shade.h", "render.h", "path.h", "wavefront.h", "output.h", "skin.h",
+           "brdf_dual.h", "adjoint.h"]
 PUBLIC = [os.path.join(ROOT, "include", "rt", "rt.h"), os.path.join(ROOT, "include", "rt", "rt_layout.h")]
 
 # Packed fp32 instructions are disabled: hipcc would otherwise form them from

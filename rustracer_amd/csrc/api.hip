@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "lbvh.h"
+#include "skin.h"
 #include "render.h"
 #include "wavefront.h"
 #include "adjoint.h"
@@ -39,6 +40,9 @@ using DevPtr = std::unique_ptr<void, DevFree>;
 struct BlasPool {
     DevPtr nodes, tris, descs;
     size_t bytes = 0;
+    uint32_t count = 0, num_nodes = 0, num_tris = 0;
+    std::vector<uint32_t> prim_count;   // per BLAS, in build order
+    std::vector<uint32_t> level_start;  // node levels of the pool's trees (refit)
 };
 
 struct BlasEntry {
@@ -250,6 +254,34 @@ int make_frame(rt_ctx* ctx, const TlasEntry& tl, const rt_bindings* b, float* ac
     return RT_OK;
 }
 
+// Triangle segments of a BLAS batch from the caller's inputs (validated).
+int make_segments(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count, std::vector<TriSegment>& segs,
+                  std::vector<uint32_t>& seg_start, uint64_t& total) {
+    segs.resize(count);
+    seg_start.resize(count + 1);
+    total = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const rt_blas_input& in = inputs[i];
+        if (in.primitive_count > 0) {
+            if (!in.vertex_data || !in.index_data)
+                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": NULL vertex or index data");
+            if (in.vertex_stride < 12 || (in.vertex_stride & 3u))
+                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": vertex_stride must be >= 12 and a multiple of 4");
+            if (in.primitive_offset & 3u)
+                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": primitive_offset must be a multiple of 4");
+        }
+        seg_start[i] = uint32_t(total);
+        segs[i] = TriSegment{(const uint8_t*)in.vertex_data,
+                             in.index_data ? in.index_data + in.primitive_offset / 4 : nullptr,
+                             in.vertex_stride, in.max_vertex, in.first_vertex, in.primitive_count, uint32_t(total), 0u};
+        total += in.primitive_count;
+        if (total >= (1ull << kLeafCountShift))
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "batch exceeds 2^27 triangles");
+    }
+    seg_start[count] = uint32_t(total);
+    return RT_OK;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- context
@@ -323,6 +355,8 @@ int rt_sync(rt_ctx* ctx) {
         RT_HIP(ctx, hipMemset(ctx->d_err, 0, sizeof(uint32_t)), "reset error flags");
         if (flags & kErrStackOverflow)
             return fail(ctx, RT_ERR_STACK_OVERFLOW, "a traversal exceeded its stack restart budget; results are incomplete");
+        if (flags & kErrBadSkin)
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a vertex's skin index or joint is out of range (left unskinned)");
     }
     return RT_OK;
 }
@@ -332,28 +366,11 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!inputs || !out || count == 0) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "inputs/out must be non-NULL, count > 0");
     DeviceGuard g(ctx->device);
-    std::vector<TriSegment> segs(count);
-    std::vector<uint32_t> seg_start(count + 1);
+    std::vector<TriSegment> segs;
+    std::vector<uint32_t> seg_start;
     uint64_t total = 0;
-    for (uint32_t i = 0; i < count; i++) {
-        const rt_blas_input& in = inputs[i];
-        if (in.primitive_count > 0) {
-            if (!in.vertex_data || !in.index_data)
-                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": NULL vertex or index data");
-            if (in.vertex_stride < 12 || (in.vertex_stride & 3u))
-                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": vertex_stride must be >= 12 and a multiple of 4");
-            if (in.primitive_offset & 3u)
-                return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": primitive_offset must be a multiple of 4");
-        }
-        seg_start[i] = uint32_t(total);
-        segs[i] = TriSegment{(const uint8_t*)in.vertex_data,
-                             in.index_data ? in.index_data + in.primitive_offset / 4 : nullptr,
-                             in.vertex_stride, in.max_vertex, in.first_vertex, in.primitive_count, uint32_t(total), 0u};
-        total += in.primitive_count;
-        if (total >= (1ull << kLeafCountShift))
-            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "batch exceeds 2^27 triangles");
-    }
-    seg_start[count] = uint32_t(total);
+    int src = make_segments(ctx, inputs, count, segs, seg_start, total);
+    if (src) return src;
     const uint32_t n = uint32_t(total);
     hipStream_t st = ctx->stream;
     hipError_t e;
@@ -388,6 +405,11 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     pool->descs.reset(dev_alloc<BlasDesc>(count, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
     pool->bytes = sizeof(QNode) * size_t(res.num_nodes) + 48 * size_t(n) + sizeof(BlasDesc) * count;
+    pool->count = count;
+    pool->num_nodes = res.num_nodes;
+    pool->num_tris = n;
+    pool->level_start = res.level_start;
+    for (uint32_t i = 0; i < count; i++) pool->prim_count.push_back(inputs[i].primitive_count);
     rc = reorder_triangles(st, (const float4*)d_tri_in.get(), (const uint32_t*)perm.get(), n, (float4*)pool->tris.get());
     if (rc) return hip_fail(ctx, hipError_t(rc), "reorder_triangles");
     std::vector<float> hb(6 * size_t(count));
@@ -420,6 +442,65 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
 
 int rt_build_blas(rt_ctx* ctx, const rt_blas_input* input, rt_blas* out) {
     return rt_build_blas_batch(ctx, input, 1, out);
+}
+
+int rt_skin_vertices(rt_ctx* ctx, const rt_vertex* d_in, rt_vertex* d_out, uint32_t count, const float* d_skins,
+                     uint32_t num_skins) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (count && (!d_in || !d_out)) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "vertex buffers must be non-NULL");
+    if (num_skins && !d_skins) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "d_skins is NULL");
+    if ((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) & 15u)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "vertex buffers must be 16-byte aligned");
+    DeviceGuard g(ctx->device);
+    int rc = launch_skin(ctx->stream, d_in, d_out, count, d_skins, num_skins, ctx->d_err);
+    if (rc) return hip_fail(ctx, hipError_t(rc), "skinning launch");
+    return RT_OK;
+}
+
+int rt_refit_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count, const rt_blas* blases) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!inputs || !blases || count == 0) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "inputs/blases must be non-NULL, count > 0");
+    std::vector<BlasEntry*> ents(count);
+    std::shared_ptr<BlasPool> pool;
+    for (uint32_t i = 0; i < count; i++) {
+        auto it = ctx->blas.find(blases[i]);
+        if (it == ctx->blas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown BLAS handle");
+        ents[i] = &it->second;
+        if (i == 0) pool = it->second.pool;
+        if (it->second.pool != pool || it->second.h_desc.root != i)
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "blases must be one rt_build_blas_batch call's handles, in its order");
+    }
+    if (pool->count != count)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "blases must be all of one rt_build_blas_batch call's handles");
+    for (uint32_t i = 0; i < count; i++)
+        if (inputs[i].primitive_count != pool->prim_count[i])
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "input " + std::to_string(i) + ": primitive_count differs from the build");
+    DeviceGuard g(ctx->device);
+    std::vector<TriSegment> segs;
+    std::vector<uint32_t> seg_start;
+    uint64_t total = 0;
+    int rc = make_segments(ctx, inputs, count, segs, seg_start, total);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    DevPtr d_segs(dev_alloc<TriSegment>(count, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    DevPtr d_bad(dev_alloc<uint32_t>(1, &e));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    RT_HIP(ctx, hipMemcpyAsync(d_segs.get(), segs.data(), sizeof(TriSegment) * count, hipMemcpyHostToDevice, st), "copy segments");
+    RT_HIP(ctx, hipMemsetAsync(d_bad.get(), 0, sizeof(uint32_t), st), "memset");
+    std::string err;
+    rc = lbvh_refit(st, (const TriSegment*)d_segs.get(), count, pool->num_tris, (float4*)pool->tris.get(),
+                    (QNode*)pool->nodes.get(), pool->num_nodes, pool->level_start, (BlasDesc*)pool->descs.get(),
+                    (uint32_t*)d_bad.get(), &err);
+    if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "BLAS refit: " + err);
+    uint32_t bad = 0;
+    std::vector<BlasDesc> descs(count);
+    RT_HIP(ctx, hipMemcpy(&bad, d_bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost), "copy flag");
+    RT_HIP(ctx, hipMemcpy(descs.data(), pool->descs.get(), sizeof(BlasDesc) * count, hipMemcpyDeviceToHost), "copy descs");
+    for (uint32_t i = 0; i < count; i++) std::memcpy(ents[i]->h_desc.bounds, descs[i].bounds, sizeof(descs[i].bounds));
+    if (bad) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "an index exceeds max_vertex");
+    return RT_OK;
 }
 
 int rt_blas_device_address(rt_ctx* ctx, rt_blas blas, uint64_t* out) {

@@ -96,7 +96,7 @@ struct alignas(16) InstanceRecord {
 };
 static_assert(sizeof(InstanceRecord) == 256, "InstanceRecord is 256 bytes");
 
-enum : uint32_t { kErrStackOverflow = 1u };
+enum : uint32_t { kErrStackOverflow = 1u, kErrBadSkin = 2u };
 
 // Flattened BVH: instances whose transform is the identity are not entered through
 // the TLAS; their triangles are copied into one BVH in world space whose triangle

@@ -34,6 +34,7 @@ struct LbvhResult {
     uint32_t num_nodes = 0;
     uint32_t* perm = nullptr;       // sorted position -> input primitive index
     float* seg_bounds = nullptr;    // 6 floats per segment (padded root box)
+    std::vector<uint32_t> level_start;  // nodes of tree level L: [level_start[L], level_start[L + 1])
 };
 
 // Builds the wide BVH of `n` primitives with boxes (lo[i], hi[i]) split into
@@ -69,6 +70,16 @@ struct FlatSource {
 // v2.w = source record index.
 int gather_flat(hipStream_t stream, const FlatSource* d_src, uint32_t nsrc, uint32_t n, float4* d_tri, float4* d_lo,
                 float4* d_hi);
+
+// Refit of a built BLAS pool to new vertex data with the same topology: rewrites the
+// triangle records' positions from `d_segs` (record r keeps its primitive index and
+// segment), then recomputes every node's child boxes bottom-up level by level, padded
+// and quantised as the build does, and the BlasDescs' root bounds.  Sets *d_bad to 1
+// if an index exceeds its segment's max_vertex or a primitive index its segment's
+// count.
+int lbvh_refit(hipStream_t stream, const TriSegment* d_segs, uint32_t nseg, uint32_t n, float4* d_tri, QNode* d_nodes,
+               uint32_t num_nodes, const std::vector<uint32_t>& level_start, BlasDesc* d_descs, uint32_t* d_bad,
+               std::string* err);
 
 // Instance boxes (world space) for the TLAS build.
 int prepare_instances(hipStream_t stream, const InstanceRecord* d_inst, uint32_t n, float4* d_lo,

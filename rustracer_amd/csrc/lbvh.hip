@@ -143,10 +143,7 @@ __device__ inline void quantize_axis(const float lo[4], const float hi[4], const
     exp = int8_t(e);
 }
 
-__global__ void k_quantize_nodes(const WideNode* in, uint32_t n, QNode* out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const WideNode w = in[i];
+__device__ inline QNode quantize_node(const WideNode& w) {
     QNode q;
     q._pad0 = 0;
     q._pad1 = 0;
@@ -157,7 +154,13 @@ __global__ void k_quantize_nodes(const WideNode* in, uint32_t n, QNode* out) {
         q.child[k] = w.child[k];
         q.count[k] = uint8_t(w.count[k]);
     }
-    out[i] = q;
+    return q;
+}
+
+__global__ void k_quantize_nodes(const WideNode* in, uint32_t n, QNode* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = quantize_node(in[i]);
 }
 
 __global__ void k_gather_flat(const FlatSource* src, uint32_t nsrc, uint32_t n, float4* tri, float4* lo, float4* hi) {
@@ -669,6 +672,121 @@ __global__ void k_seg_pad(const uint32_t* bounds, uint32_t nseg, float* pad, flo
     }
 }
 
+// ------------------------------------------------------------ refit
+// Record r of the pool gets new positions for its primitive (v0.w, kept) of the
+// segment whose records contain r; the other words are kept.  Raises the segment's
+// largest coordinate magnitude (non-negative floats order as their bits).
+__global__ void k_refit_records(const TriSegment* segs, uint32_t nseg, uint32_t n, float4* tri, uint32_t* seg_mag,
+                                uint32_t* bad) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t a = 0, b = nseg;
+    while (b - a > 1) {
+        uint32_t m = (a + b) >> 1;
+        if (segs[m].prim_start <= r) a = m; else b = m;
+    }
+    const TriSegment& s = segs[a];
+    float4* t = tri + 3 * size_t(r);
+    const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+    const uint32_t prim = __float_as_uint(t0.w);
+    float3 v[3];
+    bool ok = prim < s.prim_count;
+    for (int k = 0; k < 3; k++) {
+        uint32_t idx = ok ? s.index_data[3 * size_t(prim) + k] : 0u;
+        if (idx > s.max_vertex) { ok = false; idx = 0; }
+        const float* f = (const float*)(s.vertex_data + size_t(s.first_vertex + idx) * s.stride);
+        v[k] = mk3(f[0], f[1], f[2]);
+    }
+    if (!ok) {
+        atomicOr(bad, 1u);
+        v[1] = v[0];
+        v[2] = v[0];
+    }
+    t[0] = make_float4(v[0].x, v[0].y, v[0].z, t0.w);
+    t[1] = make_float4(v[1].x, v[1].y, v[1].z, t1.w);
+    t[2] = make_float4(v[2].x, v[2].y, v[2].z, t2.w);
+    float mag = 0.f;
+    for (int k = 0; k < 3; k++) mag = fmaxf(mag, fmaxf(fabsf(v[k].x), fmaxf(fabsf(v[k].y), fabsf(v[k].z))));
+    if (isfinite(mag)) atomicMax(seg_mag + a, __float_as_uint(mag));
+}
+
+// Segment of each node: the roots are nodes 0..nseg-1, and the internal children of
+// one level's nodes inherit their parent's segment (run top-down, level by level).
+__global__ void k_refit_seg(const QNode* nodes, uint32_t begin, uint32_t end, uint32_t* node_seg) {
+    uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const QNode q = nodes[i];
+    const uint32_t seg = node_seg[i];
+    for (int k = 0; k < 4; k++)
+        if (q.child[k] != kEmpty && q.count[k] == 0) node_seg[q.child[k]] = seg;
+}
+
+// One level, bottom-up: child boxes from the triangle records (leaves) or from the
+// refit boxes of the deeper level (internal children), padded by the segment's
+// padding as the build pads them, quantised; the node's own (unpadded) box is kept
+// for its parent.
+__global__ void k_refit_level(QNode* nodes, uint32_t begin, uint32_t end, const float4* tri, const uint32_t* node_seg,
+                              const uint32_t* seg_mag, float* node_box) {
+    uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const QNode q = nodes[i];
+    // about 8 ulp of the segment's largest coordinate magnitude (as the build pads)
+    const float pad = __uint_as_float(seg_mag[node_seg[i]]) * 9.5367431640625e-07f + 1e-30f;
+    WideNode w;
+    float3 nl = mk3(INFINITY, INFINITY, INFINITY), nh = mk3(-INFINITY, -INFINITY, -INFINITY);
+    for (int k = 0; k < 4; k++) {
+        const uint32_t c = q.child[k], cnt = q.count[k];
+        w.child[k] = c;
+        w.count[k] = cnt;
+        if (c == kEmpty) {
+            w.lo_x[k] = w.lo_y[k] = w.lo_z[k] = 0.f;
+            w.hi_x[k] = w.hi_y[k] = w.hi_z[k] = 0.f;
+            continue;
+        }
+        float3 l = mk3(INFINITY, INFINITY, INFINITY), h = mk3(-INFINITY, -INFINITY, -INFINITY);
+        if (cnt) {
+            for (uint32_t r = c; r < c + cnt; r++) {
+                const float4* t = tri + 3 * size_t(r);
+                for (int v = 0; v < 3; v++) {
+                    const float3 p = xyz(t[v]);
+                    l = fmin3(l, p);
+                    h = fmax3(h, p);
+                }
+            }
+        } else {
+            const float* b = node_box + 6 * size_t(c);
+            l = mk3(b[0], b[1], b[2]);
+            h = mk3(b[3], b[4], b[5]);
+        }
+        nl = fmin3(nl, l);
+        nh = fmax3(nh, h);
+        w.lo_x[k] = l.x - pad; w.lo_y[k] = l.y - pad; w.lo_z[k] = l.z - pad;
+        w.hi_x[k] = h.x + pad; w.hi_y[k] = h.y + pad; w.hi_z[k] = h.z + pad;
+    }
+    float* b = node_box + 6 * size_t(i);
+    b[0] = nl.x; b[1] = nl.y; b[2] = nl.z;
+    b[3] = nh.x; b[4] = nh.y; b[5] = nh.z;
+    nodes[i] = quantize_node(w);
+}
+
+__global__ void k_iota(uint32_t* out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
+}
+
+// Root bounds of every BLAS (padded), as the build reports them; 0 for an empty one.
+__global__ void k_refit_bounds(BlasDesc* descs, uint32_t nseg, const float* node_box, const uint32_t* seg_mag) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const float* b = node_box + 6 * size_t(s);
+    const float pad = __uint_as_float(seg_mag[s]) * 9.5367431640625e-07f + 1e-30f;
+    const bool empty = !(b[0] <= b[3]);
+    for (int k = 0; k < 3; k++) {
+        descs[s].bounds[k] = empty ? 0.f : b[k] - pad;
+        descs[s].bounds[3 + k] = empty ? 0.f : b[3 + k] + pad;
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host
@@ -841,6 +959,7 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
         hipLaunchKernelGGL(k_init_tasks, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, seg_root, nseg, ta);
         if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_init_tasks");
         uint32_t count = nseg, node_base = nseg;
+        out->level_start.assign({0u, nseg});
         for (int level = 0; count > 0; level++) {
             if (level > 200) return fail(int(hipErrorUnknown), "collapse did not terminate");
             hipLaunchKernelGGL(k_collapse_gather, dim3(blocks_for(count, kBlock)), dim3(kBlock), 0, st, a, ta, count,
@@ -858,6 +977,7 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
                 return fail(int(hipErrorUnknown), "read level size");
             count = last[0] + last[1];
             node_base += count;
+            if (count) out->level_start.push_back(node_base);
             if (node_base > max_nodes) return fail(int(hipErrorUnknown), "node capacity exceeded during collapse");
             std::swap(ta, tb);
         }
@@ -865,6 +985,7 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
             return fail(int(hipErrorUnknown), "copy node count");
     } else {
         // no primitives at all: every segment root is an empty node
+        out->level_start.assign({0u, nseg});
         uint32_t h_init[4] = {nseg, 0, 0, 0};
         if (hipMemcpyAsync(nc, h_init, sizeof(h_init), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(int(hipErrorUnknown), "copy counters");
@@ -889,6 +1010,47 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
     out->perm = perm_out;
     out->seg_bounds = seg_bounds;
     return 0;
+}
+
+int lbvh_refit(hipStream_t st, const TriSegment* d_segs, uint32_t nseg, uint32_t n, float4* d_tri, QNode* d_nodes,
+               uint32_t num_nodes, const std::vector<uint32_t>& level_start, BlasDesc* d_descs, uint32_t* d_bad,
+               std::string* err) {
+    if (level_start.size() < 2 || level_start.front() != 0u || level_start[1] != nseg || level_start.back() != num_nodes) {
+        *err = "the pool's level table does not match its nodes";
+        return int(hipErrorInvalidValue);
+    }
+    Scratch s;
+    uint32_t* seg_mag = s.alloc<uint32_t>(nseg);
+    uint32_t* node_seg = s.alloc<uint32_t>(num_nodes);
+    float* node_box = s.alloc<float>(6 * size_t(num_nodes));
+    if (!seg_mag || !node_seg || !node_box) {
+        *err = "out of device memory";
+        return int(hipErrorOutOfMemory);
+    }
+    hipError_t e = hipMemsetAsync(seg_mag, 0, sizeof(uint32_t) * nseg, st);
+    if (e != hipSuccess) {
+        *err = "memset";
+        return int(e);
+    }
+    if (n) hipLaunchKernelGGL(k_refit_records, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_segs, nseg, n, d_tri,
+                              seg_mag, d_bad);
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, node_seg, nseg);
+    const size_t levels = level_start.size() - 1;
+    for (size_t L = 0; L < levels; L++) {
+        const uint32_t b = level_start[L], c = level_start[L + 1];
+        if (c > b) hipLaunchKernelGGL(k_refit_seg, dim3(blocks_for(c - b, kBlock)), dim3(kBlock), 0, st, d_nodes, b, c, node_seg);
+    }
+    for (size_t L = levels; L-- > 0;) {
+        const uint32_t b = level_start[L], c = level_start[L + 1];
+        if (c > b)
+            hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(c - b, kBlock)), dim3(kBlock), 0, st, d_nodes, b, c,
+                               (const float4*)d_tri, node_seg, seg_mag, node_box);
+    }
+    hipLaunchKernelGGL(k_refit_bounds, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, d_descs, nseg, node_box, seg_mag);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // the scratch is freed on return
+    if (e != hipSuccess) *err = "refit launch";
+    return int(e);
 }
 
 }  // namespace rt

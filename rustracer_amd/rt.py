@@ -38,7 +38,7 @@ EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", 
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
            "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint",
-           "rt_trace_rays_batch", "rt_autotune"]
+           "rt_trace_rays_batch", "rt_autotune", "rt_skin_vertices", "rt_refit_blas_batch"]
 
 
 class RtError(RuntimeError):
@@ -129,6 +129,8 @@ def load_library(path=LIB_PATH):
         "rt_blas_device_address": (i, [vp, u64, C.POINTER(u64)]), "rt_destroy_blas": (None, [vp, u64]),
         "rt_build_tlas": (i, [vp, vp, u, C.POINTER(u64)]), "rt_destroy_tlas": (None, [vp, u64]),
         "rt_trace_rays": (i, [vp, u64, vp, vp, vp, u, u, u, vp]),
+        "rt_skin_vertices": (i, [vp, vp, vp, u, vp, u]),
+        "rt_refit_blas_batch": (i, [vp, vp, u, C.POINTER(u64)]),
         "rt_trace_ray_queries": (i, [vp, u64, vp, vp, u, u, vp]),
         "rt_trace_primary": (i, [vp, u64, vp, u, u, vp]),
         "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
@@ -303,11 +305,14 @@ class Blas:
             pass
 
 
-def blas_inputs(scene, buffers):
-    """primitive_to_vk_geometry (acceleration_structures.rs:17-76) for every geometry id."""
+def blas_inputs(scene, buffers, vertices=None):
+    """primitive_to_vk_geometry (acceleration_structures.rs:17-76) for every geometry id;
+    `vertices` (a device tensor in the vertex layout) replaces the vertex buffer, as the
+    reference uses its animation buffer for skinned geometry (:23-27)."""
     geo = scene.geometries
     ins = (BlasInput * len(geo))()
-    vb, ib = buffers.vertices.data_ptr(), buffers.indices.data_ptr()
+    vb = (buffers.vertices if vertices is None else vertices).data_ptr()
+    ib = buffers.indices.data_ptr()
     for g in range(len(geo)):
         first_vertex, index_offset, count, opaque = (int(x) for x in geo[g])
         idx = scene.indices[index_offset:index_offset + 3 * count]
@@ -325,6 +330,31 @@ def create_as(ctx, scene, buffers):
     ctx.check(ctx.lib.rt_build_blas_batch(ctx.handle, ins, len(ins), handles))
     blases = [Blas(ctx, h) for h in handles]
     return blases, create_top_as(ctx, scene, blases)
+
+
+def refit_as(ctx, scene, buffers, blases, vertices=None):
+    """rt_refit_blas_batch over the BLASes of create_as with the scene's (or the given)
+    vertex data, then create_top_as again (the reference rebuilds its TLAS too).
+    Returns the new TopAS."""
+    ins = blas_inputs(scene, buffers, vertices)
+    handles = (C.c_uint64 * len(blases))(*[b.handle for b in blases])
+    ctx.check(ctx.lib.rt_refit_blas_batch(ctx.handle, ins, len(ins), handles))
+    return create_top_as(ctx, scene, blases)
+
+
+def skin_vertices(ctx, vertices_in, vertices_out, skins):
+    """rt_skin_vertices (AnimationCompute.comp): vertices_* are device tensors holding
+    vertex records (128 B each; may be the same tensor), skins a float32 device tensor
+    of shape (num_skins, 256, 4, 4) holding column-major mat4 (skins[s, j, c] is
+    column c).  Enqueued on the context stream."""
+    n = vertices_in.numel() * vertices_in.element_size() // L.VERTEX.itemsize
+    if vertices_out.numel() * vertices_out.element_size() < n * L.VERTEX.itemsize:
+        raise ValueError("vertices_out is smaller than vertices_in")
+    if skins is not None and (skins.dtype != _torch().float32 or not skins.is_contiguous() or skins.shape[1:] != (256, 4, 4)):
+        raise ValueError("skins must be a contiguous float32 tensor of shape (num_skins, 256, 4, 4)")
+    ns = 0 if skins is None else skins.shape[0]
+    ctx.check(ctx.lib.rt_skin_vertices(ctx.handle, C.c_void_p(vertices_in.data_ptr()), C.c_void_p(vertices_out.data_ptr()),
+                                       n, C.c_void_p(skins.data_ptr()) if ns else None, ns))
 
 
 def create_top_as(ctx, scene, blases):

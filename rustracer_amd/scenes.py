@@ -362,6 +362,41 @@ def cornell_sphere(width=256, height=256, spp=4, bounces=4, seed=0):
     return b.finish("cornell_sphere", inst, plights, [L.default_dlight()], ubo, width, height)
 
 
+def bend_skins(angle):
+    """One skin (the reference's Skin: 256 column-major mat4) with bone 0 the identity
+    and bone 1 a rotation by `angle` about the z axis through (0, 0.2, 0) plus a lift:
+    float32 (1, 256, 4, 4), skins[0, j, c] = column c of bone j."""
+    sk = np.zeros((1, 256, 4, 4), np.float32)
+    sk[0, :] = np.eye(4, dtype=np.float32)  # rows of the identity are its columns
+    c, s_ = np.cos(angle), np.sin(angle)
+    m = np.eye(4)
+    m[:2, :2] = ((c, -s_), (s_, c))
+    pivot = np.array([0.0, 0.2, 0.0])
+    m[:3, 3] = pivot - m[:3, :3] @ pivot + (0.0, 0.1, 0.0)
+    sk[0, 1] = m.T.astype(np.float32)  # column-major
+    return sk
+
+
+def skinned_cornell(width=256, height=256, spp=4, bounces=4, seed=0):
+    """The Cornell-box scene with a skinned sphere (AnimationCompute.comp's inputs):
+    the sphere's vertices use skin 0, joints (0, 1), weights blending from bone 0 below
+    y = -0.2 to bone 1 above y = 0.6 (object space).  Skin it with bend_skins."""
+    sc = cornell_sphere(width=width, height=height, spp=spp, bounces=bounces, seed=seed)
+    v = sc.vertices
+    g = sc.geometries[-1]  # the sphere
+    first = int(g["first_vertex"])
+    idx = sc.indices[int(g["index_offset"]):int(g["index_offset"]) + 3 * int(g["prim_count"])]
+    sel = np.arange(first, first + int(idx.max()) + 1)
+    y = v["pos"][sel, 1].astype(np.float64)
+    w1 = np.clip((y + 0.2) / 0.8, 0.0, 1.0)
+    v["skin_index"][:] = -1
+    v["skin_index"][sel] = 0
+    v["joints"][sel] = (0, 1, 0, 0)
+    v["weights"][sel] = np.stack([1.0 - w1, w1, np.zeros_like(w1), np.zeros_like(w1)], 1).astype(np.float32)
+    sc.name = "skinned_cornell"
+    return sc
+
+
 def displaced_sphere(width=1920, height=1080, spp=64, bounces=8, seed=1, freq=316, group=1024):
     rng = np.random.default_rng(seed)
     pos, tri = geodesic_sphere(freq)
