@@ -140,6 +140,62 @@ def adjoint_iteration(ctx, steps, warmup):
                         "parameter textures; forward + L2 gradient + path-replay adjoint"}
 
 
+def dynamic_step(ctx, sc, buf, blases, steps):
+    """One animation step on the benchmark scene, as the reference does it for skinned
+    geometry (main.rs:377-404): skin the vertex buffer (AnimationCompute.comp; every
+    vertex skinned, blending two bones by height), then refit the BLASes to the skinned
+    vertices and rebuild the TLAS; against rebuilding the BLASes and TLAS from scratch.
+    Returns milliseconds and the skinning kernel's bandwidth (128 B read + 128 B
+    written per vertex)."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from rustracer_amd import layouts as L, rt, scenes
+    v = np.array(sc.vertices, copy=True)
+    y = v["pos"][:, 1].astype(np.float64)
+    w1 = np.clip((y + 0.2) / 0.8, 0.0, 1.0)
+    v["skin_index"] = 0
+    v["joints"] = (0, 1, 0, 0)
+    v["weights"] = np.stack([1.0 - w1, w1, 0 * w1, 0 * w1], 1).astype(np.float32)
+    d_in = rt.to_device(v)
+    d_out = torch.empty_like(d_in)
+    skins = torch.from_numpy(scenes.bend_skins(0.05)).cuda()
+    ins = rt.blas_inputs(sc, buf, d_out)
+    handles = (C.c_uint64 * len(blases))(*[b.handle for b in blases])
+    stream = torch.cuda.current_stream()
+    skin_ms, refit_ms, tlas_ms, rebuild_ms = [], [], [], []
+    for k in range(steps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        rt.skin_vertices(ctx, d_in, d_out, skins)
+        e1.record(stream)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.check(ctx.lib.rt_refit_blas_batch(ctx.handle, ins, len(ins), handles))
+        t1 = time.perf_counter()
+        tl = rt.create_top_as(ctx, sc, blases)
+        ctx.sync()
+        t2 = time.perf_counter()
+        del tl
+        nb, nt = rt.create_as(ctx, sc, buf)
+        ctx.sync()
+        t3 = time.perf_counter()
+        del nb, nt
+        if k:  # the first round warms up
+            skin_ms.append(e0.elapsed_time(e1))
+            refit_ms.append((t1 - t0) * 1e3)
+            tlas_ms.append((t2 - t1) * 1e3)
+            rebuild_ms.append((t3 - t2) * 1e3)
+    med = lambda x: round(float(np.median(x)), 3)
+    n = len(v)
+    return {"vertices": n, "triangles": int(sc.num_triangles), "skin_ms": med(skin_ms),
+            "skin_GBps": round(n * 2 * L.VERTEX.itemsize / (float(np.median(skin_ms)) / 1e3) / 1e9, 1),
+            "refit_ms": med(refit_ms), "tlas_ms": med(tlas_ms), "rebuild_ms": med(rebuild_ms),
+            "workload": "skin every vertex of the benchmark scene (two bones), refit its 1951 BLASes, rebuild the TLAS; "
+                        "vs rebuilding BLASes and TLAS"}
+
+
 def batch_benchmark(ctx, steps):
     """The many-scene batch: 64 seeded scenes (room + displaced sphere of 1k-500k
     triangles, 128^2 - 1024^2, 16 spp, depth 6) traced by one rt_trace_rays_batch
@@ -203,6 +259,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-dynamic", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -322,6 +379,8 @@ def main():
                      "launch_ms": round(launch_ms, 4), "algorithmic_bytes": int(bytes_per_launch),
                      "stage_ms": {k: round(v, 3) for k, v in stage.items()}},
     }
+    if not args.no_dynamic:
+        line["config"]["dynamic"] = dynamic_step(ctx, sc, buf, blases, 3)
     if not args.no_adjoint:
         line["config"]["adjoint"] = adjoint_iteration(ctx, max(3, args.steps), 1)
     if not args.no_batch:
