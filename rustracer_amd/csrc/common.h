@@ -117,6 +117,24 @@ struct TlasView {
     uint32_t _pad;
 };
 
+// ------------------------------------------------------------ global loads
+// Loads through the global address space.  Pointers the kernels read from memory
+// (BVH pools, instance records) are generic to the compiler, which then emits FLAT
+// loads; those share a wait counter with LDS operations (the traversal stack), so a
+// stack access would also wait for every outstanding node fetch.  A native vector
+// type keeps the load a single global_load_dwordx4.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline uint4 ldg_u4(const void* p) {
+    const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float4 ldg_f4(const void* p) {
+    const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline uint32_t ldg_u32(const void* p) { return *(const __attribute__((address_space(1))) uint32_t*)p; }
+
 // ------------------------------------------------------------ float3 math
 __host__ __device__ inline float3 mk3(float x, float y, float z) { return make_float3(x, y, z); }
 __host__ __device__ inline float3 operator+(float3 a, float3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }

@@ -104,7 +104,7 @@ __device__ inline void refine_hit(const TlasView& tl, const HitInfo& h, D3 o, D3
     D3 od = d3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
                m[8] * d.x + m[9] * d.y + m[10] * d.z);
     const float4* tp = ir.tris + 3 * size_t(h.rec);
-    float4 fa = tp[0], fb = tp[1], fc = tp[2];
+    float4 fa = ldg_f4(tp), fb = ldg_f4(tp + 1), fc = ldg_f4(tp + 2);
     double ax = fabs(od.x), ay = fabs(od.y), az = fabs(od.z);
     int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;

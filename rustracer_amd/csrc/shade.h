@@ -110,10 +110,10 @@ __device__ inline int wrap_coord(int i, int size, uint32_t mode) {
 __device__ inline D4 texel(const rt_texture& t, const float* lut, int x, int y) {
     size_t idx = size_t(y) * t.width + size_t(x);
     if (t.format == RT_FMT_RGBA32F) {
-        float4 f = reinterpret_cast<const float4*>(t.data)[idx];
+        float4 f = ldg_f4(reinterpret_cast<const float4*>(t.data) + idx);
         return D4{f.x, f.y, f.z, f.w};
     }
-    uint32_t v = reinterpret_cast<const uint32_t*>(t.data)[idx];
+    uint32_t v = ldg_u32(reinterpret_cast<const uint32_t*>(t.data) + idx);
     uint32_t r = v & 255u, g = (v >> 8) & 255u, b = (v >> 16) & 255u, a = v >> 24;
     if (t.format == RT_FMT_RGBA8_SRGB) return D4{lut[r], lut[g], lut[b], a / 255.0};
     return D4{r / 255.0, g / 255.0, b / 255.0, a / 255.0};

@@ -125,7 +125,7 @@ __device__ inline float qbyte(uint32_t w, int k) { return float((w >> (8 * k)) &
 // Four dwordx4 loads and the decode of the child planes (fmaf(q, 2^e, origin)).
 __device__ inline WideNode load_node(const QNode* nodes, uint32_t i) {
     const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
-    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    const uint4 w0 = ldg_u4(p), w1 = ldg_u4(p + 1), w2 = ldg_u4(p + 2), w3 = ldg_u4(p + 3);
     const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
     const float sx = pow2f(int(int8_t(w0.w & 255u))), sy = pow2f(int(int8_t((w0.w >> 8) & 255u)));
     const float sz = pow2f(int(int8_t((w0.w >> 16) & 255u)));
@@ -314,14 +314,14 @@ struct Traversal {
             const uint32_t first = e & kLeafFirstMask;
             const uint32_t count = ((e >> kLeafCountShift) & 15u) + 1u;
             const float4* tp = tris + 3 * size_t(first);
-            float4 na = tp[0], nb = tp[1], nc = tp[2];
+            float4 na = ldg_f4(tp), nb = ldg_f4(tp + 1), nc = ldg_f4(tp + 2);
             for (uint32_t k = 0; k < count; k++) {
                 // the next record's loads are in flight while this one is tested
                 const float4 a = na, b = nb, c = nc;
                 if (k + 1 < count) {
-                    na = tp[3 * (k + 1)];
-                    nb = tp[3 * (k + 1) + 1];
-                    nc = tp[3 * (k + 1) + 2];
+                    na = ldg_f4(tp + 3 * (k + 1));
+                    nb = ldg_f4(tp + 3 * (k + 1) + 1);
+                    nc = ldg_f4(tp + 3 * (k + 1) + 2);
                 }
                 if (kStats) st.tris++;
                 float t, bu, bv;
