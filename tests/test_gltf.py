@@ -1,0 +1,214 @@
+"""glTF ingestion (rustracer_amd.gltf) against hand-computed expectations on a small
+glTF written here: geometry ids in mesh / primitive order, local indices and
+PrimInfo offsets, generated flat normals, materials (texture index + 1, alpha modes,
+KHR extensions), the size-10 normalisation, depth-first instances and lights, the
+light fallbacks, texture slots (dummy slot 0, sRGB vs linear) and .glb = .gltf.
+The reference's own Cornell box asset is loaded when the checkout is present."""
+import base64
+import io
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from rustracer_amd import gltf, layouts as L
+
+REF_CORNELL = "/root/reference/assets/models/CornellBox/cornellBox.gltf"
+
+
+def _png(rgba):
+    from PIL import Image
+    b = io.BytesIO()
+    Image.fromarray(rgba, "RGBA").save(b, format="PNG")
+    return b.getvalue()
+
+
+def _write(tmp_path, glb=False, lights=True):
+    """Two meshes: mesh 0 = a unit quad (indexed, with normals, uv) and a triangle
+    without normals or indices (two primitives); mesh 1 = the quad again with
+    material 1.  Nodes: 0 (translation (2, 0, 0), scale 2) with child 1 (mesh 0),
+    node 2 (mesh 1, translation (0, 0, -4)), node 3 a point light at (0, 3, 0),
+    node 4 a directional light rotated 90 degrees about x."""
+    quad_p = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    quad_n = np.tile(np.array([0, 0, 1], np.float32), (4, 1))
+    quad_uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    quad_i = np.array([0, 1, 2, 0, 2, 3], np.uint16)
+    tri_p = np.array([[0, 0, 1], [1, 0, 1], [0, 1, 1]], np.float32)
+    blobs, views, accs = [], [], []
+
+    def add(arr, ctype, typ, minmax=False):
+        off = sum(len(b) for b in blobs)
+        data = arr.tobytes()
+        pad = (-len(data)) % 4
+        blobs.append(data + b"\0" * pad)
+        views.append({"buffer": 0, "byteOffset": off, "byteLength": len(data)})
+        a = {"bufferView": len(views) - 1, "componentType": ctype, "count": len(arr), "type": typ}
+        if minmax:
+            a["min"], a["max"] = arr.min(0).tolist(), arr.max(0).tolist()
+        accs.append(a)
+        return len(accs) - 1
+    qp, qn, qu, qi = add(quad_p, 5126, "VEC3", True), add(quad_n, 5126, "VEC3"), add(quad_uv, 5126, "VEC2"), \
+        add(quad_i, 5123, "SCALAR")
+    tp = add(tri_p, 5126, "VEC3", True)
+    tex = np.zeros((2, 2, 4), np.uint8)
+    tex[..., 0] = [[10, 20], [30, 40]]
+    tex[..., 3] = 255
+    png = _png(tex)
+    js = {
+        "asset": {"version": "2.0"},
+        "scene": 0,
+        "scenes": [{"nodes": [0, 2] + ([3, 4] if lights else [])}],
+        "nodes": [{"translation": [2, 0, 0], "scale": [2, 2, 2], "children": [1]}, {"mesh": 0},
+                  {"mesh": 1, "translation": [0, 0, -4]}],
+        "meshes": [{"primitives": [{"attributes": {"POSITION": qp, "NORMAL": qn, "TEXCOORD_0": qu}, "indices": qi,
+                                    "material": 0},
+                                   {"attributes": {"POSITION": tp}, "material": 0},
+                                   {"attributes": {"POSITION": tp}, "mode": 1}]},  # lines: skipped
+                   {"primitives": [{"attributes": {"POSITION": qp, "NORMAL": qn, "TEXCOORD_0": qu}, "indices": qi,
+                                    "material": 1}]}],
+        "materials": [{"pbrMetallicRoughness": {"baseColorFactor": [0.8, 0.2, 0.2, 1], "metallicFactor": 0.0,
+                                                "roughnessFactor": 0.7, "baseColorTexture": {"index": 0}}},
+                      {"alphaMode": "MASK", "alphaCutoff": 0.3, "emissiveFactor": [1, 0.5, 0],
+                       "pbrMetallicRoughness": {"metallicRoughnessTexture": {"index": 1}},
+                       "extensions": {"KHR_materials_ior": {"ior": 1.33},
+                                      "KHR_materials_specular": {"specularFactor": 0.5}}}],
+        "textures": [{"source": 0, "sampler": 0}, {"source": 1}],
+        "samplers": [{"magFilter": 9728, "minFilter": 9728, "wrapS": 33071, "wrapT": 33648}],
+        "images": [{"uri": "data:image/png;base64," + base64.b64encode(png).decode()},
+                   {"uri": "data:image/png;base64," + base64.b64encode(png).decode()}],
+        "accessors": accs, "bufferViews": views,
+    }
+    if lights:
+        js["nodes"] += [{"translation": [0, 3, 0], "extensions": {"KHR_lights_punctual": {"light": 0}}},
+                        {"rotation": [0.7071067811865476, 0, 0, 0.7071067811865476],
+                         "extensions": {"KHR_lights_punctual": {"light": 1}}}]
+        js["extensions"] = {"KHR_lights_punctual": {"lights": [
+            {"type": "point", "color": [1, 0.9, 0.8], "intensity": 30.0},
+            {"type": "directional", "intensity": 2.0}]}}
+    binary = b"".join(blobs)
+    if glb:
+        js["buffers"] = [{"byteLength": len(binary)}]
+        jb = json.dumps(js).encode()
+        jb += b" " * ((-len(jb)) % 4)
+        out = struct.pack("<III", 0x46546C67, 2, 12 + 8 + len(jb) + 8 + len(binary))
+        out += struct.pack("<II", len(jb), 0x4E4F534A) + jb + struct.pack("<II", len(binary), 0x004E4942) + binary
+        p = tmp_path / "scene.glb"
+        p.write_bytes(out)
+    else:
+        (tmp_path / "scene.bin").write_bytes(binary)
+        js["buffers"] = [{"uri": "scene.bin", "byteLength": len(binary)}]
+        p = tmp_path / "scene.gltf"
+        p.write_text(json.dumps(js))
+    return str(p)
+
+
+def test_geometry_and_prim_infos(tmp_path):
+    sc = gltf.load(_write(tmp_path))
+    # mesh 0: quad (geo 0), triangle (geo 1); lines skipped; mesh 1: quad (geo 2)
+    assert len(sc.geometries) == 3
+    assert sc.prim_infos[["v_offset", "i_offset", "material_id"]].tolist() == [(0, 0, 0), (4, 6, 0), (7, 9, 1)]
+    assert sc.geometries["prim_count"].tolist() == [2, 1, 2]
+    assert sc.indices.tolist() == [0, 1, 2, 0, 2, 3, 0, 1, 2, 0, 1, 2, 0, 2, 3]  # local indices
+    v = sc.vertices
+    assert np.all(v["pos"][:, 3] == 0) and np.all(v["skin_index"] == -1) and np.all(v["color"] == 1)
+    assert np.array_equal(v["tangent"][0], [1, 0, 0, 0])
+    # the triangle had no normals: the normalised cross product of its edges
+    assert np.allclose(v["normal"][4:7, :3], [0, 0, 1]) and np.all(v["normal"][:, 3] == 0)
+    assert np.array_equal(v["uv0"][0:4], [[0, 0], [1, 0], [1, 1], [0, 1]])
+
+
+def test_materials_and_textures(tmp_path):
+    sc = gltf.load(_write(tmp_path))
+    m0, m1 = sc.materials
+    assert m0["alpha_mode"] == L.ALPHA_OPAQUE and m1["alpha_mode"] == L.ALPHA_MASK
+    assert np.isclose(m1["alpha_cutoff"], 0.3) and np.isclose(m0["alpha_cutoff"], 0.5)
+    assert tuple(m0["base_color_texture"]) == (1, 0) and tuple(m1["base_color_texture"]) == (-1, -1)
+    assert tuple(m1["metallic_roughness_texture"]) == (2, 0)
+    assert np.allclose(m0["base_color"], [0.8, 0.2, 0.2, 1]) and np.isclose(m0["roughness_factor"], 0.7)
+    assert np.isclose(m1["ior"], 1.33) and np.isclose(m0["ior"], 1.5)
+    assert m1["specular_exist"] == 1 and np.isclose(m1["specular_factor"], 0.5)
+    assert np.allclose(m1["emissive_factor"], [1, 0.5, 0, 0])
+    assert not sc.fully_opaque()
+    # slot 0: the 1x1 default image; slot 1 glTF texture 0 (sRGB colour), slot 2 (linear MR)
+    assert len(sc.textures) == 3
+    assert sc.textures[0].data.tolist() == [[[1, 1, 1, 1]]]
+    assert sc.textures[1].format == L.FMT_RGBA8_SRGB and sc.textures[2].format == L.FMT_RGBA8_UNORM
+    assert sc.textures[1].data[..., 0].tolist() == [[10, 20], [30, 40]]
+    assert (sc.textures[1].wrap_s, sc.textures[1].wrap_t) == (L.WRAP_CLAMP_TO_EDGE, L.WRAP_MIRRORED_REPEAT)
+    assert sc.textures[1].mag_filter == L.FILTER_NEAREST and sc.textures[2].mag_filter == L.FILTER_LINEAR
+
+
+def test_normalisation_instances_and_lights(tmp_path):
+    sc = gltf.load(_write(tmp_path))
+    inst = sc.instances
+    # depth-first: node 0 -> node 1 (mesh 0: geo 0, 1), then node 2 (mesh 1: geo 2)
+    assert [int(i) & 0xFFFFFF for i in inst["custom_index_and_mask"]] == [0, 1, 2]
+    assert all(int(i) >> 24 == 0xFF for i in inst["custom_index_and_mask"])
+    assert inst["acceleration_structure_reference"].tolist() == [0, 1, 2]
+    # world-space corners of every instance's triangles: larger side 10, centred
+    pts = []
+    for k, i in enumerate(inst):
+        g = sc.geometries[int(i["acceleration_structure_reference"])]
+        idx = sc.indices[g["index_offset"]:g["index_offset"] + 3 * g["prim_count"]].astype(int) + g["first_vertex"]
+        p = sc.vertices["pos"][idx, :3].astype(np.float64)
+        m = i["transform"].astype(np.float64)
+        pts.append(p @ m[:, :3].T + m[:, 3])
+    pts = np.concatenate(pts)
+    lo, hi = pts.min(0), pts.max(0)
+    assert np.isclose((hi - lo).max(), 10.0, atol=1e-5) and np.allclose((hi + lo) / 2, 0, atol=1e-5)
+    # scene box: node 0 puts the quad and triangle in [2, 4]x[0, 2]x[0, 2], node 2 the quad in
+    # [0, 1]x[0, 1] at z = -4: box [0, 4]x[0, 2]x[-4, 2], larger side 6 -> scale 10/6
+    s = 10 / 6
+    assert np.allclose(inst[0]["transform"], [[2 * s, 0, 0, (2 - 2) * s], [0, 2 * s, 0, (0 - 1) * s],
+                                              [0, 0, 2 * s, (0 + 1) * s]], atol=1e-5)
+    assert len(sc.plights) == 1 and len(sc.dlights) == 1
+    assert np.allclose(sc.plights[0]["transform"], [(0 - 2) * s, (3 - 1) * s, (0 + 1) * s, 1], atol=1e-5)
+    assert sc.plights[0]["kind"] == L.LIGHT_POINT and np.isclose(sc.plights[0]["intensity"], 30)
+    # directional light: -z rotated +90 degrees about x is +y, scaled by the normalisation
+    d = sc.dlights[0]["transform"]
+    assert d[3] == 0 and np.allclose(d[:3] / np.linalg.norm(d[:3]), [0, 1, 0], atol=1e-6)
+
+
+def test_light_fallbacks_and_glb(tmp_path):
+    sc = gltf.load(_write(tmp_path, lights=False))
+    assert len(sc.plights) == 5 and np.all(sc.plights["intensity"] == 0)
+    assert len(sc.dlights) == 1 and sc.dlights[0]["kind"] == L.LIGHT_DIRECTIONAL and sc.dlights[0]["intensity"] == 0
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    a = gltf.load(_write(tmp_path / "a"))
+    b = gltf.load(_write(tmp_path / "b", glb=True))
+    assert a.vertices.tobytes() == b.vertices.tobytes() and a.instances.tobytes() == b.instances.tobytes()
+    assert a.materials.tobytes() == b.materials.tobytes()
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CORNELL), reason="the reference checkout is not present")
+def test_reference_cornell_box_asset():
+    sc = gltf.load(REF_CORNELL)
+    assert sc.num_triangles == 16720
+    assert len(sc.materials) == 9
+    assert int((sc.materials["alpha_mode"] == L.ALPHA_BLEND).sum()) >= 1
+    assert float(sc.materials["emissive_factor"][:, :3].max()) > 0
+    assert len(sc.plights) == 5 and np.all(sc.plights["intensity"] == 0)  # no lights defined
+    pts = sc.vertices["pos"][:, :3]
+    assert np.all(np.isfinite(pts))
+
+
+def test_room_scene_extensions_and_spot_light(tmp_path):
+    from tests import gltf_kit as GK
+    sc = gltf.load(GK.room().save(str(tmp_path / "room.gltf")), width=32, height=24)
+    m = sc.materials
+    glass = m[3]
+    assert glass["transmission_exist"] == 1 and np.isclose(glass["transmission_factor"], 0.9)
+    assert glass["volume_exists"] == 1 and np.isclose(glass["ior"], 1.45)
+    assert np.isclose(glass["attenuation_distance"], 2.0) and np.allclose(glass["attenuation_color"], [0.8, 0.9, 1.0])
+    assert m[4]["alpha_mode"] == L.ALPHA_MASK and m[4]["double_sided"] == 1
+    # the spot light is ingested as a point light at its node's origin (light.rs:51-56)
+    assert sc.plights["kind"].tolist() == [L.LIGHT_POINT, L.LIGHT_POINT]
+    assert sc.plights["intensity"].tolist() == [12.0, 10.0]
+    assert len(sc.dlights) == 1 and np.isclose(sc.dlights[0]["intensity"], 1.5)
+    # the sphere mesh has no normals: every generated normal is unit length and w = 0
+    n = sc.vertices["normal"]
+    assert np.allclose(np.linalg.norm(n[:, :3], axis=1), 1, atol=1e-6) and np.all(n[:, 3] == 0)
+    assert [t.format for t in sc.textures] == [L.FMT_RGBA8_SRGB, L.FMT_RGBA8_SRGB, L.FMT_RGBA8_UNORM, L.FMT_RGBA8_SRGB]
