@@ -165,6 +165,14 @@ typedef struct rt_bindings {
     uint32_t num_indices;
     uint32_t num_prim_infos;
     uint32_t num_materials;
+    /* binding 12 (SKYBOX_BIND, RayTracing.rmiss:10): HOST pointer to one descriptor
+     * whose data holds the six cube faces +x, -x, +y, -y, +z, -z (the order
+     * cubumap.rs:19-27 sorts the face files into), each width x width texels
+     * (width 1..16384), one after another (the reference's `collector` buffer).
+     * Sampled at LOD 0 with mag_filter and seamless cube edges; wrap modes are
+     * ignored, as for any cube image.  Read only when ubo.has_sky != 0, and
+     * required then. */
+    const rt_texture* skybox;
 } rt_bindings;
 
 enum { RT_MODE_MEGAKERNEL = 0, RT_MODE_WAVEFRONT = 1 };

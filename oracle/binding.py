@@ -48,7 +48,7 @@ class OrcSceneDesc(C.Structure):
                 ("plights", C.c_void_p), ("num_plights", C.c_uint32),
                 ("textures", C.c_void_p), ("num_textures", C.c_uint32),
                 ("blases", C.c_void_p), ("num_blases", C.c_uint32),
-                ("instances", C.c_void_p), ("num_instances", C.c_uint32)]
+                ("instances", C.c_void_p), ("num_instances", C.c_uint32), ("skybox", C.c_void_p)]
 
 
 class OrcStats(C.Structure):
@@ -86,6 +86,7 @@ def lib():
             "orc_rotate_point": (None, [dp, dp, dp]), "orc_convert_metallic": (d, [dp, dp, d]),
             "orc_frostbite_diffuse": (d, [d, d, d, d]), "orc_light_intensity_luminance": (d, [vp, d]),
             "orc_offset_ray": (None, [vp, vp, vp]), "orc_texture_sample": (None, [vp, d, d, dp]),
+            "orc_cube_sample": (None, [vp, vp, vp]),
             "orc_intersect_triangle": (i, [dp, dp, dp, dp, dp, d, d, dp]),
             "orc_scene_create": (vp, [vp]), "orc_scene_destroy": (None, [vp]),
             "orc_trace_rays": (i, [vp, vp, u, u, u, vp, vp, i, i]),
@@ -149,13 +150,21 @@ class OracleScene:
                       dlights=np.ascontiguousarray(scene.dlights), plights=np.ascontiguousarray(scene.plights),
                       geo=geo, inst=inst)
         self._keep.extend(arrays.values())
+        self._sky = None
+        sky = getattr(scene, "skybox", None)
+        if sky is not None:
+            data = np.ascontiguousarray(sky.data)
+            self._keep.append(data)
+            self._sky = RtTexture(data.ctypes.data, data.shape[2], data.shape[1], sky.format, sky.wrap_s, sky.wrap_t,
+                                  sky.mag_filter, sky.min_filter)
         a = arrays
         self.desc = OrcSceneDesc(a["vertices"].ctypes.data, len(a["vertices"]), a["indices"].ctypes.data,
                                  len(a["indices"]), a["prim_infos"].ctypes.data, len(a["prim_infos"]),
                                  a["materials"].ctypes.data, len(a["materials"]), a["dlights"].ctypes.data,
                                  len(a["dlights"]), a["plights"].ctypes.data, len(a["plights"]),
                                  C.addressof(self._textures), len(scene.textures), geo.ctypes.data, len(geo),
-                                 inst.ctypes.data, len(inst))
+                                 inst.ctypes.data, len(inst),
+                                 C.addressof(self._sky) if self._sky is not None else None)
         self.handle = lib().orc_scene_create(C.byref(self.desc))
         if not self.handle:
             raise ValueError("oracle rejected the scene (instance references out of range)")
@@ -211,6 +220,8 @@ class OracleScene:
         st = OrcStats()
         rc = lib().orc_render_pixels(self.handle, ubo.ctypes.data, w, h, xy.ctypes.data, len(xy), precision, threads,
                                      out.ctypes.data, C.byref(st))
+        if rc == -2:
+            raise ValueError("ubo.has_sky needs the scene's skybox")
         if rc != 0:
             raise ValueError("orc_render_pixels rejected the pixel list")
         stats = {n: getattr(st, n) for n, _ in OrcStats._fields_}
@@ -239,11 +250,26 @@ class OracleScene:
         st = OrcStats()
         rc = lib().orc_render(self.handle, ubo.ctypes.data, w, h, x0, y0, x1, y1, precision, threads,
                               accum.ctypes.data, C.byref(st))
+        if rc == -2:
+            raise ValueError("ubo.has_sky needs the scene's skybox")
         if rc != 0:
             raise ValueError("orc_render rejected the arguments")
         stats = {n: getattr(st, n) for n, _ in OrcStats._fields_}
         stats["rays"] = stats["primary_rays"] + stats["extension_rays"] + stats["shadow_rays"]
         return accum, stats
+
+
+def cube_sample(faces, fmt, direction, mag_filter=1):
+    """orc_cube_sample: faces (6, w, w, 4) in face order +x, -x, +y, -y, +z, -z
+    (uint8 for the RGBA8 formats, float32 for RGBA32F), one direction -> RGBA."""
+    data = np.ascontiguousarray(faces)
+    assert data.ndim == 4 and data.shape[0] == 6 and data.shape[1] == data.shape[2] and data.shape[3] == 4
+    w = data.shape[1]
+    t = RtTexture(data.ctypes.data, w, w, int(fmt), 0, 0, int(mag_filter), int(mag_filter))
+    d = np.ascontiguousarray(direction, np.float64)
+    out = np.zeros(4, np.float64)
+    lib().orc_cube_sample(C.byref(t), d.ctypes.data, out.ctypes.data)
+    return out
 
 
 def tone_map(c, mode):

@@ -444,6 +444,91 @@ inline void texture_sample(const rt_texture& t, double u, double v, double out[4
     }
 }
 
+// ------------------------------------------------------------ cube map
+// texture(samplerCube, dir) at LOD 0, RayTracing.rmiss:27-33 with the skybox of
+// cubumap.rs / globals.rs:201-240 (six square layers +x, -x, +y, -y, +z, -z).
+// Vulkan's rules: the face is the largest-magnitude axis (equal magnitudes pick z,
+// then y); (s, t) = ((sc / |ma| + 1) / 2, (tc / |ma| + 1) / 2) with the face's
+// basis below; a bilinear texel off the face is the neighbouring face's texel
+// under the same texel centre continued over the edge (seamless cube edges); a
+// texel off two edges is the mean of the three texels meeting at that corner.
+struct CubeBasis {
+    double ma[3], sc[3], tc[3];  // unit vectors: face normal, direction of growing s, of growing t
+};
+inline CubeBasis cube_basis(int face) {
+    static const CubeBasis kFaces[6] = {
+        {{1, 0, 0}, {0, 0, -1}, {0, -1, 0}},   // +x: sc = -z, tc = -y
+        {{-1, 0, 0}, {0, 0, 1}, {0, -1, 0}},   // -x: sc = +z, tc = -y
+        {{0, 1, 0}, {1, 0, 0}, {0, 0, 1}},     // +y: sc = +x, tc = +z
+        {{0, -1, 0}, {1, 0, 0}, {0, 0, -1}},   // -y: sc = +x, tc = -z
+        {{0, 0, 1}, {1, 0, 0}, {0, -1, 0}},    // +z: sc = +x, tc = -y
+        {{0, 0, -1}, {-1, 0, 0}, {0, -1, 0}},  // -z: sc = -x, tc = -y
+    };
+    return kFaces[face];
+}
+inline void cube_project(const double d[3], int& face, double& s, double& t) {
+    double ax = std::fabs(d[0]), ay = std::fabs(d[1]), az = std::fabs(d[2]);
+    if (az >= ax && az >= ay) face = d[2] < 0 ? 5 : 4;
+    else if (ay >= ax) face = d[1] < 0 ? 3 : 2;
+    else face = d[0] < 0 ? 1 : 0;
+    CubeBasis b = cube_basis(face);
+    double ma = d[0] * b.ma[0] + d[1] * b.ma[1] + d[2] * b.ma[2];  // = |major component|
+    s = 0.5 * ((d[0] * b.sc[0] + d[1] * b.sc[1] + d[2] * b.sc[2]) / ma + 1.0);
+    t = 0.5 * ((d[0] * b.tc[0] + d[1] * b.tc[1] + d[2] * b.tc[2]) / ma + 1.0);
+}
+inline void cube_texel_fetch(const rt_texture& c, int face, int i, int j, double out[4]) {
+    const int W = int(c.width);
+    const bool oi = i < 0 || i >= W, oj = j < 0 || j >= W;
+    if (oi && oj) {
+        int ic = std::min(std::max(i, 0), W - 1), jc = std::min(std::max(j, 0), W - 1);
+        double p[4], q[4], r[4];
+        cube_texel_fetch(c, face, ic, jc, p);
+        cube_texel_fetch(c, face, i, jc, q);
+        cube_texel_fetch(c, face, ic, j, r);
+        for (int k = 0; k < 4; k++) out[k] = (p[k] + q[k] + r[k]) / 3.0;
+        return;
+    }
+    if (oi || oj) {
+        // the texel centre in face units (the face spans -1..1), past the edge
+        CubeBasis b = cube_basis(face);
+        double u = (2.0 * i + 1.0) / W - 1.0, v = (2.0 * j + 1.0) / W - 1.0, p[3];
+        for (int k = 0; k < 3; k++) p[k] = b.ma[k] + u * b.sc[k] + v * b.tc[k];
+        int g;
+        double s, t;
+        cube_project(p, g, s, t);
+        cube_texel_fetch(c, g, std::min(int(std::floor(s * W)), W - 1), std::min(int(std::floor(t * W)), W - 1), out);
+        return;
+    }
+    rt_texture f = c;
+    f.height = c.width;
+    size_t texel_bytes = c.format == RT_FMT_RGBA32F ? 16 : 4;
+    f.data = (const uint8_t*)c.data + size_t(face) * W * W * texel_bytes;
+    texel_fetch(f, i, j, out);
+}
+inline void cube_sample(const rt_texture& c, const double d[3], double out[4]) {
+    int face;
+    double s, t;
+    cube_project(d, face, s, t);
+    const int W = int(c.width);
+    if (c.mag_filter == RT_FILTER_NEAREST) {
+        cube_texel_fetch(c, face, int(std::floor(s * W)), int(std::floor(t * W)), out);
+        return;
+    }
+    double fu = s * W - 0.5, fv = t * W - 0.5;
+    double iu = std::floor(fu), iv = std::floor(fv);
+    double a = fu - iu, b = fv - iv;
+    int i0 = int(iu), j0 = int(iv);
+    double t00[4], t10[4], t01[4], t11[4];
+    cube_texel_fetch(c, face, i0, j0, t00);
+    cube_texel_fetch(c, face, i0 + 1, j0, t10);
+    cube_texel_fetch(c, face, i0, j0 + 1, t01);
+    cube_texel_fetch(c, face, i0 + 1, j0 + 1, t11);
+    for (int k = 0; k < 4; k++) {
+        double top = t00[k] + a * (t10[k] - t00[k]), bot = t01[k] + a * (t11[k] - t01[k]);
+        out[k] = top + b * (bot - top);
+    }
+}
+
 // ------------------------------------------------- ray/triangle (watertight)
 // Woop, Benthin & Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013.
 // Returns the Vulkan barycentrics (u, v) = weights of vertices 1 and 2.
@@ -597,6 +682,8 @@ struct Scene {
     // with these textures while lobe choices, directions and densities keep using
     // d.textures.
     std::vector<rt_texture> eval_tex;
+    rt_texture sky{};  // copy of *d.skybox
+    bool has_sky_tex = false;
     std::vector<std::vector<double>> tri_pos;  // per BLAS: 9 doubles per triangle
     std::vector<Bvh> blas_bvh;
     std::vector<Box> blas_box;
@@ -628,6 +715,10 @@ inline void xform_dir(const double m[12], const double p[3], double out[3]) {
 Scene* build_scene(const orc_scene_desc* desc) {
     auto* s = new Scene();
     s->d = *desc;
+    if (desc->skybox) {
+        s->sky = *desc->skybox;
+        s->has_sky_tex = true;
+    }
     uint32_t nb = desc->num_blases;
     s->tri_pos.resize(nb);
     s->blas_bvh.resize(nb);
@@ -959,8 +1050,13 @@ template <class R> struct Tracer {
                 if (c < R(0)) acc = acc + load3<R>(li.color) * (-c * R(li.intensity));
             }
         }
-        // HasSky (skybox cube) is not supported: the constant term of the sky-off branch.
-        acc = acc + V3<R>(R(0.01));
+        if (ubo.has_sky) {  // rmiss:27-33: light_acc += skyColor + light_acc
+            double dd[3] = {double(rd.x), double(rd.y), double(rd.z)}, c[4];
+            cube_sample(s.sky, dd, c);
+            acc = acc + (V3<R>(R(c[0]), R(c[1]), R(c[2])) + acc);
+        } else {
+            acc = acc + V3<R>(R(0.01));
+        }
         ray.hit_value = V3<R>(R(0));
         ray.hit_value_s = ray.hit_value;
         ray.need_scatter = false;
@@ -1341,6 +1437,7 @@ double orc_frostbite_diffuse(double r, double ndl, double ndv, double ldh) { ret
 double orc_light_intensity_luminance(const rt_light* l, double distance) { return luminance(light_intensity_at<double>(*l, distance)); }
 void orc_offset_ray(const float p[3], const float n[3], float out[3]) { offset_ray_f(p, n, out); }
 void orc_texture_sample(const rt_texture* tex, double u, double v, double out[4]) { texture_sample(*tex, u, v, out); }
+void orc_cube_sample(const rt_texture* cube, const double dir[3], double out[4]) { cube_sample(*cube, dir, out); }
 int orc_intersect_triangle(const double o[3], const double d[3], const double v0[3], const double v1[3],
                            const double v2[3], double tmin, double tmax, double out[3]) {
     WoopRay w = woop_setup(d);
@@ -1400,6 +1497,7 @@ int orc_trace_rays(void* sp, const float* rays, uint32_t n, uint32_t flags, uint
 int orc_render_pixels(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, const uint32_t* xy, uint32_t n,
                       int precision, int threads, double* out, orc_stats* stats) {
     const Scene& s = *(Scene*)sp;
+    if (ubo->has_sky && !s.has_sky_tex) return -2;
     if (threads > 0) omp_set_num_threads(threads);
     for (uint32_t i = 0; i < n; i++)
         if (xy[2 * i] >= W || xy[2 * i + 1] >= H) return -1;
@@ -1479,6 +1577,7 @@ int orc_render(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, uint32_t x0,
                int precision, int threads, double* accum, orc_stats* stats) {
     const Scene& s = *(Scene*)sp;
     if (x1 > W || y1 > H || x0 > x1 || y0 > y1) return -1;
+    if (ubo->has_sky && !s.has_sky_tex) return -2;
     if (threads > 0) omp_set_num_threads(threads);
     bool reset = ubo->number_of_samples == ubo->total_number_of_samples;
     uint32_t w = x1 - x0;

@@ -42,6 +42,7 @@ typedef struct orc_scene_desc {
     const rt_texture* textures; uint32_t num_textures;
     const orc_blas* blases; uint32_t num_blases;
     const rt_instance* instances; uint32_t num_instances;
+    const rt_texture* skybox;  /* six square faces +x, -x, +y, -y, +z, -z, or NULL */
 } orc_scene_desc;
 
 typedef struct orc_stats {
@@ -96,6 +97,8 @@ void orc_offset_ray(const float p[3], const float n[3], float out[3]);
 
 /* --- texture sampling (LOD 0), host texture --- */
 void orc_texture_sample(const rt_texture* tex, double u, double v, double out[4]);
+/* cube map (six square faces +x, -x, +y, -y, +z, -z, one after another) at LOD 0 */
+void orc_cube_sample(const rt_texture* cube, const double dir[3], double out[4]);
 
 /* --- ray/triangle (Woop, Benthin & Wald 2013), FP64 --- */
 int orc_intersect_triangle(const double o[3], const double d[3], const double v0[3],

@@ -181,6 +181,15 @@ int validate_bindings(rt_ctx* ctx, const rt_bindings* b) {
             t.wrap_t > RT_WRAP_MIRRORED_REPEAT || t.mag_filter > RT_FILTER_LINEAR || t.min_filter > RT_FILTER_LINEAR)
             return fail(ctx, RT_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has an invalid descriptor");
     }
+    if (b->ubo->has_sky) {
+        const rt_texture* t = b->skybox;
+        if (!t) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "ubo.has_sky needs the skybox binding");
+        if (!t->data || !t->width || t->width != t->height || t->width > kMaxCubeFace || t->format > RT_FMT_RGBA32F ||
+            t->mag_filter > RT_FILTER_LINEAR)
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT,
+                        "the skybox needs square faces of 1.." + std::to_string(kMaxCubeFace) +
+                            " texels and a valid format and filter");
+    }
     return RT_OK;
 }
 
@@ -226,6 +235,8 @@ int make_frame(rt_ctx* ctx, const TlasEntry& tl, const rt_bindings* b, float* ac
     sv.num_indices = b->num_indices;
     sv.num_prim_infos = b->num_prim_infos;
     sv.num_materials = b->num_materials;
+    sv.has_sky = u.has_sky ? 1u : 0u;
+    if (sv.has_sky) sv.sky = *b->skybox;
     fa->accum = accum;
     fa->width = w;
     fa->height = h;
@@ -786,7 +797,6 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "the display image needs total_number_of_samples > 0");
     auto it = ctx->tlas.find(tlas);
     if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
-    if (b->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, "HasSky (skybox miss) is not supported");
     DeviceGuard g(ctx->device);
     rc = upload_textures(ctx, b);
     if (rc) return rc;
@@ -838,7 +848,6 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
         if (it.width == 0 || it.height == 0 || uint64_t(it.width) * it.height > (1ull << 31))
             return fail(ctx, RT_ERR_INVALID_ARGUMENT, where + "width/height out of range");
         if (ctx->tlas.find(it.tlas) == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, where + "unknown TLAS handle");
-        if (it.bindings->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, where + "HasSky (skybox miss) is not supported");
         ntex += it.bindings->num_textures;
     }
     if (count == 0) return RT_OK;
@@ -915,7 +924,6 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
     auto it = ctx->tlas.find(tlas);
     if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
-    if (b->ubo->has_sky) return fail(ctx, RT_ERR_UNSUPPORTED, "HasSky (skybox miss) is not supported");
     if (b->ubo->number_of_bounces > kMaxAdjointBounces)
         return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint replays at most 16 bounces");
     const uint32_t nt = b->num_textures;

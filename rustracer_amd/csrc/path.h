@@ -140,7 +140,7 @@ struct Payload {
     double volume_dis;
 };
 
-// RayTracing.rmiss:14-45 (skybox off: HasSky is not supported)
+// RayTracing.rmiss:14-45
 __device__ inline void miss_shader(const SceneView& sv, Payload& ray, D3 dir) {
     D3 acc = d3(0.0);
     D3 rd = dnormalize(dir);
@@ -151,7 +151,13 @@ __device__ inline void miss_shader(const SceneView& sv, Payload& ray, D3 dir) {
             if (c < 0.0) acc = acc + ld3(li.color) * (-c * double(li.intensity));
         }
     }
-    acc = acc + d3(0.01);
+    if (sv.has_sky) {
+        // rmiss:27-33: light_acc += skyColor + light_acc
+        const D4 sky = sample_cube(sv.sky, sv.srgb_lut, rd);
+        acc = acc + (xyz(sky) + acc);
+    } else {
+        acc = acc + d3(0.01);
+    }
     ray.hit_value = d3(0.0);
     ray.need_scatter = false;
     ray.emittance = sv.num_dlights == 0 ? d3(0.0) : acc;

@@ -80,6 +80,8 @@ struct SceneView {
     const float* srgb_lut;       // 256 entries
     uint32_t num_dlights, num_plights, num_textures;
     uint32_t num_vertices, num_indices, num_prim_infos, num_materials;
+    uint32_t has_sky;            // ubo.HasSky
+    rt_texture sky;              // six square faces +x, -x, +y, -y, +z, -z
 };
 
 struct D4 {
@@ -107,8 +109,7 @@ __device__ inline int wrap_coord(int i, int size, uint32_t mode) {
     return t < 0 ? t + size : t;
 }
 
-__device__ inline D4 texel(const rt_texture& t, const float* lut, int x, int y) {
-    size_t idx = size_t(y) * t.width + size_t(x);
+__device__ inline D4 texel_at(const rt_texture& t, const float* lut, size_t idx) {
     if (t.format == RT_FMT_RGBA32F) {
         float4 f = ldg_f4(reinterpret_cast<const float4*>(t.data) + idx);
         return D4{f.x, f.y, f.z, f.w};
@@ -117,6 +118,9 @@ __device__ inline D4 texel(const rt_texture& t, const float* lut, int x, int y) 
     uint32_t r = v & 255u, g = (v >> 8) & 255u, b = (v >> 16) & 255u, a = v >> 24;
     if (t.format == RT_FMT_RGBA8_SRGB) return D4{lut[r], lut[g], lut[b], a / 255.0};
     return D4{r / 255.0, g / 255.0, b / 255.0, a / 255.0};
+}
+__device__ inline D4 texel(const rt_texture& t, const float* lut, int x, int y) {
+    return texel_at(t, lut, size_t(y) * t.width + size_t(x));
 }
 
 __device__ inline double bilerp(double t00, double t10, double t01, double t11, double a, double b) {
@@ -145,6 +149,119 @@ __device__ inline D4 sample_texture(const SceneView& sv, int32_t index, double u
     // the rounding or contraction of the arithmetic
     return D4{bilerp(t00.x, t10.x, t01.x, t11.x, a, b), bilerp(t00.y, t10.y, t01.y, t11.y, a, b),
               bilerp(t00.z, t10.z, t01.z, t11.z, a, b), bilerp(t00.w, t10.w, t01.w, t11.w, a, b)};
+}
+
+// --------------------------------------------------------------- cube map
+// texture(samplerCube, dir) at LOD 0 for the skybox (RayTracing.rmiss:27-33), by the
+// Vulkan rules: the face is the direction's largest-magnitude axis, (s, t) come
+// from the face table below, bilinear texels that fall off the face are taken from
+// the neighbouring face (seamless edges) and a texel off both edges is the mean of
+// the three texels that meet at that cube corner.  Equal magnitudes pick z, then y
+// (the order AMD's cube-face instruction uses).
+//   face  major  sc   tc
+//   0     +x     -z   -y
+//   1     -x     +z   -y
+//   2     +y     +x   +z
+//   3     -y     +x   -z
+//   4     +z     +x   -y
+//   5     -z     -x   -y
+constexpr uint32_t kMaxCubeFace = 16384;  // face size limit: the edge projection's W (2W + 2) fits in int
+
+struct CubeAxes {
+    int ma, sc, tc;         // axis index (0 x, 1 y, 2 z)
+    int ma_s, sc_s, tc_s;   // +1 / -1
+};
+__device__ inline CubeAxes cube_axes(int face) {
+    switch (face) {
+        case 0: return CubeAxes{0, 2, 1, 1, -1, -1};
+        case 1: return CubeAxes{0, 2, 1, -1, 1, -1};
+        case 2: return CubeAxes{1, 0, 2, 1, 1, 1};
+        case 3: return CubeAxes{1, 0, 2, -1, 1, -1};
+        case 4: return CubeAxes{2, 0, 1, 1, 1, -1};
+        default: return CubeAxes{2, 0, 1, -1, -1, -1};
+    }
+}
+template <class T> __device__ inline int cube_face_of(T x, T y, T z) {
+    const T ax = x < T(0) ? -x : x, ay = y < T(0) ? -y : y, az = z < T(0) ? -z : z;
+    if (az >= ax && az >= ay) return z < T(0) ? 5 : 4;
+    if (ay >= ax) return y < T(0) ? 3 : 2;
+    return x < T(0) ? 1 : 0;
+}
+template <class T> __device__ inline T pick3(T x, T y, T z, int axis) { return axis == 0 ? x : (axis == 1 ? y : z); }
+
+// Linear texel index of (i, j) on `face` when exactly one of i, j is one texel off
+// the face: the texel centre, as an integer point of the cube scaled by W (a face
+// spans -W..W), has its largest component W + 1 on the neighbouring face's axis;
+// project it there.
+__device__ inline uint32_t cube_edge_index(int W, int face, int i, int j) {
+    const CubeAxes f = cube_axes(face);
+    const int a = f.ma_s * W, b = f.sc_s * (2 * i + 1 - W), c = f.tc_s * (2 * j + 1 - W);
+    const int px = f.ma == 0 ? a : (f.sc == 0 ? b : c);
+    const int py = f.ma == 1 ? a : (f.sc == 1 ? b : c);
+    const int pz = f.ma == 2 ? a : (f.sc == 2 ? b : c);
+    const int g = cube_face_of(px, py, pz);
+    const CubeAxes h = cube_axes(g);
+    const int m = W + 1;
+    const int sc = h.sc_s * pick3(px, py, pz, h.sc), tc = h.tc_s * pick3(px, py, pz, h.tc);
+    return (uint32_t(g) * uint32_t(W) + uint32_t((W * (tc + m)) / (2 * m))) * uint32_t(W) +
+           uint32_t((W * (sc + m)) / (2 * m));
+}
+
+// Texel (i, j) of `face`, i and j at most one texel off it.  The index arithmetic
+// runs first and the fetches share one loop, so the shading kernels that inline
+// the miss shader keep their register budget.
+__device__ inline D4 cube_texel(const rt_texture& t, const float* lut, int face, int i, int j) {
+    const int W = int(t.width);
+    const bool oi = i < 0 || i >= W, oj = j < 0 || j >= W;
+    uint32_t idx0, idx1 = 0, idx2 = 0;
+    int n = 1;
+    if (!oi && !oj) {
+        idx0 = (uint32_t(face) * uint32_t(W) + uint32_t(j)) * uint32_t(W) + uint32_t(i);
+    } else if (oi != oj) {
+        idx0 = cube_edge_index(W, face, i, j);
+    } else {
+        const int ic = min(max(i, 0), W - 1), jc = min(max(j, 0), W - 1);
+        idx0 = (uint32_t(face) * uint32_t(W) + uint32_t(jc)) * uint32_t(W) + uint32_t(ic);
+        idx1 = cube_edge_index(W, face, i, jc);
+        idx2 = cube_edge_index(W, face, ic, j);
+        n = 3;
+    }
+    D4 acc = D4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int k = 0; k < n; k++) {
+        const D4 v = texel_at(t, lut, k == 0 ? idx0 : (k == 1 ? idx1 : idx2));
+        acc = D4{acc.x + v.x, acc.y + v.y, acc.z + v.z, acc.w + v.w};
+    }
+    if (n == 3) acc = D4{acc.x / 3.0, acc.y / 3.0, acc.z / 3.0, acc.w / 3.0};
+    return acc;
+}
+
+__device__ inline D4 sample_cube(const rt_texture& t, const float* lut, D3 d) {
+    const int face = cube_face_of(d.x, d.y, d.z);
+    const CubeAxes f = cube_axes(face);
+    const double ma = fabs(pick3(d.x, d.y, d.z, f.ma));
+    const double s = 0.5 * (f.sc_s * pick3(d.x, d.y, d.z, f.sc) / ma + 1.0);
+    const double u = 0.5 * (f.tc_s * pick3(d.x, d.y, d.z, f.tc) / ma + 1.0);
+    const int W = int(t.width);
+    if (t.mag_filter == RT_FILTER_NEAREST)
+        return cube_texel(t, lut, face, min(int(floor(s * W)), W), min(int(floor(u * W)), W));
+    const double fu = s * W - 0.5, fv = u * W - 0.5;
+    const double iu = floor(fu), iv = floor(fv);
+    const double a = fu - iu, b = fv - iv;
+    const int i0 = int(iu), j0 = int(iv);
+    // two rows of two texels, each row lerped in a, then the rows in b: the nested
+    // lerps of sample_texture
+    D4 top = D4{0.0, 0.0, 0.0, 0.0}, bot = top;
+#pragma unroll 1
+    for (int r = 0; r < 2; r++) {
+        const D4 x0 = cube_texel(t, lut, face, i0, j0 + r), x1 = cube_texel(t, lut, face, i0 + 1, j0 + r);
+        const D4 row = D4{x0.x + a * (x1.x - x0.x), x0.y + a * (x1.y - x0.y), x0.z + a * (x1.z - x0.z),
+                          x0.w + a * (x1.w - x0.w)};
+        if (r == 0) top = row;
+        else bot = row;
+    }
+    return D4{top.x + b * (bot.x - top.x), top.y + b * (bot.y - top.y), top.z + b * (bot.z - top.z),
+              top.w + b * (bot.w - top.w)};
 }
 
 // getUV, lib/RayTracingCommons.glsl:65-72

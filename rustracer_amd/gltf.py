@@ -240,10 +240,37 @@ def _linear_images(js):
     return out
 
 
+_FACE_NAMES = {"posx": 0, "negx": 1, "posy": 2, "negy": 3, "posz": 4, "negz": 5,
+               "right": 0, "left": 1, "top": 2, "bottom": 3, "front": 4, "back": 5}
+
+
+def load_skybox(directory):
+    """A skybox directory as SkyBox::new reads it (crates/libs/asset_loader/src/cubumap.rs:
+    19-50, 85-110): its six .png / .jpg files ordered by face (posx negx posy negy posz
+    negz, or right left top bottom front back), decoded to RGBA8, sRGB, with the
+    default sampler (linear).  Returns a scenes.Texture with data (6, w, w, 4)."""
+    from PIL import Image
+    files = sorted(f for f in os.listdir(directory) if f.endswith(".png") or f.endswith(".jpg"))
+    if len(files) != 6:
+        raise ValueError(f"a skybox needs six face images, found {len(files)} in {directory}")
+    faces = [None] * 6
+    for f in files:
+        stem = f.split(".")[0]
+        if stem not in _FACE_NAMES:
+            raise ValueError(f"unknown skybox face name {f}")
+        with Image.open(os.path.join(directory, f)) as im:
+            faces[_FACE_NAMES[stem]] = np.asarray(im.convert("RGBA"), np.uint8)
+    if any(x is None for x in faces) or len({x.shape for x in faces}) != 1 or faces[0].shape[0] != faces[0].shape[1]:
+        raise ValueError("a skybox needs six distinct square faces of one size")
+    return S.Texture(np.stack(faces), L.FMT_RGBA8_SRGB)
+
+
 def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), target=(0.0, 0.0, 0.0),
-         fov=60.0, znear=0.1, zfar=10.0, seed=0):
+         fov=60.0, znear=0.1, zfar=10.0, seed=0, skybox=None):
     """Loads a .gltf / .glb file into a Scene; the camera defaults to the viewer's
-    (crates/libs/app/src/lib.rs:330-337: at (0, 0, 1) looking down -z, 60 degrees)."""
+    (crates/libs/app/src/lib.rs:330-337: at (0, 0, 1) looking down -z, 60 degrees).
+    `skybox` (a directory for load_skybox, or a scenes.Texture with data (6, w, w, 4))
+    becomes binding 12 and sets ubo.HasSky."""
     doc = Document(path)
     js = doc.json
     if js.get("skins"):
@@ -383,4 +410,8 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
     proj = S.perspective_vk(width / height, fov, znear, zfar)
     ubo = S.make_ubo(view, proj, samples=spp, bounces=bounces, random_seed=seed, fully_opaque=fully_opaque)
     name = os.path.splitext(os.path.basename(path))[0]
-    return b.finish(name, inst, pl, dl, ubo, width, height, textures)
+    sc = b.finish(name, inst, pl, dl, ubo, width, height, textures)
+    if skybox is not None:
+        sc.skybox = load_skybox(skybox) if isinstance(skybox, str) else skybox
+        sc.ubo["has_sky"] = 1
+    return sc

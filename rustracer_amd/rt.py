@@ -64,7 +64,8 @@ class Bindings(C.Structure):
                 ("prim_infos", C.c_void_p), ("materials", C.c_void_p), ("dlights", C.c_void_p),
                 ("num_dlights", C.c_uint32), ("plights", C.c_void_p), ("num_plights", C.c_uint32),
                 ("textures", C.c_void_p), ("num_textures", C.c_uint32), ("num_vertices", C.c_uint32),
-                ("num_indices", C.c_uint32), ("num_prim_infos", C.c_uint32), ("num_materials", C.c_uint32)]
+                ("num_indices", C.c_uint32), ("num_prim_infos", C.c_uint32), ("num_materials", C.c_uint32),
+                ("skybox", C.c_void_p)]
 
 
 class Tuning(C.Structure):
@@ -233,6 +234,16 @@ class Buffers:
             h, w = t.data.shape[:2]
             self._tex[k] = Texture(self.textures[k].data_ptr(), w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter,
                                    t.min_filter)
+        # skybox (binding 12): six square faces +x, -x, +y, -y, +z, -z, end to end
+        self.skybox, self._sky = None, None
+        sky = getattr(scene, "skybox", None)
+        if sky is not None:
+            if sky.data.ndim != 4 or sky.data.shape[0] != 6 or sky.data.shape[1] != sky.data.shape[2]:
+                raise ValueError("the skybox needs six square faces, shape (6, w, w, 4)")
+            self.skybox = to_device(sky.data, device)
+            w = sky.data.shape[1]
+            self._sky = Texture(self.skybox.data_ptr(), w, w, sky.format, sky.wrap_s, sky.wrap_t, sky.mag_filter,
+                                sky.min_filter)
 
     def set_texture(self, k, tensor):
         """Points texture slot k at `tensor` (device, (h, w, 4), the slot's format:
@@ -257,7 +268,8 @@ class Buffers:
                         self.dlights.data_ptr() if self.dlights is not None else None, len(s.dlights),
                         self.plights.data_ptr() if self.plights is not None else None, len(s.plights),
                         C.addressof(self._tex) if s.textures else None, len(s.textures), len(s.vertices),
-                        len(s.indices), len(s.prim_infos), len(s.materials))
+                        len(s.indices), len(s.prim_infos), len(s.materials),
+                        C.addressof(self._sky) if self._sky is not None else None)
 
 
 class TopAS:

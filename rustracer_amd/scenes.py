@@ -100,6 +100,7 @@ class Scene:
     width: int
     height: int
     textures: list = field(default_factory=list)
+    skybox: object = None       # Texture with data (6, w, w, 4): faces +x, -x, +y, -y, +z, -z
 
     @property
     def num_triangles(self):

@@ -16,6 +16,7 @@ import pytest
 from rustracer_amd import gltf, layouts as L
 
 REF_CORNELL = "/root/reference/assets/models/CornellBox/cornellBox.gltf"
+REF_SKYBOX = "/root/reference/assets/skyboxs/Yokohama"
 
 
 def _png(rgba):
@@ -212,3 +213,26 @@ def test_room_scene_extensions_and_spot_light(tmp_path):
     n = sc.vertices["normal"]
     assert np.allclose(np.linalg.norm(n[:, :3], axis=1), 1, atol=1e-6) and np.all(n[:, 3] == 0)
     assert [t.format for t in sc.textures] == [L.FMT_RGBA8_SRGB, L.FMT_RGBA8_SRGB, L.FMT_RGBA8_UNORM, L.FMT_RGBA8_SRGB]
+
+
+def test_skybox_directory_face_order(tmp_path):
+    """cubumap.rs:19-50: files named posx..negz or right/left/top/bottom/front/back
+    are ordered +x, -x, +y, -y, +z, -z whatever the directory order."""
+    from PIL import Image
+    names = ["back", "front", "bottom", "top", "left", "right"]  # -z +z -y +y -x +x
+    layer = {"right": 0, "left": 1, "top": 2, "bottom": 3, "front": 4, "back": 5}
+    for n in names:
+        img = np.zeros((4, 4, 3), np.uint8)
+        img[..., 0] = 10 * layer[n] + 5
+        Image.fromarray(img).save(tmp_path / f"{n}.png")
+    sky = gltf.load_skybox(str(tmp_path))
+    assert sky.data.shape == (6, 4, 4, 4) and sky.format == L.FMT_RGBA8_SRGB
+    assert sky.data[:, 0, 0, 0].tolist() == [5, 15, 25, 35, 45, 55] and np.all(sky.data[..., 3] == 255)
+    sc = gltf.load(_write(tmp_path), skybox=str(tmp_path))
+    assert sc.ubo["has_sky"] == 1 and sc.skybox.data.shape == (6, 4, 4, 4)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SKYBOX), reason="the reference checkout is not present")
+def test_reference_skybox_directory():
+    sky = gltf.load_skybox(REF_SKYBOX)
+    assert sky.data.shape[0] == 6 and sky.data.shape[1] == sky.data.shape[2] and sky.data.shape[3] == 4

@@ -72,10 +72,16 @@ def test_trace_argument_checks(gpu_ctx):
     b2 = ds.buffers.bindings(sc.ubo)
     b2.num_materials = 0
     assert lib.rt_trace_rays(h, ds.tlas.handle, C.byref(b2), C.c_void_p(acc.data_ptr()), None, 8, 8, 0, None) == -1
+    # HasSky needs the skybox binding, with square faces
     sky = sc.ubo.copy()
     sky["has_sky"] = 1
     b3 = ds.buffers.bindings(sky)
-    assert lib.rt_trace_rays(h, ds.tlas.handle, C.byref(b3), C.c_void_p(acc.data_ptr()), None, 8, 8, 0, None) == -6
+    assert lib.rt_trace_rays(h, ds.tlas.handle, C.byref(b3), C.c_void_p(acc.data_ptr()), None, 8, 8, 0, None) == -1
+    assert b"skybox" in lib.rt_last_error(h)
+    faces = rt.to_device(np.zeros((6, 2, 4, 4), np.uint8))
+    not_square = rt.Texture(faces.data_ptr(), 4, 2, L.FMT_RGBA8_SRGB, 0, 0, 1, 1)
+    b3.skybox = C.cast(C.pointer(not_square), C.c_void_p)
+    assert lib.rt_trace_rays(h, ds.tlas.handle, C.byref(b3), C.c_void_p(acc.data_ptr()), None, 8, 8, 0, None) == -1
 
 
 def test_out_of_range_material_is_reported_not_read(gpu_ctx):
