@@ -217,6 +217,22 @@ def batch_benchmark(ctx, steps):
     ctx.sync()
     st = rt.trace_rays_batch(ctx, items, stats=True)
     rays = st["rays"]
+    # traversal variant for the batch (the context's was tuned on the single frame)
+    t_var = {}
+    for v in range(rt.NUM_TRAVERSAL_VARIANTS):
+        ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, v)
+        try:
+            rt.trace_rays_batch(ctx, items)
+            ctx.sync()
+        except rt.RtError:
+            continue  # a ring too shallow for one of the scenes
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rt.trace_rays_batch(ctx, items)
+        torch.cuda.synchronize()
+        t_var[v] = time.perf_counter() - t0
+    variant = min(t_var, key=t_var.get)
+    ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, variant)
     t_graph = {}
     for graph in (0, 1):
         ctx.set_option(rt.OPTION_GRAPH, graph)
@@ -245,7 +261,7 @@ def batch_benchmark(ctx, steps):
             "batch_ms": round(tb * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
             "batch_ms_eager": round(t_graph[0] * 1e3, 2), "batch_ms_graph": round(t_graph[1] * 1e3, 2),
             "batch_mrays_per_s": round(rays / tb / 1e6, 1), "sequential_mrays_per_s": round(rays / t_seq / 1e6, 1),
-            "scene_gen_s": round(gen_s, 1)}
+            "traversal_variant": variant, "scene_gen_s": round(gen_s, 1)}
 
 
 def main():

@@ -250,9 +250,10 @@ enum {
      * and barycentrics as the two-level path.  Applies to TLASes built afterwards. */
     RT_OPTION_FLATTEN_IDENTITY_INSTANCES = 2,
     /* Wavefront traversal kernels: 0 (default) LDS stack ring of 32 entries,
-     * registers for 5 waves/SIMD; 1: 16 entries, 6 waves; 2: 16 entries, 8 waves;
-     * 3: 64 entries (LDS holds 2 waves/SIMD).  Results are identical; speed differs by scene (a
-     * ring too shallow for the BVH depth restarts traversals). */
+     * registers for 3 waves/SIMD (no spills); 1: 32 entries, 4 waves; 2: 32
+     * entries, 5 waves (spills); 3: 64 entries (LDS holds 2 waves/SIMD).  Results
+     * are identical; speed differs by scene (a ring too shallow for the BVH depth
+     * restarts traversals). */
     RT_OPTION_TRAVERSAL_VARIANT = 3,
     /* 1: the wavefront trace captures one chunk of its queue iterations (8 x
      * extend, shade, shadow, resolve) as a HIP graph and replays it until no path
@@ -297,9 +298,10 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 /* Autotuner: traces the frame (the bindings' UBO with at most 8 samples per pixel)
  * in wavefront mode and tunes, one after another, the traversal variant (stack
  * depth, register target), the number of concurrent slot sets, the live-path
- * count that hands the tail to one finishing launch, the refill threshold and
- * graph replay, keeping the fastest settings on this context (the RT_OPTION_*
- * values).
+ * count that hands the tail to one finishing launch, the refill threshold,
+ * graph replay, and the traversal variant once more under those settings,
+ * keeping the fastest settings on this context (the RT_OPTION_* values).  Each
+ * probe times the faster of two frames after a warm-up frame.
  * A variant whose stack ring gives up on a ray of this frame (restart budget) is
  * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */
 typedef struct rt_tuning {

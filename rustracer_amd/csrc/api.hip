@@ -996,23 +996,30 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
     const int keep_streams = ctx->streams;
     const uint32_t keep_finish = ctx->finish;
-    // one probe: frame time of the current settings, or < 0 when a traversal gave up
+    // one probe: the faster of two timed frames of the current settings (after a
+    // warm-up frame), or < 0 when a traversal gave up
     auto probe = [&](double& ms_out) -> int {
         int r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
         if (r) return r;
-        hipError_t er = hipEventRecord(ctx->ev0, ctx->stream);
-        if (er == hipSuccess)
-            r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
-        if (r) return r;
-        if (er == hipSuccess) er = hipEventRecord(ctx->ev1, ctx->stream);
-        if (er == hipSuccess) er = hipEventSynchronize(ctx->ev1);
-        float ms = 0.f;
-        if (er == hipSuccess) er = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        hipError_t er = hipSuccess;
+        float best_ms = -1.f;
+        for (int rep = 0; rep < 2 && er == hipSuccess; rep++) {
+            er = hipEventRecord(ctx->ev0, ctx->stream);
+            if (er == hipSuccess)
+                r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT,
+                                  nullptr);
+            if (r) return r;
+            if (er == hipSuccess) er = hipEventRecord(ctx->ev1, ctx->stream);
+            if (er == hipSuccess) er = hipEventSynchronize(ctx->ev1);
+            float ms = 0.f;
+            if (er == hipSuccess) er = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+            if (er == hipSuccess && (best_ms < 0.f || ms < best_ms)) best_ms = ms;
+        }
         uint32_t flags = 0;
         if (er == hipSuccess) er = hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost);
         if (er == hipSuccess && flags) er = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
         if (er != hipSuccess) return hip_fail(ctx, er, "autotune probe");
-        ms_out = flags ? -1.0 : double(ms);
+        ms_out = flags ? -1.0 : double(best_ms);
         return RT_OK;
     };
     double best = -1.0;
@@ -1077,6 +1084,19 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
             if (consider(ms)) best = ms;
             else ctx->use_graph = false;
         }
+    }
+    // 6. the traversal variants again, under the settings chosen after them (the
+    //    slot sets change how many rays each trace launch holds)
+    if (rc == RT_OK) {
+        int chosen = ctx->variant;
+        for (int v = 0; v < wavefront_num_variants() && rc == RT_OK; v++) {
+            if (v == chosen) continue;
+            ctx->variant = v;
+            double ms = -1.0;
+            rc = probe(ms);
+            if (rc == RT_OK && consider(ms)) { best = ms; chosen = v; }
+        }
+        ctx->variant = chosen;
     }
     ctx->stage_timing = timing;
     if (rc) {

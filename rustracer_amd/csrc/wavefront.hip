@@ -214,11 +214,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_init(Frames fr, WfArrays a, uin
 struct TravVariant {
     int depth, waves;
 };
-constexpr TravVariant kTravVariants[] = {{32, 5}, {16, 6}, {16, 8}, {64, 2}};
+constexpr TravVariant kTravVariants[] = {{32, 3}, {32, 4}, {32, 5}, {64, 2}};
 constexpr int kNumTravVariants = int(sizeof(kTravVariants) / sizeof(kTravVariants[0]));
 // The any-hit kernels (alpha-tested geometry) hold more state: a variant's target
-// maps to 3 waves (no spills), 4 for the 16-deep variants (spills), or 2.
-constexpr int anyhit_waves(int w) { return w >= 6 ? 4 : (w < 3 ? w : 3); }
+// maps to 3 waves (no spills) or 2.
+constexpr int anyhit_waves(int w) { return w < 3 ? w : 3; }
 
 // The ray a trace-kernel lane works on, in fp64 for instance entry, re-read from the
 // slot (keeps it out of the traversal's registers): an extension ray from the fp64
