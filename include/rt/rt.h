@@ -232,7 +232,8 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
  * (the paths are replayed exactly), so the result is the derivative of the
  * image's estimate with the sampling detached.  d_grad_accum is the device
  * array dLoss/d(accum), width * height * 4 floats (alpha ignored).  At most 16
- * bounces (RT_ERR_UNSUPPORTED above).  Replaces the reference's
+ * bounces, and ubo.debug == 1 (the legacy materials) is not differentiated
+ * (RT_ERR_UNSUPPORTED for both).  Replaces the reference's
  * record_raytracing_commands for an optimisation loop; there is no adjoint in
  * the reference (crates/examples/gltf_viewer/src/main.rs:244-270 traces only). */
 int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const float* d_grad_accum,

@@ -129,6 +129,21 @@ inline void lcg_unit_disk(uint32_t& seed, float& px, float& py) {
         if (px * px + py * py < 1.0f) return;
     }
 }
+// RandomInUnitSphere, lib/Random.glsl:56-67.  The components 2u - 1 are exact in
+// fp32 and their squares sum exactly in fp64, so the acceptance test is exact.  Cut
+// at 256 tries (with a rejection rate of 1 - pi/6 never reached).
+inline void lcg_unit_sphere(uint32_t& seed, double p[3]) {
+    for (int k = 0; k < 256; k++) {
+        double x = 2.0 * double(lcg_float(seed)) - 1.0;
+        double y = 2.0 * double(lcg_float(seed)) - 1.0;
+        double z = 2.0 * double(lcg_float(seed)) - 1.0;
+        if (x * x + y * y + z * z < 1.0) {
+            p[0] = x; p[1] = y; p[2] = z;
+            return;
+        }
+    }
+    p[0] = p[1] = p[2] = 0.0;
+}
 inline uint32_t stream_b_seed(uint32_t random_seed, uint32_t frame) { return tea16(random_seed, frame); }
 
 // ------------------------------------------------------------ BRDF (PBR.glsl)
@@ -289,6 +304,11 @@ template <class R> V3<R> sample_ggx_vndf(V3<R> Ve, R ax, R ay, R ux, R uy) {  //
     return normalize(V3<R>(ax * Nh.x, ay * Nh.y, std::max(R(0), Nh.z)));
 }
 template <class R> V3<R> reflect(V3<R> I, V3<R> N) { return I - N * (R(2) * dot(N, I)); }
+template <class R> R schlick(R cosine, R ri) {  // PBR.glsl:48-53
+    R r0 = (R(1) - ri) / (R(1) + ri);
+    r0 = r0 * r0;
+    return r0 + (R(1) - r0) * std::pow(R(1) - cosine, R(5));
+}
 template <class R> V3<R> refract(V3<R> I, V3<R> N, R eta) {  // GLSL refract
     R k = R(1) - eta * eta * (R(1) - dot(N, I) * dot(N, I));
     if (k < R(0)) return V3<R>(R(0));
@@ -1267,6 +1287,33 @@ template <class R> struct Tracer {
         ray.scatter_direction = dir;
         ray.hit_value = thr;
         ray.rng = rng;
+        if (ubo.debug == 1) {  // rchit:437-475, from the unnormalised gl_WorldRayDirectionEXT
+            double u3[3];
+            if (trans > R(0)) {
+                R ratio = front ? R(1) / R(mat.ior) : R(mat.ior);
+                R cos_theta = std::fabs(cosv);
+                V3<R> refracted = refract(world_dir, outward, ratio);
+                R reflect_prob = (refracted.x != R(0) || refracted.y != R(0) || refracted.z != R(0))
+                                     ? schlick(cos_theta, ratio) : R(1);
+                ray.hit_value = color;
+                ray.need_scatter = true;
+                if (R(lcg_float(ray.random_seed)) < reflect_prob) ray.scatter_direction = reflect(world_dir, normal);
+                else ray.scatter_direction = refracted;
+            } else if (length(emit) < R(0.01) && roughness == R(1)) {
+                bool scattered = dot(world_dir, geo_normal) < R(0);
+                lcg_unit_sphere(ray.random_seed, u3);
+                ray.need_scatter = scattered;
+                ray.scatter_direction = normalize(outward + V3<R>(R(u3[0]), R(u3[1]), R(u3[2])));
+                ray.hit_value = scattered ? color : V3<R>(R(0));
+            } else if (metallic > R(0)) {
+                V3<R> reflected = reflect(world_dir, outward);
+                bool scattered = dot(reflected, geo_normal) > R(0);
+                ray.need_scatter = scattered;
+                ray.hit_value = scattered ? color : V3<R>(R(0));
+                lcg_unit_sphere(ray.random_seed, u3);
+                ray.scatter_direction = reflected + V3<R>(R(u3[0]), R(u3[1]), R(u3[2])) * R(0.08);
+            }
+        }
     }
 
     // RayTracing.rgen main :25-167, for one pixel; returns the summed radiance.

@@ -254,6 +254,7 @@ int make_frame(rt_ctx* ctx, const TlasEntry& tl, const rt_bindings* b, float* ac
     fa->bounces = u.number_of_bounces;
     fa->antialiasing = u.antialiasing;
     fa->mapping = u.mapping;
+    fa->debug = u.debug == 1u ? 1u : 0u;
     fa->frame_count = u.frame_count;
     fa->fully_opaque = u.fully_opaque;
     fa->o.out = nullptr;
@@ -805,6 +806,8 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     fa.o.out = reinterpret_cast<uint32_t*>(out_rgba8);
     fa.o.bgra = ctx->out_bgra ? 1u : 0u;
     const bool any_hit = !fa.fully_opaque;
+    // the counting kernels also carry the debug == 1 materials
+    const bool counting = stats != nullptr || fa.debug != 0u;
     hipStream_t st = ctx->stream;
     if (stats) {
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
@@ -813,7 +816,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     StageTimer* tm = ctx->stage_timing ? &ctx->timer : nullptr;
     ctx->timer.used = 0;
     if (mode == RT_MODE_MEGAKERNEL) {
-        rc = launch_megakernel(fa, st, any_hit, stats != nullptr, tm);
+        rc = launch_megakernel(fa, st, any_hit, counting, tm);
     } else {
         if (!ctx->wavefront) {
             ctx->wavefront = wavefront_create();
@@ -824,7 +827,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         wavefront_set_refill(ctx->wavefront, ctx->refill_ext, ctx->refill_sh);
         wavefront_set_streams(ctx->wavefront, ctx->streams);
         wavefront_set_finish(ctx->wavefront, ctx->finish);
-        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, stats != nullptr, ctx->variant, tm, &err);
+        rc = launch_wavefront(ctx->wavefront, fa, st, any_hit, counting, ctx->variant, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
@@ -867,6 +870,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
     std::vector<FrameArgs> frames(count);
     size_t toff = 0;
     bool any_hit = false;
+    bool counting = stats != nullptr;  // the counting kernels also carry the debug == 1 materials
     for (uint32_t i = 0; i < count; i++) {
         const rt_batch_item& it = items[i];
         make_frame(ctx, ctx->tlas.at(it.tlas), it.bindings, it.accum, it.width, it.height, &frames[i]);
@@ -877,6 +881,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
         frames[i].sv.textures = ctx->d_batch_tex + toff;
         toff += n;
         any_hit = any_hit || !frames[i].fully_opaque;
+        counting = counting || frames[i].debug != 0u;
     }
     if (stats) {
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
@@ -887,7 +892,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
     int rc = 0;
     if (mode == RT_MODE_MEGAKERNEL) {
         for (uint32_t i = 0; i < count && rc == 0; i++)
-            rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, stats != nullptr, tm);
+            rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, counting, tm);
         if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
     } else {
         if (!ctx->wavefront) {
@@ -897,15 +902,14 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
         // frames without samples or bounces take the megakernel (it applies the reset)
         for (uint32_t i = 0; i < count && rc == 0; i++)
             if (frames[i].samples == 0u || frames[i].bounces == 0u)
-                rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, stats != nullptr);
+                rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, counting);
         if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
         std::string err;
         wavefront_set_graph(ctx->wavefront, ctx->use_graph);
         wavefront_set_refill(ctx->wavefront, ctx->refill_ext, ctx->refill_sh);
         wavefront_set_streams(ctx->wavefront, ctx->streams);
         wavefront_set_finish(ctx->wavefront, ctx->finish);
-        rc = launch_wavefront_batch(ctx->wavefront, frames.data(), count, st, any_hit, stats != nullptr, ctx->variant, tm,
-                                    &err);
+        rc = launch_wavefront_batch(ctx->wavefront, frames.data(), count, st, any_hit, counting, ctx->variant, tm, &err);
         if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "wavefront: " + err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
@@ -926,6 +930,8 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
     if (b->ubo->number_of_bounces > kMaxAdjointBounces)
         return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint replays at most 16 bounces");
+    if (b->ubo->debug == 1u)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint differentiates the PBR materials, not debug == 1");
     const uint32_t nt = b->num_textures;
     if (nt && !grad_textures) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "grad_textures is NULL");
     for (uint32_t i = 0; i < nt; i++)

@@ -70,7 +70,9 @@ __device__ inline void camera_ray(const FrameArgs& fa, uint32_t x, uint32_t y, R
     double uvx = (px / double(fa.width)) * 2.0 - 1.0;
     double uvy = (py / double(fa.height)) * 2.0 - 1.0;
     double offx = 0.0, offy = 0.0;
-    if (fa.aperture != 0.f) {  // RandomInUnitDisk (lib/Random.glsl:44-54); with aperture 0 the offset is 0
+    // RandomInUnitDisk (lib/Random.glsl:44-54).  With aperture 0 the offset is 0 and the
+    // draws are skipped, unless debug == 1 shading reads the LCG after them.
+    if (fa.aperture != 0.f || fa.debug) {
         float dx, dy;
         for (;;) {
             float a = lcg_float(lcg), b = lcg_float(lcg);
@@ -138,6 +140,7 @@ struct Payload {
     bool need_scatter;
     Rng4 rng;
     double volume_dis;
+    uint32_t lcg;  // Ray.RandomSeed (the LCG of RandomInUnitDisk / RandomInUnitSphere)
 };
 
 // RayTracing.rmiss:14-45
@@ -195,6 +198,10 @@ struct ShadeState {
     uint32_t lobe;    // kDiffuse / kSpecular / kTransmission chosen by shade_end
     double p_lobe;    // its probability (1 for the metallic-1 roughness-0 mirror)
     bool delta;       // the metallic-1 roughness-0 mirror (no lobe draw)
+    // inputs of the debug == 1 scatter (rchit:437-475) beyond the material
+    bool normal_flipped;  // outward = -normal
+    bool emit_small;      // length(emittance) < 0.01 (before exposure)
+    float ior_raw;        // MaterialRaw.ior
 };
 
 __device__ inline D4 mix4(D4 a, D4 b, D4 c, double b0, double bu, double bv) {
@@ -364,6 +371,9 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
     s.outward = outward;
     s.V = V;
     s.front = front;
+    s.normal_flipped = ddot(geo_normal, normal) < 0.0;
+    s.emit_small = dlength(emit) < 0.01;
+    s.ior_raw = mat.ior;
     // sampleLightRIS, rchit:77-122 (uniform candidate index, Yuksel-attenuated weights)
     Rng4 rng = ray.rng;
     s.has_light = false;
@@ -454,6 +464,38 @@ __device__ inline void shade_end(Payload& ray, ShadeState& s) {
     ray.rng = rng;
 }
 
+// ubo.debug == 1, rchit:437-475: the legacy materials replace the scatter (hitValue =
+// colour when the ray scatters), drawing from the LCG; `dir` is the unnormalised
+// gl_WorldRayDirectionEXT.
+__device__ inline void debug_scatter(Payload& ray, const ShadeState& s, D3 dir) {
+    const MatBrdf& m = s.m;
+    const D3 normal = s.normal_flipped ? -s.outward : s.outward;
+    bool scatter;
+    D3 next;
+    if (m.transmission > 0.0) {  // dielectric: Schlick reflection or refraction
+        const double ior = s.ior_raw;
+        const double ratio = s.front ? 1.0 / ior : ior;
+        const double cos_theta = fabs(ddot(s.V, s.geo_normal));
+        const D3 refracted = refract3(dir, s.outward, ratio);
+        const bool tir = refracted.x == 0.0 && refracted.y == 0.0 && refracted.z == 0.0;
+        const double reflect_prob = tir ? 1.0 : schlick(cos_theta, ratio);
+        scatter = true;
+        next = double(lcg_float(ray.lcg)) < reflect_prob ? reflect3(dir, normal) : refracted;
+    } else if (s.emit_small && m.roughness == 1.0) {  // Lambertian
+        scatter = ddot(dir, s.geo_normal) < 0.0;
+        next = dnormalize(s.outward + lcg_unit_sphere(ray.lcg));
+    } else if (m.metallic > 0.0) {  // fuzzy metal
+        const D3 reflected = reflect3(dir, s.outward);
+        scatter = ddot(reflected, s.geo_normal) > 0.0;
+        next = reflected + lcg_unit_sphere(ray.lcg) * 0.08;
+    } else {
+        return;
+    }
+    ray.need_scatter = scatter;
+    ray.hit_value = scatter ? m.base_color : d3(0.0);
+    ray.scatter_dir = next;
+}
+
 // Integrator state of one pixel between bounces (registers in the megakernel,
 // structure-of-arrays in HBM in the wavefront form).  Streams A and B keep only
 // their counters: their other words are (x, y, frame_count) and (x, y, seed_b).
@@ -518,6 +560,7 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
     ray.hit_point = p.hit_point;
     ray.t = p.bounce == 0u ? 0.0 : 1.0;  // only "Ray.t != 0" is read before it is set (rmiss:18)
     ray.rng = Rng4{x, y, fa.seed_b, p.wb};
+    ray.lcg = p.lcg;
     ray.volume_dis = p.volume_dis;
     ray.emittance = d3(0.0);
     ray.need_scatter = false;
@@ -539,6 +582,9 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
                 nee = eval_combined_brdf(ss.outward, ss.light_dir, ss.V, ss.m) * ss.light_scale;
             }
             shade_end(ray, ss);
+            // the counting kernels carry the debug == 1 materials (the host runs them
+            // for such frames), so the timed kernels keep their registers
+            if (kStats && fa.debug) debug_scatter(ray, ss, p.d);
             tap.vertex(ss, ray);
         }
     } else {
@@ -553,6 +599,7 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
     p.hit_point = ray.hit_point;
     p.volume_dis = ray.volume_dis;
     p.wb = ray.rng.w;
+    p.lcg = ray.lcg;
     p.last_t = float(ray.t);
     r.cont = false;
     if (p.bounce + 1u >= fa.bounces) return;

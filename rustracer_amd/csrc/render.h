@@ -22,6 +22,7 @@ struct FrameArgs {
     float mvi[16], pin[16];        // ModelViewInverse, ProjectionInverse (column-major)
     float aperture, focus_distance, ortho, exposure;
     uint32_t samples, bounces, antialiasing, mapping, frame_count, fully_opaque;
+    uint32_t debug;        // ubo.debug == 1: the legacy materials of RayTracing.rchit:437-475
     OutputArgs o;                  // display image (o.out NULL: none)
 };
 

@@ -417,6 +417,28 @@ __device__ inline D3 refract3(D3 I, D3 N, double eta) {  // GLSL refract
     if (k < 0.0) return d3(0.0);
     return I * eta - N * (eta * c + sqrt(k));
 }
+__device__ inline D3 reflect3(D3 I, D3 N) { return I - N * (2.0 * ddot(N, I)); }  // GLSL reflect
+
+// Schlick's reflectance, lib/PBR.glsl:48-53.
+__device__ inline double schlick(double cosine, double ri) {
+    double r0 = (1.0 - ri) / (1.0 + ri);
+    r0 *= r0;
+    return r0 + (1.0 - r0) * pow(1.0 - cosine, 5.0);
+}
+
+// RandomInUnitSphere, lib/Random.glsl:56-67: LCG triples 2u - 1 until one lies inside
+// the unit ball.  The components are exact in fp32 and their squares sum exactly in
+// fp64, so the acceptance test is exact.  The loop is cut at 256 tries (a rejection
+// rate of 1 - pi/6 makes that unreachable) so that every lane ends.
+__device__ inline D3 lcg_unit_sphere(uint32_t& seed) {
+    for (int k = 0; k < 256; k++) {
+        const double x = 2.0 * double(lcg_float(seed)) - 1.0;
+        const double y = 2.0 * double(lcg_float(seed)) - 1.0;
+        const double z = 2.0 * double(lcg_float(seed)) - 1.0;
+        if (x * x + y * y + z * z < 1.0) return d3(x, y, z);
+    }
+    return d3(0.0);
+}
 
 // evalIndirectCombinedBRDF, PBR.glsl:649-729 (+ sampleSpecularMicrofacet :587-616)
 __device__ inline bool sample_indirect(double ux, double uy, D3 sn, D3 gn, D3 V, const MatBrdf& m, uint32_t type,
