@@ -117,8 +117,8 @@ int rt_skin_vertices(rt_ctx* ctx, const rt_vertex* d_in, rt_vertex* d_out, uint3
  * faster and gives the same hits (a looser tree if the mesh deforms far).
  * inputs[i] must have the build's primitive_count and describe the same triangles
  * (vertex_data may be another buffer, e.g. the skinned one).  TLASes built over
- * these BLASes must be rebuilt afterwards (rt_build_tlas), as the reference rebuilds
- * its TLAS.  Synchronises the context stream. */
+ * these BLASes must be refit (rt_refit_tlas) or rebuilt (rt_build_tlas, as the
+ * reference rebuilds its TLAS) afterwards.  Synchronises the context stream. */
 int rt_refit_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count, const rt_blas* blases);
 
 /* --------------------------------------------------- top-level (instances) */
@@ -128,6 +128,16 @@ int rt_refit_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
  * acceleration_structures.rs:136-249).  Every acceleration_structure_reference
  * must come from rt_blas_device_address() of a live BLAS. */
 int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, rt_tlas* out);
+/* Refits a TLAS to the present geometry of the BLASes it references, after
+ * rt_refit_blas_batch moved their vertices (same instances, transforms and
+ * topology): the flattened BVH's triangle records take their positions again from
+ * their BLAS records, and the node boxes of the flattened BVH and of the TLAS over
+ * the other instances are recomputed bottom-up, padded and quantised as the build
+ * does.  The trees are kept, so a far deformation gives looser trees but the same
+ * hits.  Replaces the create_top_as of the reference's animation step
+ * (crates/examples/gltf_viewer/src/main.rs:377-404) when only the vertices moved;
+ * changed transforms need rt_build_tlas.  Synchronises the stream. */
+int rt_refit_tlas(rt_ctx* ctx, rt_tlas tlas);
 void rt_destroy_tlas(rt_ctx* ctx, rt_tlas tlas);
 
 /* ----------------------------------------------------------- trace (frame) */

@@ -60,6 +60,7 @@ struct TlasEntry {
     DevPtr flat_nodes, flat_tris;  // flattened BVH over the identity instances
     uint32_t flat_num_nodes = 0, flat_num_tris = 0;
     float flat_bounds[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<uint32_t> level_start, flat_level_start;  // node ranges per tree level (refit)
     std::vector<std::shared_ptr<BlasPool>> pools;
     bool any_non_opaque = false;
 };
@@ -642,6 +643,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
     te.nodes.reset(res.nodes);
     te.num_nodes = res.num_nodes;
+    te.level_start = res.level_start;
     te.inst.reset(dev_alloc<InstanceRecord>(count + 1, &e));
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
     te.num_inst = count;
@@ -692,6 +694,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
         DevPtr fperm(fres.perm), fseg_bounds(fres.seg_bounds);
         te.flat_nodes.reset(fres.nodes);
         te.flat_num_nodes = fres.num_nodes;
+        te.flat_level_start = fres.level_start;
         te.flat_tris.reset(dev_alloc<float4>(3 * size_t(n) + 3, &e));
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
         te.flat_num_tris = n;
@@ -705,6 +708,28 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
     uint64_t hnd = ctx->next_handle++;
     ctx->tlas.emplace(hnd, std::move(te));
     *out = hnd;
+    return RT_OK;
+}
+
+int rt_refit_tlas(rt_ctx* ctx, rt_tlas tlas) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    TlasEntry& te = it->second;
+    DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
+    std::string err;
+    if (te.flat_num_tris) {
+        int rc = lbvh_refit_flat(st, (const InstanceRecord*)te.inst.get(), te.flat_num_tris, (float4*)te.flat_tris.get(),
+                                 (QNode*)te.flat_nodes.get(), te.flat_num_nodes, te.flat_level_start, te.flat_bounds,
+                                 &err);
+        if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "flattened BVH refit: " + err);
+    }
+    if (te.num_tlas) {
+        int rc = lbvh_refit_instances(st, (const InstanceRecord*)te.inst.get(), te.num_tlas, (QNode*)te.nodes.get(),
+                                      te.num_nodes, te.level_start, te.bounds, &err);
+        if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "TLAS refit: " + err);
+    }
     return RT_OK;
 }
 

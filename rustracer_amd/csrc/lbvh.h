@@ -81,6 +81,19 @@ int lbvh_refit(hipStream_t stream, const TriSegment* d_segs, uint32_t nseg, uint
                uint32_t num_nodes, const std::vector<uint32_t>& level_start, BlasDesc* d_descs, uint32_t* d_bad,
                std::string* err);
 
+// Refit of a flattened BVH (one segment, root node 0) after the BLASes its records
+// come from were refit: every record takes its positions again from its BLAS record,
+// then the node boxes are recomputed bottom-up as in lbvh_refit.  `bounds` gets the
+// padded root box.
+int lbvh_refit_flat(hipStream_t stream, const InstanceRecord* d_inst, uint32_t n, float4* d_tri, QNode* d_nodes,
+                    uint32_t num_nodes, const std::vector<uint32_t>& level_start, float bounds[6], std::string* err);
+
+// Refit of the TLAS over n instance records in leaf order (their BLAS bounds and
+// transforms as they are now), node boxes recomputed bottom-up.
+int lbvh_refit_instances(hipStream_t stream, const InstanceRecord* d_inst, uint32_t n, QNode* d_nodes,
+                         uint32_t num_nodes, const std::vector<uint32_t>& level_start, float bounds[6],
+                         std::string* err);
+
 // Instance boxes (world space) for the TLAS build.
 int prepare_instances(hipStream_t stream, const InstanceRecord* d_inst, uint32_t n, float4* d_lo,
                       float4* d_hi);

@@ -725,8 +725,9 @@ __global__ void k_refit_seg(const QNode* nodes, uint32_t begin, uint32_t end, ui
 // refit boxes of the deeper level (internal children), padded by the segment's
 // padding as the build pads them, quantised; the node's own (unpadded) box is kept
 // for its parent.
-__global__ void k_refit_level(QNode* nodes, uint32_t begin, uint32_t end, const float4* tri, const uint32_t* node_seg,
-                              const uint32_t* seg_mag, float* node_box) {
+template <bool kBoxLeaves>
+__global__ void k_refit_level(QNode* nodes, uint32_t begin, uint32_t end, const float4* tri, const float4* leaf_lo,
+                              const float4* leaf_hi, const uint32_t* node_seg, const uint32_t* seg_mag, float* node_box) {
     uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= end) return;
     const QNode q = nodes[i];
@@ -746,6 +747,11 @@ __global__ void k_refit_level(QNode* nodes, uint32_t begin, uint32_t end, const 
         float3 l = mk3(INFINITY, INFINITY, INFINITY), h = mk3(-INFINITY, -INFINITY, -INFINITY);
         if (cnt) {
             for (uint32_t r = c; r < c + cnt; r++) {
+                if (kBoxLeaves) {
+                    l = fmin3(l, xyz(leaf_lo[r]));
+                    h = fmax3(h, xyz(leaf_hi[r]));
+                    continue;
+                }
                 const float4* t = tri + 3 * size_t(r);
                 for (int v = 0; v < 3; v++) {
                     const float3 p = xyz(t[v]);
@@ -767,6 +773,37 @@ __global__ void k_refit_level(QNode* nodes, uint32_t begin, uint32_t end, const 
     b[0] = nl.x; b[1] = nl.y; b[2] = nl.z;
     b[3] = nh.x; b[4] = nh.y; b[5] = nh.z;
     nodes[i] = quantize_node(w);
+}
+
+// Flattened-BVH refit: record p takes its positions again from its BLAS record (v1.w
+// names the instance, v2.w the record in that instance's BLAS pool); the w words are
+// kept.  Raises the largest coordinate magnitude (one segment).
+__global__ void k_refit_flat_records(const InstanceRecord* inst, uint32_t n, float4* tri, uint32_t* seg_mag) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4* t = tri + 3 * size_t(p);
+    const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+    const uint32_t ii = __float_as_uint(t1.w) & ~kFlatOpaque, rec = __float_as_uint(t2.w);
+    const float4* src = inst[ii].tris + 3 * size_t(rec);
+    const float4 a = src[0], b = src[1], c = src[2];
+    t[0] = make_float4(a.x, a.y, a.z, t0.w);
+    t[1] = make_float4(b.x, b.y, b.z, t1.w);
+    t[2] = make_float4(c.x, c.y, c.z, t2.w);
+    const float3 m = fmax3(fmax3(mk3(fabsf(a.x), fabsf(a.y), fabsf(a.z)), mk3(fabsf(b.x), fabsf(b.y), fabsf(b.z))),
+                           mk3(fabsf(c.x), fabsf(c.y), fabsf(c.z)));
+    const float mag = fmaxf(m.x, fmaxf(m.y, m.z));
+    if (isfinite(mag)) atomicMax(seg_mag, __float_as_uint(mag));
+}
+
+// Largest finite coordinate magnitude of a set of leaf boxes (one segment).
+__global__ void k_box_mag(const float4* lo, const float4* hi, uint32_t n, uint32_t* seg_mag) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v[6] = {lo[i].x, lo[i].y, lo[i].z, hi[i].x, hi[i].y, hi[i].z};
+    float m = 0.f;
+    for (int k = 0; k < 6; k++)
+        if (isfinite(v[k])) m = fmaxf(m, fabsf(v[k]));
+    atomicMax(seg_mag, __float_as_uint(m));
 }
 
 __global__ void k_iota(uint32_t* out, uint32_t n) {
@@ -1043,14 +1080,98 @@ int lbvh_refit(hipStream_t st, const TriSegment* d_segs, uint32_t nseg, uint32_t
     for (size_t L = levels; L-- > 0;) {
         const uint32_t b = level_start[L], c = level_start[L + 1];
         if (c > b)
-            hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(c - b, kBlock)), dim3(kBlock), 0, st, d_nodes, b, c,
-                               (const float4*)d_tri, node_seg, seg_mag, node_box);
+            hipLaunchKernelGGL(k_refit_level<false>, dim3(blocks_for(c - b, kBlock)), dim3(kBlock), 0, st, d_nodes, b, c,
+                               (const float4*)d_tri, (const float4*)nullptr, (const float4*)nullptr, node_seg, seg_mag,
+                               node_box);
     }
     hipLaunchKernelGGL(k_refit_bounds, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, d_descs, nseg, node_box, seg_mag);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);  // the scratch is freed on return
     if (e != hipSuccess) *err = "refit launch";
     return int(e);
+}
+
+namespace {
+// Bottom-up refit of a one-segment tree (root node 0) whose leaf entries index
+// triangle records (tri) or boxes (leaf_lo / leaf_hi); `prepare` has raised
+// seg_mag[0] beforehand on the same stream.  Writes the padded root box to bounds.
+int refit_tree(hipStream_t st, Scratch& s, QNode* d_nodes, uint32_t num_nodes, const std::vector<uint32_t>& level_start,
+               const float4* tri, const float4* leaf_lo, const float4* leaf_hi, uint32_t* seg_mag, float bounds[6],
+               std::string* err) {
+    if (level_start.size() < 2 || level_start.front() != 0u || level_start[1] != 1u || level_start.back() != num_nodes) {
+        *err = "the tree's level table does not match its nodes";
+        return int(hipErrorInvalidValue);
+    }
+    uint32_t* node_seg = s.alloc<uint32_t>(num_nodes);
+    float* node_box = s.alloc<float>(6 * size_t(num_nodes));
+    BlasDesc* desc = s.alloc<BlasDesc>(1);
+    if (!node_seg || !node_box || !desc) {
+        *err = "out of device memory";
+        return int(hipErrorOutOfMemory);
+    }
+    hipError_t e = hipMemsetAsync(node_seg, 0, sizeof(uint32_t) * num_nodes, st);
+    const size_t levels = level_start.size() - 1;
+    for (size_t L = levels; e == hipSuccess && L-- > 0;) {
+        const uint32_t b = level_start[L], c = level_start[L + 1];
+        if (c <= b) continue;
+        if (leaf_lo)
+            hipLaunchKernelGGL(k_refit_level<true>, dim3(blocks_for(c - b, kBlock)), dim3(kBlock), 0, st, d_nodes, b, c,
+                               (const float4*)nullptr, leaf_lo, leaf_hi, node_seg, seg_mag, node_box);
+        else
+            hipLaunchKernelGGL(k_refit_level<false>, dim3(blocks_for(c - b, kBlock)), dim3(kBlock), 0, st, d_nodes, b,
+                               c, tri, (const float4*)nullptr, (const float4*)nullptr, node_seg, seg_mag, node_box);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_refit_bounds, dim3(1), dim3(kBlock), 0, st, desc, 1u, node_box, seg_mag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(bounds, desc->bounds, sizeof(float) * 6, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // the scratch is freed on return
+    if (e != hipSuccess) *err = "refit launch";
+    return int(e);
+}
+}  // namespace
+
+int lbvh_refit_flat(hipStream_t st, const InstanceRecord* d_inst, uint32_t n, float4* d_tri, QNode* d_nodes,
+                    uint32_t num_nodes, const std::vector<uint32_t>& level_start, float bounds[6], std::string* err) {
+    Scratch s;
+    uint32_t* seg_mag = s.alloc<uint32_t>(1);
+    if (!seg_mag) {
+        *err = "out of device memory";
+        return int(hipErrorOutOfMemory);
+    }
+    hipError_t e = hipMemsetAsync(seg_mag, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) {
+        *err = "memset";
+        return int(e);
+    }
+    if (n) hipLaunchKernelGGL(k_refit_flat_records, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_inst, n, d_tri,
+                              seg_mag);
+    return refit_tree(st, s, d_nodes, num_nodes, level_start, d_tri, nullptr, nullptr, seg_mag, bounds, err);
+}
+
+int lbvh_refit_instances(hipStream_t st, const InstanceRecord* d_inst, uint32_t n, QNode* d_nodes, uint32_t num_nodes,
+                         const std::vector<uint32_t>& level_start, float bounds[6], std::string* err) {
+    Scratch s;
+    uint32_t* seg_mag = s.alloc<uint32_t>(1);
+    float4* lo = s.alloc<float4>(n + 1);
+    float4* hi = s.alloc<float4>(n + 1);
+    if (!seg_mag || !lo || !hi) {
+        *err = "out of device memory";
+        return int(hipErrorOutOfMemory);
+    }
+    hipError_t e = hipMemsetAsync(seg_mag, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) {
+        *err = "memset";
+        return int(e);
+    }
+    int rc = prepare_instances(st, d_inst, n, lo, hi);
+    if (rc) {
+        *err = "instance boxes";
+        return rc;
+    }
+    if (n) hipLaunchKernelGGL(k_box_mag, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, lo, hi, n, seg_mag);
+    return refit_tree(st, s, d_nodes, num_nodes, level_start, nullptr, lo, hi, seg_mag, bounds, err);
 }
 
 }  // namespace rt

@@ -38,7 +38,7 @@ EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", 
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
            "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint",
-           "rt_trace_rays_batch", "rt_autotune", "rt_skin_vertices", "rt_refit_blas_batch"]
+           "rt_trace_rays_batch", "rt_autotune", "rt_skin_vertices", "rt_refit_blas_batch", "rt_refit_tlas"]
 
 
 class RtError(RuntimeError):
@@ -132,6 +132,7 @@ def load_library(path=LIB_PATH):
         "rt_trace_rays": (i, [vp, u64, vp, vp, vp, u, u, u, vp]),
         "rt_skin_vertices": (i, [vp, vp, vp, u, vp, u]),
         "rt_refit_blas_batch": (i, [vp, vp, u, C.POINTER(u64)]),
+        "rt_refit_tlas": (i, [vp, u64]),
         "rt_trace_ray_queries": (i, [vp, u64, vp, vp, u, u, vp]),
         "rt_trace_primary": (i, [vp, u64, vp, u, u, vp]),
         "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
@@ -344,13 +345,16 @@ def create_as(ctx, scene, buffers):
     return blases, create_top_as(ctx, scene, blases)
 
 
-def refit_as(ctx, scene, buffers, blases, vertices=None):
+def refit_as(ctx, scene, buffers, blases, vertices=None, tlas=None):
     """rt_refit_blas_batch over the BLASes of create_as with the scene's (or the given)
-    vertex data, then create_top_as again (the reference rebuilds its TLAS too).
-    Returns the new TopAS."""
+    vertex data, then rt_refit_tlas on `tlas` when one is given, else create_top_as
+    again (the reference rebuilds its TLAS).  Returns the TopAS."""
     ins = blas_inputs(scene, buffers, vertices)
     handles = (C.c_uint64 * len(blases))(*[b.handle for b in blases])
     ctx.check(ctx.lib.rt_refit_blas_batch(ctx.handle, ins, len(ins), handles))
+    if tlas is not None:
+        ctx.check(ctx.lib.rt_refit_tlas(ctx.handle, tlas.handle))
+        return tlas
     return create_top_as(ctx, scene, blases)
 
 

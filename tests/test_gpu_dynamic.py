@@ -3,7 +3,8 @@
 reference rebuilds its acceleration structures from the skinned vertex buffer
 every animation step, main.rs:377-404): the refit BVH is valid in float, and the
 image traced through it is the image traced through a fresh build over the same
-skinned vertices, bit for bit, and matches the FP64 oracle."""
+skinned vertices, bit for bit, and matches the FP64 oracle; rt_refit_tlas likewise
+for the flattened BVH and the TLAS over the instances."""
 import numpy as np
 import pytest
 import torch
@@ -106,6 +107,37 @@ def test_refit_after_skinning_equals_a_fresh_build(gpu_ctx, angle):
     o, _ = B.OracleScene(sc2).render()
     rel = G.rel_rmse(a.cpu().numpy()[..., :3].astype(np.float64), o[..., :3])
     assert rel <= 1e-3, rel
+
+
+@pytest.mark.parametrize("flatten", [True, False])
+def test_tlas_refit_equals_a_fresh_build(gpu_ctx, flatten):
+    """rt_refit_tlas after the BLAS refit: the flattened BVH (identity instances) or
+    the TLAS over the instances (flattening off) keeps its tree and takes the skinned
+    geometry; its image equals the image of a fresh BLAS + TLAS build."""
+    gpu_ctx.set_option(rt.OPTION_FLATTEN_IDENTITY_INSTANCES, int(flatten))
+    try:
+        sc = S.skinned_cornell()
+        buf = rt.Buffers(sc)
+        blases, tlas = rt.create_as(gpu_ctx, sc, buf)
+        assert (tlas.flat_info().num_prims > 0) == flatten
+        skinned = torch.empty_like(buf.vertices)
+        rt.skin_vertices(gpu_ctx, buf.vertices, skinned, torch.from_numpy(S.bend_skins(1.2)).cuda())
+        assert rt.refit_as(gpu_ctx, sc, buf, blases, vertices=skinned, tlas=tlas) is tlas
+        gpu_ctx.sync()
+        assert not bvhcheck.check_tlas(gpu_ctx, tlas, len(sc.instances))
+        sc2 = S.skinned_cornell()
+        sc2.vertices = _host_vertices(skinned).copy()
+        ds2 = G.DeviceScene(gpu_ctx, sc2, flatten=flatten)
+        buf.vertices = skinned
+        for mode in (rt.MODE_MEGAKERNEL, rt.MODE_WAVEFRONT):
+            a = rt.new_accumulation(sc.width, sc.height)
+            b = rt.new_accumulation(sc.width, sc.height)
+            rt.trace_rays(gpu_ctx, tlas, buf, sc.ubo, a, sc.width, sc.height, mode=mode)
+            rt.trace_rays(gpu_ctx, ds2.tlas, ds2.buffers, sc2.ubo, b, sc.width, sc.height, mode=mode)
+            gpu_ctx.sync()
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    finally:
+        gpu_ctx.set_option(rt.OPTION_FLATTEN_IDENTITY_INSTANCES, 1)
 
 
 def test_refit_argument_errors(gpu_ctx):
