@@ -52,14 +52,17 @@ def trace_bytes(st):
             (st["primary_rays"] + st["extension_rays"]) * EXTEND_RAY_BYTES + st["shadow_rays"] * SHADOW_RAY_BYTES)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, variant):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration), or None when
+    there is none for this traversal variant."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_pmc.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
+        if d.get("traversal_variant") != variant:
+            return None
         return (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024.0
     except (KeyError, ValueError):
         return None
@@ -100,10 +103,12 @@ def adjoint_iteration(ctx, steps, warmup):
     """One optimisation iteration of the parameter-texture workload: the Cornell box
     at 512x512, 16 spp, depth 4 with 512x512 RGBA32F base-colour and
     metallic-roughness parameter textures: forward trace, L2 loss gradient, adjoint
-    (path replay).  Returns milliseconds per iteration and its split."""
+    (path replay).  The forward runs with settings autotuned on this scene, in a
+    context of its own.  Returns milliseconds per iteration and its split."""
     import numpy as np
     import torch
     from rustracer_amd import diff, rt, scenes
+    ctx = rt.Context(ctx.device)
     sc = scenes.cornell_params()
     tsc = scenes.cornell_params(target=True)
     tbuf = rt.Buffers(tsc)
@@ -114,6 +119,8 @@ def adjoint_iteration(ctx, steps, warmup):
     W, H = sc.width, sc.height
     grads = {k: torch.zeros_like(p) for k, p in zip(ds.slots, ds.params)}
     acc = rt.new_accumulation(W, H)
+    tuned = ctx.autotune(ds.tlas, ds.buffers, sc.ubo, W, H)
+    tuned["frame_ms"] = round(tuned["frame_ms"], 2)
 
     def once():
         rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=rt.MODE_WAVEFRONT)
@@ -135,7 +142,7 @@ def adjoint_iteration(ctx, steps, warmup):
         times.append(e0.elapsed_time(e1))
         fwd.append(e0.elapsed_time(ev))
     return {"ms_per_iter": round(float(np.median(times)), 3), "forward_ms": round(float(np.median(fwd)), 3),
-            "adjoint_ms": round(float(np.median(times)) - float(np.median(fwd)), 3),
+            "adjoint_ms": round(float(np.median(times)) - float(np.median(fwd)), 3), "autotune": tuned,
             "workload": "Cornell box 512x512, 16 spp, depth 4, 512^2 RGBA32F base-colour + metallic-roughness "
                         "parameter textures; forward + L2 gradient + path-replay adjoint"}
 
@@ -371,7 +378,7 @@ def main():
         bytes_per_launch = algorithmic_bytes(st, W * H)
         launch_ms = stage["extend_ms"]
     achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
-    traffic = measured_traffic(kernel_name + "_displaced_sphere")
+    traffic = measured_traffic(kernel_name + "_displaced_sphere", (tuned or {}).get("traversal_variant"))
     line = {
         "metric": "Mrays/s at 1080p/64spp (wavefront vs megakernel) at RMSE<=1e-3; adjoint ms/iter",
         "value": round(r["mrays"], 2),
