@@ -312,7 +312,8 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * count that hands the tail to one finishing launch, the refill threshold,
  * graph replay, and the traversal variant once more under those settings,
  * keeping the fastest settings on this context (the RT_OPTION_* values).  Each
- * probe times the faster of two frames after a warm-up frame.
+ * probe times the faster of two frames after a warm-up frame, and a setting
+ * replaces the one it is compared with only when it is more than 1.5 % faster.
  * A variant whose stack ring gives up on a ray of this frame (restart budget) is
  * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */
 typedef struct rt_tuning {

@@ -25,6 +25,7 @@ using namespace rt;
 namespace {
 
 constexpr uint32_t kAutotuneSamples = 8;  // samples per pixel of the autotuner's probe frames
+constexpr double kAutotuneMargin = 0.015;  // relative gain a probe needs to replace the incumbent
 constexpr uint32_t kBlasLeafMax = 2;  // default triangles per leaf (RT_OPTION_BLAS_LEAF_SIZE)
 constexpr uint32_t kTlasLeafMax = 1;
 
@@ -1053,8 +1054,10 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         ms_out = flags ? -1.0 : double(best_ms);
         return RT_OK;
     };
+    // a setting replaces the incumbent only when it is faster by more than the
+    // frame-to-frame noise, so repeated tunings of one scene choose alike
     double best = -1.0;
-    auto consider = [&](double ms) { return ms >= 0.0 && (best < 0.0 || ms < best); };
+    auto consider = [&](double ms) { return ms >= 0.0 && (best < 0.0 || ms < best * (1.0 - kAutotuneMargin)); };
     // 1. traversal variant (graph off, current refills)
     ctx->use_graph = false;
     int best_v = -1;
