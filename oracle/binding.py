@@ -87,6 +87,7 @@ def lib():
             "orc_frostbite_diffuse": (d, [d, d, d, d]), "orc_light_intensity_luminance": (d, [vp, d]),
             "orc_offset_ray": (None, [vp, vp, vp]), "orc_texture_sample": (None, [vp, d, d, dp]),
             "orc_cube_sample": (None, [vp, vp, vp]),
+            "orc_sample_indirect_vd": (i, [dp, dp, dp, dp, vp, u, dp, dp, dp]),
             "orc_intersect_triangle": (i, [dp, dp, dp, dp, dp, d, d, dp]),
             "orc_scene_create": (vp, [vp]), "orc_scene_destroy": (None, [vp]),
             "orc_trace_rays": (i, [vp, vp, u, u, u, vp, vp, i, i]),
@@ -113,20 +114,31 @@ def vec(*x):
 
 
 def matbrdf(base_color=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.0, transmission=0.0,
-            specular_factor=1.0, specular_color=(1, 1, 1), front_face=True, volume=False, sg_specular=None):
+            specular_factor=1.0, specular_color=(1, 1, 1), front_face=True, volume=False, sg_specular=None,
+            attenuation_color=(1, 1, 1), attenuation_distance=3.4e38):
     m = OrcMatBrdf()
     m.base_color[:] = base_color
     m.metallic, m.roughness, m.ior, m.transmission = metallic, roughness, ior, transmission
     m.specular_factor = specular_factor
     m.specular_color_factor[:] = specular_color
     m.front_face, m.volume = int(front_face), int(volume)
-    m.attenuation_color[:] = (1, 1, 1)
-    m.attenuation_distance = 3.4e38
+    m.attenuation_color[:] = attenuation_color
+    m.attenuation_distance = attenuation_distance
     if sg_specular is not None:
         m.workflow_sg = 1
         m.sg_specular[:] = sg_specular
     lib().orc_mat_build(C.byref(m))
     return m
+
+
+def sample_indirect_vd(u, shading_n, geo_n, V, m, brdf_type, volume_dis):
+    """orc_sample_indirect_vd: (need_scatter, dir, weight, volume_dis after)."""
+    d, w, vd = np.zeros(3), np.zeros(3), np.array([volume_dis], np.float64)
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (u, shading_n, geo_n, V)]
+    r = lib().orc_sample_indirect_vd(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs], C.byref(m),
+                                     int(brdf_type), vd.ctypes.data_as(C.POINTER(C.c_double)),
+                                     d.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)))
+    return bool(r), d, w, float(vd[0])
 
 
 class OracleScene:

@@ -1456,14 +1456,18 @@ void orc_eval_combined_brdf(const double N[3], const double L[3], const double V
 void orc_brdf_probability(const orc_matbrdf* m, const double V[3], const double N[3], double out[3]) {
     put3(out, brdf_probability(to_mat(m), v3d(V), v3d(N)));
 }
-int orc_sample_indirect(const double u[2], const double sn[3], const double gn[3], const double V[3], const orc_matbrdf* m,
-                        uint32_t type, double dir[3], double weight[3]) {
+int orc_sample_indirect_vd(const double u[2], const double sn[3], const double gn[3], const double V[3],
+                           const orc_matbrdf* m, uint32_t type, double* volume_dis, double dir[3], double weight[3]) {
     V3<double> d(0.0), w(0.0);
-    double vd = -1;
-    bool r = sample_indirect(u[0], u[1], v3d(sn), v3d(gn), v3d(V), to_mat(m), type, d, w, vd);
+    bool r = sample_indirect(u[0], u[1], v3d(sn), v3d(gn), v3d(V), to_mat(m), type, d, w, *volume_dis);
     put3(dir, d);
     put3(weight, w);
     return r ? 1 : 0;
+}
+int orc_sample_indirect(const double u[2], const double sn[3], const double gn[3], const double V[3], const orc_matbrdf* m,
+                        uint32_t type, double dir[3], double weight[3]) {
+    double vd = -1;
+    return orc_sample_indirect_vd(u, sn, gn, V, m, type, &vd, dir, weight);
 }
 double orc_ggx_d(double a2, double ndh) { return ggx_d(a2, ndh); }
 double orc_smith_g2_height_correlated(double a, double ndl, double ndv) { return smith_g2_hc(a, ndl, ndv); }

@@ -83,6 +83,10 @@ void orc_brdf_probability(const orc_matbrdf* m, const double V[3], const double 
 int orc_sample_indirect(const double u[2], const double shading_n[3], const double geo_n[3],
                         const double V[3], const orc_matbrdf* m, uint32_t brdf_type,
                         double dir[3], double weight[3]);
+/* The same with Ray.volume_dis in and out (the transmission lobe's attenuation). */
+int orc_sample_indirect_vd(const double u[2], const double shading_n[3], const double geo_n[3],
+                           const double V[3], const orc_matbrdf* m, uint32_t brdf_type, double* volume_dis,
+                           double dir[3], double weight[3]);
 double orc_ggx_d(double alpha_squared, double n_dot_h);
 double orc_smith_g2_height_correlated(double alpha, double n_dot_l, double n_dot_v);
 double orc_smith_g1_ggx(double alpha, double n_dot_s, double alpha_sq, double n_dot_s_sq);

@@ -1,0 +1,39 @@
+"""A glass slab for the transmission / volume tests (oracle and GPU)."""
+import numpy as np
+
+from rustracer_amd import layouts as L
+from rustracer_amd import scenes as S
+
+BASE = (0.9, 0.6, 0.3)
+
+
+def slab(thickness=1.0, color=(0.8, 0.5, 0.2), dist=2.0, width=48, height=48, spp=16):
+    """A glass slab between y = 0 (normal +y) and y = -thickness (normal -y), 100 wide,
+    transmission 1, ior 1.5, volume attenuation `color` over `dist`, specular factor 0
+    (so no reflection lobe carries weight), seen from y = 5 straight down (no
+    anti-aliasing), depth 3, no lights but the 0.01 miss."""
+    b = S.Builder()
+    m = L.default_material()
+    m["base_color"] = (*BASE, 1.0)
+    m["metallic_factor"] = 0.0
+    m["roughness_factor"] = 0.3
+    m["ior"] = 1.5
+    m["transmission_exist"] = 1
+    m["transmission_factor"] = 1.0
+    m["volume_exists"] = 1
+    m["attenuation_color"] = color
+    m["attenuation_distance"] = dist
+    m["specular_factor"] = 0.0
+    mid = b.add_material(m)
+    geos = []
+    for y, up in ((0.0, True), (-thickness, False)):
+        s = 50.0
+        pos = np.array([[-s, y, -s], [s, y, -s], [s, y, s], [-s, y, s]], np.float64)
+        nrm = np.tile([0.0, 1.0 if up else -1.0, 0.0], (4, 1))
+        idx = np.array([0, 2, 1, 0, 3, 2] if up else [0, 1, 2, 0, 2, 3], np.uint32)
+        geos.append(b.add_primitive(S.make_vertices(pos, nrm), idx, mid))
+    inst = S.make_instances([(None, g) for g in geos])
+    view = S.look_at_rh((0, 5, 0), (0, 0, 0), (0, 0, -1))
+    proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
+    ubo = S.make_ubo(view, proj, samples=spp, bounces=3, antialiasing=False)
+    return b.finish("slab", inst, [], [L.default_dlight()], ubo, width, height)
