@@ -37,3 +37,29 @@ def slab(thickness=1.0, color=(0.8, 0.5, 0.2), dist=2.0, width=48, height=48, sp
     proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
     ubo = S.make_ubo(view, proj, samples=spp, bounces=3, antialiasing=False)
     return b.finish("slab", inst, [], [L.default_dlight()], ubo, width, height)
+
+
+def blend_layer(alpha=0.3, e_layer=0.4, e_back=0.1, width=48, height=48, spp=16):
+    """One alpha-BLEND quad (constant opacity `alpha`, emissive `e_layer`) at y = 0 in
+    front of an opaque emissive quad (emissive `e_back`) at y = -2, seen from y = 5
+    straight down, depth 1 (only the first hit's emission counts), no lights."""
+    b = S.Builder()
+    layer = L.default_material()
+    layer["alpha_mode"] = L.ALPHA_BLEND
+    layer["base_color"] = (0.0, 0.0, 0.0, alpha)
+    layer["emissive_factor"][:3] = (e_layer, e_layer, e_layer)
+    back = L.default_material()
+    back["base_color"] = (0.0, 0.0, 0.0, 1.0)
+    back["emissive_factor"][:3] = (e_back, e_back, e_back)
+    geos = []
+    for y, mid in ((0.0, b.add_material(layer)), (-2.0, b.add_material(back))):
+        # the quad's diagonal runs far outside the view, so no camera ray meets two
+        # triangles of the layer at their shared edge
+        pos = np.array([[-50, y, -50], [50, y, -50], [50, y, 150], [-50, y, 150]], np.float64)
+        geos.append(b.add_primitive(S.make_vertices(pos, np.tile([0.0, 1.0, 0.0], (4, 1))),
+                                    np.array([0, 2, 1, 0, 3, 2], np.uint32), mid))
+    inst = S.make_instances([(None, g) for g in geos])
+    view = S.look_at_rh((0, 5, 0), (0, 0, 0), (0, 0, -1))
+    proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
+    ubo = S.make_ubo(view, proj, samples=spp, bounces=1, antialiasing=False, fully_opaque=False)
+    return b.finish("blend_layer", inst, [], [L.default_dlight()], ubo, width, height)
