@@ -122,29 +122,44 @@ def adjoint_iteration(ctx, steps, warmup):
     tuned = ctx.autotune(ds.tlas, ds.buffers, sc.ubo, W, H)
     tuned["frame_ms"] = round(tuned["frame_ms"], 2)
 
-    def once():
-        rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=rt.MODE_WAVEFRONT)
-        ev.record()
-        g = (2.0 * (acc - target)).contiguous()
-        rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads)
+    def measure(record):
+        # record: megakernel forward that records its paths, adjoint replaying them;
+        # else: autotuned wavefront forward, adjoint tracing the paths again
+        mode = rt.MODE_MEGAKERNEL if record else rt.MODE_WAVEFRONT
+        ctx.set_option(rt.OPTION_RECORD_PATHS, int(record))
+        ev = None
 
-    for _ in range(warmup):
-        ev = torch.cuda.Event(enable_timing=True)
-        once()
-    torch.cuda.synchronize()
-    times, fwd = [], []
-    for _ in range(steps):
-        e0, ev, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        e0.record()
-        once()
-        e1.record()
+        def once():
+            rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=mode)
+            ev.record()
+            g = (2.0 * (acc - target)).contiguous()
+            rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads)
+
+        for _ in range(warmup):
+            ev = torch.cuda.Event(enable_timing=True)
+            once()
         torch.cuda.synchronize()
-        times.append(e0.elapsed_time(e1))
-        fwd.append(e0.elapsed_time(ev))
-    return {"ms_per_iter": round(float(np.median(times)), 3), "forward_ms": round(float(np.median(fwd)), 3),
-            "adjoint_ms": round(float(np.median(times)) - float(np.median(fwd)), 3), "autotune": tuned,
+        times, fwd = [], []
+        for _ in range(steps):
+            e0, ev, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+            once()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+            fwd.append(e0.elapsed_time(ev))
+        ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
+        return float(np.median(times)), float(np.median(fwd))
+
+    res = {"traced_again": measure(False), "recorded": measure(True)}
+    paths = min(res, key=lambda k: res[k][0])
+    t, f = res[paths]
+    return {"ms_per_iter": round(t, 3), "forward_ms": round(f, 3), "adjoint_ms": round(t - f, 3), "paths": paths,
+            "ms_per_iter_by_paths": {k: round(v[0], 3) for k, v in res.items()}, "autotune": tuned,
             "workload": "Cornell box 512x512, 16 spp, depth 4, 512^2 RGBA32F base-colour + metallic-roughness "
-                        "parameter textures; forward + L2 gradient + path-replay adjoint"}
+                        "parameter textures; forward + L2 gradient + path-replay adjoint (the faster of: "
+                        "wavefront forward and an adjoint that traces the paths again, or megakernel forward "
+                        "recording its paths and an adjoint replaying them)"}
 
 
 def dynamic_step(ctx, sc, buf, blases, steps):

@@ -298,7 +298,15 @@ enum {
     /* Most triangles per BVH leaf (1 .. 16) of BLASes and flattened BVHs built
      * afterwards: a subtree with at most this many triangles becomes one leaf.
      * Same hits.  Default 2. */
-    RT_OPTION_BLAS_LEAF_SIZE = 10
+    RT_OPTION_BLAS_LEAF_SIZE = 10,
+    /* 1: rt_trace_rays (megakernel mode; the wavefront mode returns
+     * RT_ERR_UNSUPPORTED) also writes, per pixel, sample and bounce, what the
+     * adjoint's replay needs (hit, shadow result, radiance and throughput factors:
+     * 88 bytes) into a context-owned buffer (at most 32 GiB per frame).  A
+     * following rt_trace_rays_adjoint of the same TLAS, size and UBO samples,
+     * bounces, seed and frame then replays these paths instead of tracing them
+     * again; any other trace drops them.  Same gradients.  Default 0. */
+    RT_OPTION_RECORD_PATHS = 11
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 

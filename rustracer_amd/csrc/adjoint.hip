@@ -6,7 +6,9 @@
 // One lane per pixel, the megakernel's tiling.  Per sample:
 //   pass 1  the forward path (traversal, shading, shadow rays), recording per
 //           bounce the hit, the shadow result, the radiance the bounce adds per
-//           unit throughput (E_k) and the factor to the next throughput (M_k);
+//           unit throughput (E_k) and the factor to the next throughput (M_k) --
+//           or these records read back from the forward trace that wrote them
+//           (RT_OPTION_RECORD_PATHS);
 //   back    R_k = E_k + M_k R_{k+1}: the radiance from bounce k on per unit of
 //           throughput at k;
 //   pass 2  the path replayed from the recorded hits (no traversal): at each
@@ -34,6 +36,7 @@ struct AdjView {
     const float4* grad_accum;  // dL/d accum per pixel
     float* const* grads;       // per texture: gradient buffer (RGBA32F layout) or NULL
     uint32_t num_textures;
+    const PathRecord* rec;     // the forward's path records (pass 1 skipped), or NULL
 };
 
 struct BounceRec {
@@ -186,9 +189,25 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av)
         begin_sample(fa, x, y, p);
         if (fa.bounces == 0u) continue;
         const PathState p0 = p;
-        // pass 1: the forward path
         uint32_t nb = 0;
-        for (;;) {
+        if (av.rec) {
+            // pass 1 from the records the forward trace wrote
+            const PathRecord* pr = av.rec + ((size_t(y) * fa.width + x) * fa.samples + p.sample) * fa.bounces;
+            for (;;) {
+                const PathRecord& q = pr[nb];
+                BounceRec& br = rec[nb];
+                br.h = HitInfo{q.t, q.u, q.v, q.inst, q.prim, q.rec};
+                br.wb = q.wb;
+                br.flags = q.flags & (kRecHit | kRecUnoccluded);
+                br.e = d3(q.e[0], q.e[1], q.e[2]);
+                br.m = d3(q.m[0], q.m[1], q.m[2]);
+                br.rr = q.rr;
+                nb++;
+                if ((q.flags & kRecLast) || nb == kMaxAdjointBounces || nb == fa.bounces) break;
+            }
+        }
+        // pass 1 without records: the forward path traced again
+        while (!av.rec) {
             Rng4 rb{x, y, fa.seed_b, p.wb};
             AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
             HitInfo h;
@@ -233,10 +252,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av)
 }  // namespace
 
 int launch_adjoint(const FrameArgs& fa, const float* grad_accum, float* const* d_grads, uint32_t num_textures,
-                   hipStream_t st, bool any_hit) {
+                   const PathRecord* rec, hipStream_t st, bool any_hit) {
     uint32_t tiles = ((fa.width + 15) / 16) * ((fa.height + 7) / 8);
     if (tiles == 0) return 0;
-    AdjView av{reinterpret_cast<const float4*>(grad_accum), d_grads, num_textures};
+    AdjView av{reinterpret_cast<const float4*>(grad_accum), d_grads, num_textures, rec};
     if (any_hit) hipLaunchKernelGGL((k_adjoint<true>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
     else hipLaunchKernelGGL((k_adjoint<false>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
     return int(hipGetLastError());

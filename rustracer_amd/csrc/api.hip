@@ -26,6 +26,7 @@ namespace {
 
 constexpr uint32_t kAutotuneSamples = 8;  // samples per pixel of the autotuner's probe frames
 constexpr double kAutotuneMargin = 0.015;  // relative gain a probe needs to replace the incumbent
+constexpr uint64_t kMaxRecordBytes = 32ull << 30;  // path records of one frame (RT_OPTION_RECORD_PATHS)
 constexpr uint32_t kBlasLeafMax = 2;  // default triangles per leaf (RT_OPTION_BLAS_LEAF_SIZE)
 constexpr uint32_t kTlasLeafMax = 1;
 
@@ -119,6 +120,14 @@ struct rt_ctx {
     uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
+    bool record = false;             // RT_OPTION_RECORD_PATHS
+    PathRecord* d_rec = nullptr;     // path records of the last recording trace
+    size_t rec_cap = 0;              // records d_rec holds
+    struct {
+        bool valid = false;
+        rt_tlas tlas = 0;
+        uint32_t width = 0, height = 0, samples = 0, bounces = 0, seed = 0, frame = 0;
+    } rec_frame;  // the frame the records belong to
 };
 
 namespace {
@@ -352,6 +361,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
     if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
     if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
+    if (ctx->d_rec) (void)hipFree(ctx->d_rec);
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_textures) (void)hipFree(ctx->d_textures);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -832,9 +842,27 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     fa.o.out = reinterpret_cast<uint32_t*>(out_rgba8);
     fa.o.bgra = ctx->out_bgra ? 1u : 0u;
     const bool any_hit = !fa.fully_opaque;
-    // the counting kernels also carry the debug == 1 materials
-    const bool counting = stats != nullptr || fa.debug != 0u;
+    // the counting kernels also carry the debug == 1 materials and the path records
+    const bool counting = stats != nullptr || fa.debug != 0u || ctx->record;
     hipStream_t st = ctx->stream;
+    ctx->rec_frame.valid = false;
+    if (ctx->record) {
+        if (mode != RT_MODE_MEGAKERNEL)
+            return fail(ctx, RT_ERR_UNSUPPORTED, "RT_OPTION_RECORD_PATHS records in the megakernel mode");
+        const uint64_t n = uint64_t(width) * height * b->ubo->number_of_samples * b->ubo->number_of_bounces;
+        if (n * sizeof(PathRecord) > kMaxRecordBytes)
+            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "the path records of this frame exceed 32 GiB");
+        if (n > ctx->rec_cap) {
+            if (ctx->d_rec) (void)hipFree(ctx->d_rec);
+            ctx->d_rec = nullptr;
+            ctx->rec_cap = 0;
+            hipError_t e;
+            ctx->d_rec = dev_alloc<PathRecord>(size_t(n), &e);
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc path records");
+            ctx->rec_cap = size_t(n);
+        }
+        fa.rec_out = ctx->d_rec;
+    }
     if (stats) {
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
         RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
@@ -858,6 +886,17 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
     if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
+    if (ctx->record) {
+        const rt_ubo& u = *b->ubo;
+        ctx->rec_frame.tlas = tlas;
+        ctx->rec_frame.width = width;
+        ctx->rec_frame.height = height;
+        ctx->rec_frame.samples = u.number_of_samples;
+        ctx->rec_frame.bounces = u.number_of_bounces;
+        ctx->rec_frame.seed = u.random_seed;
+        ctx->rec_frame.frame = u.frame_count;
+        ctx->rec_frame.valid = true;
+    }
     if (stats) return read_stats(ctx, stats);
     return RT_OK;
 }
@@ -867,6 +906,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (count && !items) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "items is NULL");
     if (mode != RT_MODE_MEGAKERNEL && mode != RT_MODE_WAVEFRONT) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown mode");
+    ctx->rec_frame.valid = false;  // batches do not record paths
     size_t ntex = 0;
     for (uint32_t i = 0; i < count; i++) {
         const rt_batch_item& it = items[i];
@@ -969,7 +1009,6 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     hipStream_t st = ctx->stream;
     if (nt > ctx->grad_cap) {
         if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
-    if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
         ctx->d_grad_ptrs = nullptr;
         ctx->grad_cap = 0;
         RT_HIP(ctx, hipMalloc(&ctx->d_grad_ptrs, sizeof(float*) * nt), "hipMalloc");
@@ -986,7 +1025,13 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
         RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
     }
-    rc = launch_adjoint(fa, d_grad_accum, ctx->d_grad_ptrs, nt, st, !fa.fully_opaque);
+    // the last recording trace's paths, when it traced this frame
+    const auto& rf = ctx->rec_frame;
+    const rt_ubo& u = *b->ubo;
+    const bool use_rec = rf.valid && rf.tlas == tlas && rf.width == width && rf.height == height &&
+                         rf.samples == u.number_of_samples && rf.bounces == u.number_of_bounces &&
+                         rf.seed == u.random_seed && rf.frame == u.frame_count;
+    rc = launch_adjoint(fa, d_grad_accum, ctx->d_grad_ptrs, nt, use_rec ? ctx->d_rec : nullptr, st, !fa.fully_opaque);
     if (rc) return hip_fail(ctx, hipError_t(rc), "adjoint launch");
     if (stats) {
         RT_HIP(ctx, hipEventRecord(ctx->ev1, st), "hipEventRecord");
@@ -1176,6 +1221,12 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (option == RT_OPTION_GRAPH) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_GRAPH takes 0 or 1");
         ctx->use_graph = value != 0;
+        return RT_OK;
+    }
+    if (option == RT_OPTION_RECORD_PATHS) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_RECORD_PATHS takes 0 or 1");
+        ctx->record = value != 0;
+        if (!ctx->record) ctx->rec_frame.valid = false;
         return RT_OK;
     }
     if (option == RT_OPTION_BLAS_LEAF_SIZE) {

@@ -667,10 +667,27 @@ __device__ inline void run_samples(const FrameArgs& fa, uint32_t x, uint32_t y, 
             bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(p.o), to_f3(p.d), RayFromF64{p.o, p.d}, float(kTMin),
                                                     float(p.tfar), flags, ah, stack, stride, h, pc.tr);
             p.wb = rb.w;
+            const uint32_t bounce = p.bounce, wb = p.wb;
             BounceOut r;
             shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
             bool unocc = r.shadow && shadow_unoccluded<kAnyHit, kStats>(fa, x, y, r, stack, stride, pc);
             p.rad = p.rad + (unocc ? r.pend_with : r.pend_without);
+            // the counting kernels also record the path for the adjoint's replay
+            if (kStats && fa.rec_out) {
+                PathRecord& q = fa.rec_out[((size_t(y) * fa.width + x) * fa.samples + p.sample) * fa.bounces + bounce];
+                q.inst = found ? h.inst : kEmpty;
+                q.prim = h.prim;
+                q.rec = h.rec;
+                q.wb = wb;
+                q.t = h.t;
+                q.u = h.u;
+                q.v = h.v;
+                q.flags = (found ? kRecHit : 0u) | (unocc ? kRecUnoccluded : 0u) | (r.cont ? 0u : kRecLast);
+                const D3 e = unocc ? r.emit + r.nee : r.emit;
+                q.e[0] = e.x; q.e[1] = e.y; q.e[2] = e.z;
+                q.m[0] = r.mult.x; q.m[1] = r.mult.y; q.m[2] = r.mult.z;
+                q.rr = r.rr;
+            }
             if (!r.cont) break;
         }
     }

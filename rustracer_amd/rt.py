@@ -97,6 +97,7 @@ OPTION_OUTPUT_BGRA = 7
 OPTION_WAVEFRONT_STREAMS = 8
 OPTION_WAVEFRONT_FINISH = 9
 OPTION_BLAS_LEAF_SIZE = 10
+OPTION_RECORD_PATHS = 11
 TONE_MAP_DEFAULT, TONE_MAP_UNCHARTED, TONE_MAP_HEJL_RICHARD, TONE_MAP_ACES, TONE_MAP_SRGB = 0, 1, 2, 3, 4
 NUM_TRAVERSAL_VARIANTS = 4
 

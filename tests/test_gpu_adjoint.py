@@ -118,3 +118,38 @@ def test_autograd_function_and_an_optimisation_step(gpu_ctx):
         losses.append(float(loss.detach()))
     print("losses", losses)
     assert losses[-1] < 0.9 * losses[0]
+
+
+def test_recorded_paths_give_the_same_gradients(gpu_ctx):
+    """RT_OPTION_RECORD_PATHS: the megakernel forward records its paths, the adjoint
+    replays them instead of tracing again; the forward image is unchanged and the
+    gradients equal those of the re-tracing adjoint (up to the order of the float
+    atomics); a trace in between drops the records; the wavefront mode refuses to
+    record."""
+    ds = G.device_scene(gpu_ctx, "cornell_params_small", _small)
+    sc = ds.scene
+    rng = np.random.default_rng(5)
+    g = rng.normal(size=(sc.height, sc.width, 4)).astype(np.float32)
+    g[..., 3] = 0
+    dg = torch.from_numpy(g).cuda()
+    a0 = rt.new_accumulation(sc.width, sc.height)
+    rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, a0, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL)
+    ref = _adjoint(gpu_ctx, ds, dg)  # no records: traces the paths again
+    gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 1)
+    try:
+        a1 = rt.new_accumulation(sc.width, sc.height)
+        rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, a1, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL)
+        gpu_ctx.sync()
+        assert torch.equal(a0, a1)
+        rec = _adjoint(gpu_ctx, ds, dg)
+        for k in ref:
+            assert np.any(ref[k] != 0)
+            assert np.allclose(rec[k], ref[k], rtol=1e-5, atol=1e-7 * np.abs(ref[k]).max()), k
+        again = _adjoint(gpu_ctx, ds, dg)  # the records stay for a second adjoint
+        for k in ref:
+            assert np.allclose(again[k], ref[k], rtol=1e-5, atol=1e-7 * np.abs(ref[k]).max())
+        with pytest.raises(rt.RtError) as e:
+            rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, a1, sc.width, sc.height, mode=rt.MODE_WAVEFRONT)
+        assert e.value.code == -6
+    finally:
+        gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
