@@ -1225,8 +1225,7 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     }
     if (option == RT_OPTION_RECORD_PATHS) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_RECORD_PATHS takes 0 or 1");
-        ctx->record = value != 0;
-        if (!ctx->record) ctx->rec_frame.valid = false;
+        ctx->record = value != 0;  // the last trace's records stay until the next trace
         return RT_OK;
     }
     if (option == RT_OPTION_BLAS_LEAF_SIZE) {

@@ -18,12 +18,19 @@ class TraceFunction(torch.autograd.Function):
     tensors, in the order of `slots`; their gradients come from the adjoint."""
 
     @staticmethod
-    def forward(fctx, rctx, tlas, buffers, ubo, width, height, slots, *params):
+    def forward(fctx, rctx, tlas, buffers, ubo, width, height, slots, record, *params):
         for k, p in zip(slots, params):
             if buffers.textures[k].data_ptr() != p.data_ptr():
                 buffers.set_texture(k, p)
         accum = rt.new_accumulation(width, height)
-        rt.trace_rays(rctx, tlas, buffers, ubo, accum, width, height, mode=rt.MODE_WAVEFRONT)
+        if record:  # megakernel forward that records its paths for the adjoint
+            rctx.set_option(rt.OPTION_RECORD_PATHS, 1)
+            try:
+                rt.trace_rays(rctx, tlas, buffers, ubo, accum, width, height, mode=rt.MODE_MEGAKERNEL)
+            finally:
+                rctx.set_option(rt.OPTION_RECORD_PATHS, 0)
+        else:
+            rt.trace_rays(rctx, tlas, buffers, ubo, accum, width, height, mode=rt.MODE_WAVEFRONT)
         fctx.state = (rctx, tlas, buffers, ubo, width, height, slots)
         fctx.save_for_backward(*params)
         return accum
@@ -34,17 +41,20 @@ class TraceFunction(torch.autograd.Function):
         params = fctx.saved_tensors
         grads = {k: torch.zeros_like(p) for k, p in zip(slots, params)}
         rt.trace_rays_adjoint(rctx, tlas, buffers, ubo, grad_accum.contiguous(), width, height, grads)
-        return (None,) * 7 + tuple(grads[k] for k in slots)
+        return (None,) * 8 + tuple(grads[k] for k in slots)
 
 
 class DifferentiableScene:
     """Device scene whose parameter texture slots are trainable tensors.
 
     slots: texture indices (RGBA32F) to optimise; their initial values come from
-    the scene's textures."""
+    the scene's textures.  record_paths: the forward traces in the megakernel mode
+    and records its paths (RT_OPTION_RECORD_PATHS), so the backward replays them
+    instead of tracing again (88 bytes per pixel, sample and bounce)."""
 
-    def __init__(self, ctx, scene, slots):
+    def __init__(self, ctx, scene, slots, record_paths=False):
         self.ctx = ctx
+        self.record_paths = bool(record_paths)
         self.scene = scene
         self.buffers = rt.Buffers(scene)
         self.blases, self.tlas = rt.create_as(ctx, scene, self.buffers)
@@ -62,4 +72,4 @@ class DifferentiableScene:
         sc = self.scene
         ubo = sc.ubo if ubo is None else ubo
         return TraceFunction.apply(self.ctx, self.tlas, self.buffers, ubo, sc.width, sc.height, self.slots,
-                                   *self.params)
+                                   self.record_paths, *self.params)

@@ -85,16 +85,18 @@ def test_adjoint_rejects_gradient_buffers_for_non_float_textures(gpu_ctx):
         rt.trace_rays_adjoint(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, g, sc.width, sc.height, bad)
 
 
-def test_autograd_function_and_an_optimisation_step(gpu_ctx):
+@pytest.mark.parametrize("record_paths", [False, True])
+def test_autograd_function_and_an_optimisation_step(gpu_ctx, record_paths):
     """loss = sum((accum - target)^2): torch.autograd gives the adjoint's gradient for
-    g = 2 (accum - target), and gradient steps on the parameters lower the loss."""
+    g = 2 (accum - target), and gradient steps on the parameters lower the loss (with
+    the forward recording its paths for the backward, or not)."""
     sc = S.cornell_params(width=64, height=64, spp=4, bounces=4, tex_size=16)
     tgt_scene = S.cornell_params(width=64, height=64, spp=4, bounces=4, tex_size=16, target=True)
     tds = G.DeviceScene(gpu_ctx, tgt_scene)
     target = rt.new_accumulation(64, 64)
     rt.trace_rays(gpu_ctx, tds.tlas, tds.buffers, tgt_scene.ubo, target, 64, 64)
     gpu_ctx.sync()
-    dscene = diff.DifferentiableScene(gpu_ctx, sc, [1, 2])
+    dscene = diff.DifferentiableScene(gpu_ctx, sc, [1, 2], record_paths=record_paths)
     img = dscene.render()
     loss = ((img - target) ** 2).sum()
     loss.backward()
