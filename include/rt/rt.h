@@ -305,8 +305,9 @@ enum {
      * 88 bytes) into a context-owned buffer (at most 32 GiB per frame).  A
      * following rt_trace_rays_adjoint of the same TLAS, size and UBO samples,
      * bounces, seed and frame then replays these paths instead of tracing them
-     * again; the next trace drops them (turning the option off does not).  Same
-     * gradients.  Default 0. */
+     * again; the next rt_trace_rays or rt_trace_rays_batch drops them (turning the
+     * option off does not).  rt_autotune does not record.  Same gradients.
+     * Default 0. */
     RT_OPTION_RECORD_PATHS = 11
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);

@@ -1073,6 +1073,8 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
     const int keep_streams = ctx->streams;
     const uint32_t keep_finish = ctx->finish;
+    const bool keep_record = ctx->record;
+    ctx->record = false;  // the probe frames are wavefront traces
     // one probe: the faster of two timed frames of the current settings (after a
     // warm-up frame), or < 0 when a traversal gave up
     auto probe = [&](double& ms_out) -> int {
@@ -1178,6 +1180,7 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         ctx->variant = chosen;
     }
     ctx->stage_timing = timing;
+    ctx->record = keep_record;
     if (rc) {
         ctx->variant = keep_variant;
         ctx->use_graph = keep_graph;

@@ -153,5 +153,10 @@ def test_recorded_paths_give_the_same_gradients(gpu_ctx):
         with pytest.raises(rt.RtError) as e:
             rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, a1, sc.width, sc.height, mode=rt.MODE_WAVEFRONT)
         assert e.value.code == -6
+        # the autotuner's wavefront probes run without recording, and leave the option on
+        gpu_ctx.autotune(ds.tlas, ds.buffers, sc.ubo, sc.width, sc.height)
+        rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, a1, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL)
+        gpu_ctx.sync()
+        assert torch.equal(a0, a1)
     finally:
         gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
