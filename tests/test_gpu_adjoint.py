@@ -160,3 +160,6 @@ def test_recorded_paths_give_the_same_gradients(gpu_ctx):
         assert torch.equal(a0, a1)
     finally:
         gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
+        for opt, v in ((rt.OPTION_TRAVERSAL_VARIANT, 0), (rt.OPTION_WAVEFRONT_STREAMS, 2),
+                       (rt.OPTION_WAVEFRONT_FINISH, 65536), (rt.OPTION_EXTEND_REFILL, 40), (rt.OPTION_GRAPH, 0)):
+            gpu_ctx.set_option(opt, v)  # the context's defaults, for the tests that follow
