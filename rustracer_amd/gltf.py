@@ -183,15 +183,21 @@ def _flat_normals(pos, idx):
     """create_geo_normal (geometry.rs:270-290): per triangle, the normalised cross
     product of the normalised edges, written to its three vertices in order."""
     n = np.zeros((len(pos), 4), np.float32)
-    for t in range(len(idx) // 3):
-        i0, i1, i2 = (int(x) for x in idx[3 * t:3 * t + 3])
-        v0 = pos[i1] - pos[i0]
-        v1 = pos[i2] - pos[i0]
-        v0 = v0 / np.linalg.norm(v0)
-        v1 = v1 / np.linalg.norm(v1)
+    tri = np.asarray(idx, np.int64).reshape(-1, 3)
+    if len(tri) == 0:
+        return n
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v0 = pos[tri[:, 1]] - pos[tri[:, 0]]
+        v1 = pos[tri[:, 2]] - pos[tri[:, 0]]
+        v0 = v0 / np.linalg.norm(v0, axis=1, keepdims=True)
+        v1 = v1 / np.linalg.norm(v1, axis=1, keepdims=True)
         c = np.cross(v0, v1)
-        c = c / np.linalg.norm(c)
-        n[i0, :3] = n[i1, :3] = n[i2, :3] = c
+        c = c / np.linalg.norm(c, axis=1, keepdims=True)
+    # a vertex shared by several triangles keeps the last one's normal (written in order)
+    last = np.full(len(pos), -1, np.int64)
+    np.maximum.at(last, tri.reshape(-1), np.repeat(np.arange(len(tri)), 3))
+    used = last >= 0
+    n[used, :3] = c[last[used]]
     return n
 
 

@@ -236,3 +236,29 @@ def test_skybox_directory_face_order(tmp_path):
 def test_reference_skybox_directory():
     sky = gltf.load_skybox(REF_SKYBOX)
     assert sky.data.shape[0] == 6 and sky.data.shape[1] == sky.data.shape[2] and sky.data.shape[3] == 4
+
+
+def test_unsupported_inputs_raise_and_extensions_map(tmp_path):
+    """Skins and sparse accessors are refused; KHR_materials_unlit and
+    KHR_materials_pbrSpecularGlossiness map to MaterialRaw's unlit flag and SG workflow
+    (material.rs:162-190, 289-330)."""
+    import json as _json
+    p = _write(tmp_path)
+    js = _json.load(open(p))
+    js["materials"][0]["extensions"] = {"KHR_materials_unlit": {}}
+    js["materials"][1]["extensions"] = {"KHR_materials_pbrSpecularGlossiness": {
+        "diffuseFactor": [0.5, 0.4, 0.3, 1.0], "specularFactor": [0.2, 0.2, 0.2], "glossinessFactor": 0.6,
+        "diffuseTexture": {"index": 0}}}
+    open(p, "w").write(_json.dumps(js))
+    sc = gltf.load(p)
+    assert sc.materials[0]["unlit"] == 1 and sc.materials[1]["workflow"] == 1
+    assert np.allclose(sc.materials[1]["sg_specular_glossiness_factor"], [0.2, 0.2, 0.2, 0.6])
+    assert tuple(sc.materials[1]["sg_diffuse_texture"]) == (1, 0)
+    skinned = dict(js, skins=[{"joints": [0]}])
+    open(p, "w").write(_json.dumps(skinned))
+    with pytest.raises(ValueError):
+        gltf.load(p)
+    js["accessors"][0]["sparse"] = {"count": 1}
+    open(p, "w").write(_json.dumps(js))
+    with pytest.raises(ValueError):
+        gltf.load(p)
