@@ -359,10 +359,11 @@ int rt_get_stage_times(rt_ctx* ctx, rt_stage_times* out);
  * `out_rgba8` (nullable, 4-byte aligned, width*height*4 bytes) receives the display
  * image of RayTracing.rgen:132-166: accum / total_number_of_samples tone-mapped by
  * ubo->tone_mapping_mode (0 default, 1 Uncharted 2, 2 Hejl-Richard, 3 ACES, else
- * gamma 2.2 only), replaced by the DISTANCE mapping's grey level or, in the
- * megakernel, the HEAT mapping's colour ramp of the pixel's shader-clock ticks
- * (RT_ERR_UNSUPPORTED in the wavefront mode), stored as 8-bit UNORM with alpha 255
- * (byte order: RT_OPTION_OUTPUT_BGRA).
+ * gamma 2.2 only), replaced by the DISTANCE mapping's grey level or the HEAT
+ * mapping's colour ramp of the pixel's shader-clock ticks (a HEAT frame with a
+ * display image runs in the megakernel form whatever `mode` says: only there does
+ * one thread run a pixel's whole raygen; the accumulation is the same), stored as
+ * 8-bit UNORM with alpha 255 (byte order: RT_OPTION_OUTPUT_BGRA).
  * `mode` selects the plain megakernel or the wavefront pipeline; both produce the
  * same image bit for bit. */
 int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,

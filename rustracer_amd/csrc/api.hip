@@ -828,8 +828,9 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     if (mode != RT_MODE_MEGAKERNEL && mode != RT_MODE_WAVEFRONT) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown mode");
     if (out_rgba8 && (reinterpret_cast<uintptr_t>(out_rgba8) & 3u))
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out_rgba8 must be 4-byte aligned");
-    if (out_rgba8 && b->ubo->mapping == RT_MAPPING_HEAT && mode == RT_MODE_WAVEFRONT)
-        return fail(ctx, RT_ERR_UNSUPPORTED, "the HEAT mapping (per-pixel shader clock) needs the megakernel mode");
+    // the HEAT mapping shows each pixel's shader clock: only the megakernel form runs a
+    // pixel's whole raygen in one thread, so such frames take it (same accumulation)
+    if (out_rgba8 && b->ubo->mapping == RT_MAPPING_HEAT) mode = RT_MODE_MEGAKERNEL;
     if (out_rgba8 && b->ubo->total_number_of_samples == 0)
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "the display image needs total_number_of_samples > 0");
     auto it = ctx->tlas.find(tlas);

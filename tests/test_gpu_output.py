@@ -96,8 +96,9 @@ def test_distance_mapping_uses_the_last_trace_t(gpu_ctx):
 
 
 def test_heat_mapping_draws_from_the_colour_ramp(gpu_ctx):
-    """HEAT (megakernel): each pixel is the ramp's colour at its clock ticks; every
-    output lies on the oracle's ramp.  The wavefront mode rejects it."""
+    """HEAT: each pixel is the ramp's colour at its clock ticks; every output lies on
+    the oracle's ramp.  A wavefront-mode call takes the megakernel form for such a
+    frame (the same accumulation)."""
     ds = G.device_scene(gpu_ctx, "cornell_sphere")
     sc = ds.scene
     ubo = sc.ubo.copy()
@@ -110,5 +111,10 @@ def test_heat_mapping_draws_from_the_colour_ramp(gpu_ctx):
     assert dist.max() <= 1
     acc = rt.new_accumulation(sc.width, sc.height)
     o = torch.zeros((sc.height, sc.width, 4), dtype=torch.uint8, device="cuda")
-    with pytest.raises(rt.RtError):
-        rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, ubo, acc, sc.width, sc.height, mode=rt.MODE_WAVEFRONT, out=o)
+    rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, ubo, acc, sc.width, sc.height, mode=rt.MODE_WAVEFRONT, out=o)
+    ref = rt.new_accumulation(sc.width, sc.height)
+    rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, ubo, ref, sc.width, sc.height, mode=rt.MODE_WAVEFRONT)
+    gpu_ctx.sync()
+    assert torch.equal(acc, ref)
+    pw = np.unique(o.cpu().numpy().reshape(-1, 4)[:, :3].astype(np.int32), axis=0)
+    assert np.abs(pw[:, None, :] - ramp[None, :, :]).max(axis=2).min(axis=1).max() <= 1
