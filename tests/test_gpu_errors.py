@@ -84,6 +84,24 @@ def test_trace_argument_checks(gpu_ctx):
     assert lib.rt_trace_rays(h, ds.tlas.handle, C.byref(b3), C.c_void_p(acc.data_ptr()), None, 8, 8, 0, None) == -1
 
 
+def test_refit_and_record_argument_checks(gpu_ctx):
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    assert lib.rt_refit_tlas(h, 987654) == -7  # unknown TLAS handle
+    assert lib.rt_set_option(h, rt.OPTION_RECORD_PATHS, 2) == -1
+    ds = G.device_scene(gpu_ctx, "cornell_sphere")
+    sc = ds.scene
+    big = sc.ubo.copy()
+    big["number_of_samples"] = big["total_number_of_samples"] = 1 << 16  # 256^2 x 65536 x 4 x 88 B > 32 GiB
+    acc = rt.new_accumulation(sc.width, sc.height)
+    gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 1)
+    try:
+        with pytest.raises(rt.RtError) as e:
+            rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, big, acc, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL)
+        assert e.value.code == -1 and "32 GiB" in str(e.value)
+    finally:
+        gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
+
+
 def test_out_of_range_material_is_reported_not_read(gpu_ctx):
     # a PrimInfo pointing past the material buffer: the counting trace reports it
     sc = G.host_scene("cornell_sphere")
