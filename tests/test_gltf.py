@@ -262,3 +262,17 @@ def test_unsupported_inputs_raise_and_extensions_map(tmp_path):
     open(p, "w").write(_json.dumps(js))
     with pytest.raises(ValueError):
         gltf.load(p)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CORNELL), reason="the reference checkout is not present")
+def test_reference_cornell_box_renders_in_the_oracle():
+    """The loaded asset through the oracle: its BLEND sphere runs any-hit, its five
+    fallback lights have zero intensity so no shadow ray is cast (RIS skips them), and
+    the emissive box lights the room above the 0.01 miss."""
+    from oracle import binding as B
+    sc = gltf.load(REF_CORNELL, width=48, height=48, spp=2, bounces=4, eye=(0, 0, 12), target=(0, 0, 0), fov=60)
+    o, st = B.OracleScene(sc).render()
+    assert np.all(np.isfinite(o))
+    assert st["anyhit_calls"] > 0 and st["shadow_rays"] == 0
+    img = o[..., :3] / 2
+    assert (img > 0.011).mean() > 0.02 and img.max() > 1.0
