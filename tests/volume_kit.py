@@ -63,3 +63,53 @@ def blend_layer(alpha=0.3, e_layer=0.4, e_back=0.1, width=48, height=48, spp=16)
     proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
     ubo = S.make_ubo(view, proj, samples=spp, bounces=1, antialiasing=False, fully_opaque=False)
     return b.finish("blend_layer", inst, [], [L.default_dlight()], ubo, width, height)
+
+
+# Emission of the near layer, the far layer and the opaque back quad of blend_stack,
+# each in a colour channel of its own (red, green, blue).
+STACK_EMIT = (0.4, 0.2, 0.1)
+
+
+def blend_stack(alphas=(0.3, 0.6), width=48, height=48, spp=16):
+    """Two alpha-BLEND quads (constant opacities `alphas`) at y = 0 and y = -1 in front
+    of an opaque quad at y = -2, seen from y = 5 straight down, depth 1, no lights.
+    The near layer emits only red, the far layer only green and the back only blue
+    (STACK_EMIT), so each channel of a pixel counts the samples that ended on one
+    surface."""
+    b = S.Builder()
+    geos = []
+    for k, y in enumerate((0.0, -1.0, -2.0)):
+        m = L.default_material()
+        if k < 2:
+            m["alpha_mode"] = L.ALPHA_BLEND
+            m["base_color"] = (0.0, 0.0, 0.0, alphas[k])
+        else:
+            m["base_color"] = (0.0, 0.0, 0.0, 1.0)
+        e = [0.0, 0.0, 0.0]
+        e[k] = STACK_EMIT[k]
+        m["emissive_factor"][:3] = e
+        mid = b.add_material(m)
+        # diagonals far outside the view, as in blend_layer
+        pos = np.array([[-50, y, -50], [50, y, -50], [50, y, 150], [-50, y, 150]], np.float64)
+        geos.append(b.add_primitive(S.make_vertices(pos, np.tile([0.0, 1.0, 0.0], (4, 1))),
+                                    np.array([0, 2, 1, 0, 3, 2], np.uint32), mid))
+    inst = S.make_instances([(None, g) for g in geos])
+    view = S.look_at_rh((0, 5, 0), (0, 0, 0), (0, 0, -1))
+    proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
+    ubo = S.make_ubo(view, proj, samples=spp, bounces=1, antialiasing=False, fully_opaque=False)
+    return b.finish("blend_stack", inst, [], [L.default_dlight()], ubo, width, height)
+
+
+def stack_counts(img, exposure=5.0):
+    """Per-pixel sample counts (near, far, back) of a blend_stack image, as float64
+    (the caller checks that they are whole numbers)."""
+    e = np.float32(STACK_EMIT) * np.float32(exposure)
+    return np.stack([img[..., c] / float(e[c]) for c in range(3)], axis=-1)
+
+
+def stack_probabilities(alphas):
+    """Probability that a sample ends on the near layer, the far layer or the back:
+    a1, (1 - a1) a2 and (1 - a1)(1 - a2), whatever order the two any-hit draws are made
+    in (a layer is kept only if its own draw keeps it and no nearer surface was kept)."""
+    a1, a2 = alphas
+    return np.array([a1, (1 - a1) * a2, (1 - a1) * (1 - a2)])
