@@ -1,6 +1,6 @@
 // oracle.cpp — CPU restatement of the reference ray-tracing path (FP64 and FP32).
 //
-// TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Loaded by tests/,
+// TEST INFRASTRUCTURE ONLY: the CPU reference the HIP library is tested against.  Loaded by tests/,
 // __graft_entry__.smoke() and bench.py's CPU baseline; never by the product path.
 //
 // Every function cites the reference file:line it restates.  Paths are relative

@@ -4,8 +4,8 @@
  * raygen / closest-hit / any-hit / miss shaders and their libraries
  * (crates/examples/gltf_viewer/shaders/RayTracing.*, shaders/lib/*.glsl).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP library in
- * rustracer_amd/csrc; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * TEST INFRASTRUCTURE ONLY.  This library is the CPU reference that tests compare the HIP library in
+ * rustracer_amd/csrc against; only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline load it.  Nothing in the product path calls it.
  *
  * Pins (see tests/test_oracle_*.py): RNG against an independent numpy restatement

@@ -1,4 +1,4 @@
-"""Failure handling of the C ABI on a GPU box: invalid arguments, handles and
+"""Failure handling of the C ABI on the GPU: invalid arguments, handles and
 references come back as status codes (the reference panics or leaves them
 undefined, e.g. crates/libs/asset_loader/src/acceleration_structures.rs:33,109);
 nothing faults."""
