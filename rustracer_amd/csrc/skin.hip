@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kSkinBlock) void k_skin(const rt_vertex* in, rt_ver
                                __float_as_uint(w[5].w)};
         if (uint32_t(skin) >= num_skins || j[0] >= kBonesPerSkin || j[1] >= kBonesPerSkin || j[2] >= kBonesPerSkin ||
             j[3] >= kBonesPerSkin) {
-            atomicOr(bad, 1u);
+            atomicOr(bad, kErrBadSkin);
         } else {
             // transform = weights.x * bones[joints.x] + ... + weights.w * bones[joints.w]
             const float* b = skins + size_t(skin) * kBonesPerSkin * 16;

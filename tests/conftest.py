@@ -27,3 +27,13 @@ def gpu_ctx():
     ctx = rt.Context(0)
     yield ctx
     ctx.close()
+
+
+@pytest.fixture(autouse=True)
+def _fresh_context_options(request):
+    """Tests share one context: whatever options a test (or the autotuner it runs)
+    set are put back to their defaults afterwards, so every test runs the settings
+    it names."""
+    yield
+    if "gpu_ctx" in request.fixturenames:
+        request.getfixturevalue("gpu_ctx").reset_options()

@@ -98,3 +98,52 @@ def plane_scene(width=33, height=33, base=(0.5, 0.6, 0.7), roughness=1.0, light_
     proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
     ubo = S.make_ubo(view, proj, samples=spp, bounces=bounces, antialiasing=False)
     return b.finish("plane", inst, plights, [L.default_dlight()] if dlights else [], ubo, width, height)
+
+
+def lit_plane(plights=(), dlights=None, base=(0.5, 0.6, 0.7), metallic=0.0, roughness=1.0, width=33, height=33,
+              spp=1, bounces=1, look_away=False, frame_count=0):
+    """The y = 0 quad of plane_scene under any list of point lights (pos, intensity,
+    colour) and directional lights (direction, intensity, colour); the camera looks
+    straight down from y = 5 (or straight up when look_away), without jitter."""
+    b = S.Builder()
+    m = L.default_material()
+    m["base_color"] = (*base, 1.0)
+    m["metallic_factor"] = metallic
+    m["roughness_factor"] = roughness
+    mid = b.add_material(m)
+    s = 50.0
+    pos = np.array([[-s, 0, -s], [s, 0, -s], [s, 0, s], [-s, 0, s]], np.float64)
+    g = b.add_primitive(S.make_vertices(pos, np.tile([0.0, 1.0, 0.0], (4, 1))), np.array([0, 2, 1, 0, 3, 2], np.uint32),
+                        mid)
+    inst = S.make_instances([(None, g)])
+    pl = [S.point_light(p, i, c) for p, i, c in plights]
+    if dlights is None:
+        dl = [L.default_dlight()]
+    else:
+        dl = []
+        for d, i, c in dlights:
+            x = L.default_dlight()
+            x["transform"] = (*d, 1.0)
+            x["intensity"] = i
+            x["color"][:3] = c
+            dl.append(x)
+    view = S.look_at_rh((0, 5, 0), (0, 10, 0) if look_away else (0, 0, 0), (0, 0, -1))
+    proj = S.perspective_vk(width / height, 40.0, 0.1, 100.0)
+    ubo = S.make_ubo(view, proj, samples=spp, bounces=bounces, antialiasing=False, frame_count=frame_count)
+    return b.finish("lit_plane", inst, pl, dl, ubo, width, height)
+
+
+def primary_dirs(sc):
+    """The camera-ray directions (unit, float64) of the pixel centres without jitter
+    (RayTracing.rgen:60-65 with aperture 0)."""
+    ubo = sc.ubo
+    mvi = ubo["model_view_inverse"].reshape(4, 4).T.astype(np.float64)
+    pin = ubo["projection_inverse"].reshape(4, 4).T.astype(np.float64)
+    ys, xs = np.meshgrid(np.arange(sc.height), np.arange(sc.width), indexing="ij")
+    u = (xs + 0.5) / sc.width * 2 - 1
+    v = (ys + 0.5) / sc.height * 2 - 1
+    tg = np.stack([u, v, np.ones_like(u), np.ones_like(u)], -1) @ pin.T
+    td = tg[..., :3] * float(ubo["focus_distance"])
+    td /= np.linalg.norm(td, axis=-1, keepdims=True)
+    d = np.concatenate([td, np.zeros_like(td[..., :1])], -1) @ mvi.T
+    return d[..., :3]

@@ -71,8 +71,9 @@ def test_out_of_range_skin_index_is_reported_and_left_unskinned(gpu_ctx):
     d = rt.to_device(v)
     out = torch.empty_like(d)
     rt.skin_vertices(gpu_ctx, d, out, torch.from_numpy(S.bend_skins(0.5)).cuda())
-    with pytest.raises(rt.RtError):
+    with pytest.raises(rt.RtError) as e:
         gpu_ctx.sync()
+    assert e.value.code == -1 and "skin index" in str(e.value)  # RT_ERR_INVALID_ARGUMENT
     got = _host_vertices(out)
     assert got[1].tobytes() == v[1].tobytes()
     gpu_ctx.sync()  # the flag was cleared
