@@ -68,6 +68,43 @@ def measured_traffic(kernel, variant):
         return None
 
 
+def host_cores():
+    """The cores the CPU baseline runs on: every core this process may run on
+    (sched_getaffinity), unless OMP_NUM_THREADS caps the process's share."""
+    allowed = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(allowed, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else allowed
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "affinity_cores": allowed, "nproc": os.cpu_count(), "omp_num_threads": cap,
+            "cpu_model": model}
+
+
+def lavapipe_probe():
+    """Which prerequisites of running the reference's own shaders under Mesa lavapipe
+    are present on this host: the Vulkan loader, a lavapipe ICD, a GLSL compiler and
+    a reference checkout ($RUSTRACER_REF).  The CPU baseline is the C++ port when any
+    is missing."""
+    import ctypes.util
+    import glob
+    import shutil
+    icds = []
+    for d in ("/usr/share/vulkan/icd.d", "/etc/vulkan/icd.d", "/usr/local/share/vulkan/icd.d"):
+        icds += glob.glob(os.path.join(d, "lvp_icd*.json"))
+    have = {"libvulkan": ctypes.util.find_library("vulkan") is not None,
+            "lavapipe_icd": bool(icds),
+            "glslc": shutil.which("glslc") is not None or shutil.which("glslangValidator") is not None,
+            "RUSTRACER_REF": bool(os.environ.get("RUSTRACER_REF")) and os.path.isdir(os.environ["RUSTRACER_REF"])}
+    return {"missing": [k for k, v in have.items() if not v], "usable": all(have.values())}
+
+
 def cpu_baseline(scene, target_s, threads):
     """FP32 oracle on a stratified pixel sample, timed on the host cores."""
     import numpy as np
@@ -99,12 +136,29 @@ def cpu_baseline(scene, target_s, threads):
                       f"({st['rays']} rays in {dt:.1f} s), FP32 C++ restatement, OpenMP"}
 
 
+def oracle_parity(scene, frame, threads, stride=32):
+    """relRMSE of the timed frame against the FP64 restatement on a stratified
+    subsample (every stride-th pixel of every stride-th row, all samples)."""
+    import numpy as np
+    from oracle import binding as OB
+    W, H = scene.width, scene.height
+    ys, xs = np.meshgrid(np.arange(stride // 2, H, stride), np.arange(stride // 2, W, stride), indexing="ij")
+    xy = np.stack([xs.reshape(-1), ys.reshape(-1)], 1)
+    o = OB.OracleScene(scene)
+    ref, _ = o.render_pixels(xy, precision=64, threads=threads)
+    o.close()
+    g = frame.astype(np.float64)[xy[:, 1], xy[:, 0], :3]
+    return {"vs_fp64_oracle_relRMSE": float(np.linalg.norm(g - ref[:, :3]) / np.linalg.norm(ref[:, :3])),
+            "oracle_pixels": int(len(xy))}
+
+
 def adjoint_iteration(ctx, steps, warmup):
     """One optimisation iteration of the parameter-texture workload: the Cornell box
     at 512x512, 16 spp, depth 4 with 512x512 RGBA32F base-colour and
-    metallic-roughness parameter textures: forward trace, L2 loss gradient, adjoint
-    (path replay).  The forward runs with settings autotuned on this scene, in a
-    context of its own.  Returns milliseconds per iteration and its split."""
+    metallic-roughness parameter textures and the point light's intensity: forward
+    trace, L2 loss gradient, path-replay adjoint (every sample traced twice, nothing
+    of the paths stored).  The forward runs with settings autotuned on this scene, in
+    a context of its own.  Returns milliseconds per iteration and its split."""
     import numpy as np
     import torch
     from rustracer_amd import diff, rt, scenes
@@ -115,51 +169,45 @@ def adjoint_iteration(ctx, steps, warmup):
     _, ttlas = rt.create_as(ctx, tsc, tbuf)
     target = rt.new_accumulation(tsc.width, tsc.height)
     rt.trace_rays(ctx, ttlas, tbuf, tsc.ubo, target, tsc.width, tsc.height, mode=rt.MODE_WAVEFRONT)
-    ds = diff.DifferentiableScene(ctx, sc, [1, 2])
+    ds = diff.DifferentiableScene(ctx, sc, [1, 2], lights=True)
     W, H = sc.width, sc.height
     grads = {k: torch.zeros_like(p) for k, p in zip(ds.slots, ds.params)}
+    gi = torch.zeros_like(ds.intensity)
     acc = rt.new_accumulation(W, H)
     tuned = ctx.autotune(ds.tlas, ds.buffers, sc.ubo, W, H)
     tuned["frame_ms"] = round(tuned["frame_ms"], 2)
+    ev = None
 
-    def measure(record):
-        # record: megakernel forward that records its paths, adjoint replaying them;
-        # else: autotuned wavefront forward, adjoint tracing the paths again
-        mode = rt.MODE_MEGAKERNEL if record else rt.MODE_WAVEFRONT
-        ctx.set_option(rt.OPTION_RECORD_PATHS, int(record))
-        ev = None
+    def once():
+        rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=rt.MODE_WAVEFRONT)
+        ev.record()
+        g = (2.0 * (acc - target)).contiguous()
+        rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads, grad_point_intensity=gi)
 
-        def once():
-            rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=mode)
-            ev.record()
-            g = (2.0 * (acc - target)).contiguous()
-            rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads)
-
-        for _ in range(warmup):
-            ev = torch.cuda.Event(enable_timing=True)
-            once()
+    for _ in range(warmup):
+        ev = torch.cuda.Event(enable_timing=True)
+        once()
+    torch.cuda.synchronize()
+    times, fwd = [], []
+    for _ in range(steps):
+        e0, ev, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        once()
+        e1.record()
         torch.cuda.synchronize()
-        times, fwd = [], []
-        for _ in range(steps):
-            e0, ev, e1 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-            e0.record()
-            once()
-            e1.record()
-            torch.cuda.synchronize()
-            times.append(e0.elapsed_time(e1))
-            fwd.append(e0.elapsed_time(ev))
-        ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
-        return float(np.median(times)), float(np.median(fwd))
-
-    res = {"traced_again": measure(False), "recorded": measure(True)}
-    paths = min(res, key=lambda k: res[k][0])
-    t, f = res[paths]
-    return {"ms_per_iter": round(t, 3), "forward_ms": round(f, 3), "adjoint_ms": round(t - f, 3), "paths": paths,
-            "ms_per_iter_by_paths": {k: round(v[0], 3) for k, v in res.items()}, "autotune": tuned,
+        times.append(e0.elapsed_time(e1))
+        fwd.append(e0.elapsed_time(ev))
+    t, f = float(np.median(times)), float(np.median(fwd))
+    # counters of one adjoint (counting kernel, not timed)
+    g = (2.0 * (acc - target)).contiguous()
+    st = rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads, stats=True, grad_point_intensity=gi)
+    return {"ms_per_iter": round(t, 3), "forward_ms": round(f, 3), "adjoint_ms": round(t - f, 3),
+            "adjoint_rays": int(st["rays"]), "grad_contributions": int(st["grad_taps"]),
+            "grad_atomics": int(st["grad_atomics"]), "autotune": tuned,
             "workload": "Cornell box 512x512, 16 spp, depth 4, 512^2 RGBA32F base-colour + metallic-roughness "
-                        "parameter textures; forward + L2 gradient + path-replay adjoint (the faster of: "
-                        "wavefront forward and an adjoint that traces the paths again, or megakernel forward "
-                        "recording its paths and an adjoint replaying them)"}
+                        "parameter textures and the point light's intensity; autotuned wavefront forward + L2 "
+                        "gradient + path-replay adjoint (each sample traced again, per-wave LDS sums before the "
+                        "float atomics)"}
 
 
 def dynamic_step(ctx, sc, buf, blases, steps):
@@ -339,7 +387,7 @@ def main():
         tuned["frame_ms"] = round(tuned["frame_ms"], 2)
     modes = {"megakernel": rt.MODE_MEGAKERNEL, "wavefront": rt.MODE_WAVEFRONT}
     wanted = list(modes) if args.mode == "best" else [args.mode]
-    results = {}
+    results, frames = {}, {}
     stream = torch.cuda.current_stream()
     for name in wanted:
         try:
@@ -364,6 +412,7 @@ def main():
         wall = time.perf_counter() - t0
         kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
         ms_per_step = wall * 1e3 / args.steps
+        frames[name] = acc.clone()  # the last timed frame (each frame resets the image)
         # stage split, after the timed frames: HIP events around every stage kernel on
         # the stream the kernels run on (one slot set and no finishing launch, so all
         # traversal runs in k_wf_trace launches)
@@ -377,6 +426,33 @@ def main():
         stage = {k: float(np.mean([s_[k] for s_ in stages])) for k in stages[0]}
         results[name] = {"st": st, "ms_per_step": ms_per_step, "kernel_ms": kern_ms, "stage": stage,
                          "mrays": st["rays"] / (ms_per_step / 1e3) / 1e6}
+    # marginal rate of the bounce rays: the same frame at depth 1 (camera rays and
+    # their shadow rays only) against the full depth
+    best0 = max(results, key=lambda k: results[k]["mrays"])
+    u1 = sc.ubo.copy()
+    u1["number_of_bounces"] = 1
+    st1 = rt.trace_rays(ctx, tlas, buf, u1, acc, W, H, mode=modes[best0], stats=True)
+    rt.trace_rays(ctx, tlas, buf, u1, acc, W, H, mode=modes[best0])
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(2, args.steps // 2)):
+        rt.trace_rays(ctx, tlas, buf, u1, acc, W, H, mode=modes[best0])
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms1 = (time.perf_counter() - t0) * 1e3 / max(2, args.steps // 2)
+    full = results[best0]
+    extra_rays = full["st"]["rays"] - st1["rays"]
+    extra_ms = full["ms_per_step"] - ms1
+    ray_split = {"depth1_ms": round(ms1, 3), "depth1_rays": int(st1["rays"]),
+                 "depth1_mrays_per_s": round(st1["rays"] / ms1 / 1e3, 1),
+                 "bounce_rays": int(extra_rays), "bounce_ms": round(extra_ms, 3),
+                 "bounce_mrays_per_s": round(extra_rays / extra_ms / 1e3, 1) if extra_ms > 0 else None}
+    parity = {}
+    if len(frames) == 2:
+        a = frames["wavefront"].double()[..., :3]
+        b = frames["megakernel"].double()[..., :3]
+        parity["wavefront_vs_megakernel_relRMSE"] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
     best = max(results, key=lambda k: results[k]["mrays"])
     r = results[best]
     st = r["st"]
@@ -415,6 +491,7 @@ def main():
                                                        "shadow_node_visits", "tri_tests", "shadow_tri_tests",
                                                        "shade_events")},
                    "modes": {k: round(v["mrays"], 2) for k, v in results.items()},
+                   "ray_split": ray_split,
                    "build_ms": round(min(build_ms), 2), "scene_gen_s": round(gen_s, 2), "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -429,9 +506,18 @@ def main():
     if not args.no_batch:
         line["config"]["batch"] = batch_benchmark(ctx, 2)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, threads)
+        host = host_cores()
+        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, host["threads"])
+        line["cpu_baseline"]["host"] = host
+        line["cpu_baseline"]["lavapipe"] = lavapipe_probe()
+        # the same leg checks the timed frame against the FP64 CPU restatement
+        parity.update(oracle_parity(sc, frames[best].cpu().numpy(), host["threads"]))
+    line["config"]["parity"] = parity
+    bad = [k for k, v in parity.items() if k.endswith("relRMSE") and not v <= 1e-3]
     print(json.dumps(line), flush=True)
+    if bad:
+        print(f"relRMSE above 1e-3: {bad}", file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

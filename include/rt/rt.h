@@ -206,6 +206,8 @@ typedef struct rt_trace_stats {
     uint64_t shadow_node_visits;     /* part of node_visits spent on shadow rays */
     uint64_t shadow_tri_tests;       /* part of tri_tests spent on shadow rays */
     uint64_t shadow_instance_visits; /* part of instance_visits spent on shadow rays */
+    uint64_t grad_taps;       /* adjoint: gradient contributions (texel or light) */
+    uint64_t grad_atomics;    /* adjoint: float atomics they became after the per-wave sums */
 } rt_trace_stats;
 
 /* ------------------------------------------------------------ batch */
@@ -230,24 +232,36 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
 
 /* ------------------------------------------------------------ adjoint */
 
-/* Path-replay adjoint of rt_trace_rays: accumulates into grad_textures[i] the
- * gradient of  sum_pixels dot(d_grad_accum, accum)  with respect to the texels of
- * texture i, for every texture i with a non-NULL entry.  Such a texture must be
- * RT_FMT_RGBA32F; its gradient buffer is a device array in the same layout
- * (width * height * 4 floats), added to (not cleared).  Gradients flow into the
- * base-colour texture (R, G, B) and the metallic-roughness texture (G roughness,
- * B metallic) of metallic-roughness materials; other uses of a texture do not
- * contribute.  Lobe choices, directions, sampling densities and Russian roulette
- * are held at the values of the forward trace with the same bindings and UBO
- * (the paths are replayed exactly), so the result is the derivative of the
- * image's estimate with the sampling detached.  d_grad_accum is the device
- * array dLoss/d(accum), width * height * 4 floats (alpha ignored).  At most 16
- * bounces, and ubo.debug == 1 (the legacy materials) is not differentiated
- * (RT_ERR_UNSUPPORTED for both).  Replaces the reference's
- * record_raytracing_commands for an optimisation loop; there is no adjoint in
- * the reference (crates/examples/gltf_viewer/src/main.rs:244-270 traces only). */
+/* Path-replay adjoint of rt_trace_rays: accumulates (adds, does not clear) the
+ * gradient of  sum_pixels dot(d_grad_accum, accum)  with respect to
+ *   - the texels of texture i, into grad_textures[i], for every texture with a
+ *     non-NULL entry: such a texture must be RT_FMT_RGBA32F, and its gradient
+ *     buffer is a device array in the same layout (width * height * 4 floats).
+ *     Gradients flow into the base-colour texture (R, G, B) and the
+ *     metallic-roughness texture (G roughness, B metallic) of metallic-roughness
+ *     materials; other uses of a texture do not contribute;
+ *   - the intensity of point light i, into d_grad_point_intensity[i] (device
+ *     array of b->num_plights floats, or NULL), through the NEE term
+ *     BRDF * light_weight * intensity * colour (RayTracing.rchit:347-348);
+ *   - the intensity of directional light i, into d_grad_directional_intensity[i]
+ *     (b->num_dlights floats, or NULL), through the miss shader's
+ *     -cos * colour * intensity (RayTracing.rmiss:18-26).
+ * Each sample is traced twice (path replay): once for its radiance and shadow
+ * results, once more to differentiate it vertex by vertex; nothing else of the
+ * path is stored.  Lobe choices, directions, sampling densities (the RIS light
+ * weight included) and Russian roulette are held at their forward values, so the
+ * result is the derivative of the image's estimate with the sampling detached, for
+ * the same bindings and UBO as the forward trace.  d_grad_accum is the device
+ * array dLoss/d(accum), width * height * 4 floats (alpha ignored); pixels whose
+ * gradient is zero are skipped.  At most 64 bounces, and ubo.debug == 1 (the legacy
+ * materials) is not differentiated (RT_ERR_UNSUPPORTED for both).  With `stats`,
+ * its ray and traversal counters count both passes, and grad_taps / grad_atomics
+ * count the gradient contributions and the float atomics they became.  There is no
+ * adjoint in the reference; it serves an optimisation loop around the trace that
+ * record_raytracing_commands issues (crates/examples/gltf_viewer/src/main.rs:244-270). */
 int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const float* d_grad_accum,
-                          uint32_t width, uint32_t height, float* const* grad_textures, rt_trace_stats* stats);
+                          uint32_t width, uint32_t height, float* const* grad_textures, float* d_grad_point_intensity,
+                          float* d_grad_directional_intensity, rt_trace_stats* stats);
 
 /* ------------------------------------------------------------ options */
 
@@ -299,16 +313,14 @@ enum {
      * afterwards: a subtree with at most this many triangles becomes one leaf.
      * Same hits.  Default 2. */
     RT_OPTION_BLAS_LEAF_SIZE = 10,
-    /* 1: rt_trace_rays (megakernel mode; the wavefront mode returns
-     * RT_ERR_UNSUPPORTED) also writes, per pixel, sample and bounce, what the
-     * adjoint's replay needs (hit, shadow result, radiance and throughput factors:
-     * 88 bytes) into a context-owned buffer (at most 32 GiB per frame).  A
-     * following rt_trace_rays_adjoint of the same TLAS, size and UBO samples,
-     * bounces, seed and frame then replays these paths instead of tracing them
-     * again; the next rt_trace_rays or rt_trace_rays_batch drops them (turning the
-     * option off does not).  rt_autotune does not record.  Same gradients.
-     * Default 0. */
-    RT_OPTION_RECORD_PATHS = 11
+    /* How BVHs built afterwards (BLASes, TLASes, flattened BVHs) turn the binary
+     * Morton-order radix tree into 4-wide nodes: 1 (default) the collapse of least
+     * surface-area-heuristic cost, solved exactly over the binary tree (leaves of at
+     * most RT_OPTION_BLAS_LEAF_SIZE triangles where the heuristic prefers them); 0
+     * the greedy collapse that opens the largest child.  Same hits.  The reference
+     * asks its driver for high-quality trees (PREFER_FAST_TRACE,
+     * crates/libs/vulkan/src/ray_tracing/acceleration_structure.rs:105-118). */
+    RT_OPTION_BVH_COLLAPSE = 12
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 

@@ -77,7 +77,9 @@ def lib():
         sig = {
             "orc_rand_sequence": (u, [vp, u, vp]), "orc_tea16": (u, [u, u]), "orc_pcg4d": (None, [vp, vp]),
             "orc_lcg_sequence": (u, [u, u, vp]), "orc_stream_b_seed": (u, [u, u]),
-            "orc_mat_build": (None, [vp]), "orc_scene_set_eval_textures": (None, [vp, vp, C.c_uint32]), "orc_eval_combined_brdf": (None, [dp, dp, dp, vp, dp]),
+            "orc_mat_build": (None, [vp]), "orc_scene_set_eval_textures": (None, [vp, vp, C.c_uint32]),
+            "orc_scene_set_eval_lights": (None, [vp, vp, u, vp, u]),
+            "orc_texel_footprints": (i, [vp, vp, u, u, vp, u, vp, vp, u, vp, C.c_uint64, vp, i]), "orc_eval_combined_brdf": (None, [dp, dp, dp, vp, dp]),
             "orc_brdf_probability": (None, [vp, dp, dp, dp]),
             "orc_sample_indirect": (i, [dp, dp, dp, dp, vp, u, dp, dp]),
             "orc_ggx_d": (d, [d, d]), "orc_smith_g2_height_correlated": (d, [d, d, d]),
@@ -199,6 +201,50 @@ class OracleScene:
             arr[k] = RtTexture(data.ctypes.data, w, h, t.format, t.wrap_s, t.wrap_t, t.mag_filter, t.min_filter)
         self._eval = (arr, keep)
         lib().orc_scene_set_eval_textures(self.handle, C.addressof(arr), len(textures))
+
+    def set_eval_lights(self, plight_intensity=None, dlight_intensity=None):
+        """Detached evaluation of light intensities (orc_scene_set_eval_lights): the NEE
+        and miss terms use these, the RIS choice and weights the scene's."""
+        keep = []
+        args = []
+        for a in (plight_intensity, dlight_intensity):
+            if a is None:
+                args += [None, 0]
+            else:
+                a = np.ascontiguousarray(a, np.float32)
+                keep.append(a)
+                args += [a.ctypes.data, len(a)]
+        lib().orc_scene_set_eval_lights(self.handle, *args)
+
+    def texel_footprints(self, xy, watch, ubo=None, width=None, height=None, threads=0):
+        """orc_texel_footprints: for each (texture, texel) of `watch`, the indices into
+        xy of the pixels whose detached evaluation reads it (eval textures must be set)."""
+        sc = self.scene
+        ubo = sc.ubo if ubo is None else ubo
+        w = sc.width if width is None else width
+        h = sc.height if height is None else height
+        xy = np.ascontiguousarray(xy, np.uint32).reshape(-1, 2)
+        uniq = sorted(set((int(t), int(x)) for t, x in watch))  # one entry per texel
+        wt = np.ascontiguousarray([t for t, _ in uniq], np.uint32)
+        wx = np.ascontiguousarray([x for _, x in uniq], np.uint32)
+        cap = max(1024, 64 * len(watch))
+        while True:
+            out = np.zeros(2 * cap, np.uint32)
+            n_out = C.c_uint64()
+            rc = lib().orc_texel_footprints(self.handle, ubo.ctypes.data, w, h, xy.ctypes.data, len(xy), wt.ctypes.data,
+                                            wx.ctypes.data, len(uniq), out.ctypes.data, cap, C.byref(n_out), threads)
+            if rc == -3:
+                cap = int(n_out.value)
+                continue
+            if rc != 0:
+                raise ValueError(f"orc_texel_footprints failed ({rc})")
+            break
+        pairs = out[:2 * int(n_out.value)].reshape(-1, 2)
+        res = [[] for _ in uniq]
+        for k, i in pairs:
+            res[int(k)].append(int(i))
+        by_key = {key: np.array(sorted(r), np.int64) for key, r in zip(uniq, res)}
+        return [by_key[(int(t), int(x))] for t, x in watch]
 
     def close(self):
         if getattr(self, "handle", None) and _lib is not None:

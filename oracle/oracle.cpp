@@ -19,6 +19,7 @@
 #include <limits>
 #include <memory>
 #include <vector>
+#include <unordered_map>
 #include <omp.h>
 
 namespace {
@@ -702,6 +703,12 @@ struct Scene {
     // with these textures while lobe choices, directions and densities keep using
     // d.textures.
     std::vector<rt_texture> eval_tex;
+    // Detached evaluation of light intensities: when not empty, the NEE term and the
+    // miss shader's directional term use these intensities while the RIS choice and
+    // weights keep using the scene's (d.plights / d.dlights).
+    std::vector<float> eval_pint, eval_dint;
+    // Texel watch list of orc_texel_footprints: (texture << 32 | texel) -> entry.
+    std::unordered_map<uint64_t, uint32_t> watch;
     rt_texture sky{};  // copy of *d.skybox
     bool has_sky_tex = false;
     std::vector<std::vector<double>> tri_pos;  // per BLAS: 9 doubles per triangle
@@ -817,13 +824,45 @@ template <class R> V4<R> fetch_tex(const Scene& s, int32_t index, V4<R> uv0and1,
     return {R(out[0]), R(out[1]), R(out[2]), R(out[3])};
 }
 
-template <class R> V4<R> fetch_tex_eval(const Scene& s, int32_t index, V4<R> uv0and1, int32_t coord) {
+// Texels a texture_sample at (u, v) reads (the same wrap and filter rules).
+inline int texture_taps(const rt_texture& t, double u, double v, uint32_t idx[4]) {
+    int W = int(t.width), H = int(t.height);
+    if (t.mag_filter == RT_FILTER_NEAREST) {
+        int x = wrap_coord(int(std::floor(u * W)), W, t.wrap_s);
+        int y = wrap_coord(int(std::floor(v * H)), H, t.wrap_t);
+        idx[0] = uint32_t(y) * t.width + uint32_t(x);
+        return 1;
+    }
+    double iu = std::floor(u * W - 0.5), iv = std::floor(v * H - 0.5);
+    int x0 = wrap_coord(int(iu), W, t.wrap_s), x1 = wrap_coord(int(iu) + 1, W, t.wrap_s);
+    int y0 = wrap_coord(int(iv), H, t.wrap_t), y1 = wrap_coord(int(iv) + 1, H, t.wrap_t);
+    idx[0] = uint32_t(y0) * t.width + uint32_t(x0);
+    idx[1] = uint32_t(y0) * t.width + uint32_t(x1);
+    idx[2] = uint32_t(y1) * t.width + uint32_t(x0);
+    idx[3] = uint32_t(y1) * t.width + uint32_t(x1);
+    return 4;
+}
+
+// Texture fetch of the detached evaluation; `touched` (or NULL) collects the
+// watched texels the fetch reads.
+template <class R> V4<R> fetch_tex_eval(const Scene& s, int32_t index, V4<R> uv0and1, int32_t coord,
+                                        std::vector<uint32_t>* touched) {
     R u = 0, v = 0;
     uint32_t c = uint32_t(coord);
     if (c == 0) { u = uv0and1.x; v = uv0and1.y; }
     else if (c == 1) { u = uv0and1.z; v = uv0and1.w; }
     double out[4] = {1, 1, 1, 1};
-    if (index >= 0 && size_t(index) < s.eval_tex.size()) texture_sample(s.eval_tex[size_t(index)], double(u), double(v), out);
+    if (index >= 0 && size_t(index) < s.eval_tex.size()) {
+        texture_sample(s.eval_tex[size_t(index)], double(u), double(v), out);
+        if (touched && !s.watch.empty()) {
+            uint32_t idx[4];
+            int n = texture_taps(s.eval_tex[size_t(index)], double(u), double(v), idx);
+            for (int k = 0; k < n; k++) {
+                auto it = s.watch.find((uint64_t(uint32_t(index)) << 32) | idx[k]);
+                if (it != s.watch.end()) touched->push_back(it->second);
+            }
+        }
+    }
     return {R(out[0]), R(out[1]), R(out[2]), R(out[3])};
 }
 
@@ -1008,6 +1047,16 @@ template <class R> struct Tracer {
     const rt_ubo& ubo;
     orc_stats* st;
     double last_t = 0.0;  // Ray.t after the pixel's last trace (read by the DISTANCE mapping, rgen:160)
+    std::vector<uint32_t>* touched = nullptr;  // watched texels read (orc_texel_footprints)
+
+    // intensity of a point / directional light in the NEE and miss terms
+    R plight_intensity(const rt_light* l) const {
+        size_t i = size_t(l - s.d.plights);
+        return i < s.eval_pint.size() ? R(s.eval_pint[i]) : R(l->intensity);
+    }
+    R dlight_intensity(uint32_t i) const {
+        return i < s.eval_dint.size() ? R(s.eval_dint[i]) : R(s.d.dlights[i].intensity);
+    }
 
     HitRec cast(V3<R> o, V3<R> d, R tmin, R tmax, uint32_t flags, U4* rng) {
         double od[3] = {double(o.x), double(o.y), double(o.z)};
@@ -1067,7 +1116,7 @@ template <class R> struct Tracer {
             for (uint32_t i = 0; i < s.d.num_dlights; i++) {
                 const rt_light& li = s.d.dlights[i];
                 R c = dot(normalize(load3<R>(li.transform)), rd);
-                if (c < R(0)) acc = acc + load3<R>(li.color) * (-c * R(li.intensity));
+                if (c < R(0)) acc = acc + load3<R>(li.color) * (-c * dlight_intensity(i));
             }
         }
         if (ubo.has_sky) {  // rmiss:27-33: light_acc += skyColor + light_acc
@@ -1219,10 +1268,12 @@ template <class R> struct Tracer {
         if (detached && mat.workflow != RT_WORKFLOW_SPECULAR_GLOSSINESS) {
             V4<R> c4 = vcol * load4<R>(mat.base_color);
             if (mat.base_color_texture.index >= 0)
-                c4 = c4 * fetch_tex_eval<R>(s, mat.base_color_texture.index, uv, mat.base_color_texture.tex_coord);
+                c4 = c4 * fetch_tex_eval<R>(s, mat.base_color_texture.index, uv, mat.base_color_texture.tex_coord,
+                                            touched);
             R me_metal = R(mat.metallic_factor), me_rough = R(mat.roughness_factor);
             if (mat.metallic_roughness_texture.index >= 0) {
-                V4<R> tx = fetch_tex_eval<R>(s, mat.metallic_roughness_texture.index, uv, mat.metallic_roughness_texture.tex_coord);
+                V4<R> tx = fetch_tex_eval<R>(s, mat.metallic_roughness_texture.index, uv,
+                                             mat.metallic_roughness_texture.tex_coord, touched);
                 me_rough = me_rough * tx.y;
                 me_metal = me_metal * tx.z;
             }
@@ -1241,7 +1292,7 @@ template <class R> struct Tracer {
             lv = normalize(lv);
             if (shadow_unoccluded(origin, lv, ldist, ray.rng)) {
                 V3<R> f = eval_combined_brdf(outward, lv, V, detached ? me : m);
-                ray.emittance = ray.emittance + f * (lw * R(light->intensity)) * load3<R>(light->color);
+                ray.emittance = ray.emittance + f * (lw * plight_intensity(light)) * load3<R>(light->color);
             }
         }
         uint32_t type;
@@ -1509,6 +1560,12 @@ void orc_scene_set_eval_textures(void* s, const rt_texture* tex, uint32_t n) {
     Scene* sc = (Scene*)s;
     sc->eval_tex.assign(tex, tex + n);
 }
+void orc_scene_set_eval_lights(void* s, const float* plight_intensity, uint32_t np, const float* dlight_intensity,
+                               uint32_t nd) {
+    Scene* sc = (Scene*)s;
+    sc->eval_pint.assign(plight_intensity, plight_intensity + (plight_intensity ? np : 0));
+    sc->eval_dint.assign(dlight_intensity, dlight_intensity + (dlight_intensity ? nd : 0));
+}
 
 static void fill_hit(const HitRec& h, const Scene& s, rt_hit* out, double* tuv) {
     std::memset(out, 0, sizeof(*out));
@@ -1586,6 +1643,48 @@ int orc_render_pixels(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, const
         }
     }
     if (stats) *stats = total;
+    return 0;
+}
+
+int orc_texel_footprints(void* sp, const rt_ubo* ubo, uint32_t W, uint32_t H, const uint32_t* xy, uint32_t n,
+                         const uint32_t* watch_tex, const uint32_t* watch_texel, uint32_t n_watch, uint32_t* out_pairs,
+                         uint64_t capacity, uint64_t* n_out, int threads) {
+    Scene& s = *(Scene*)sp;
+    if (ubo->has_sky && !s.has_sky_tex) return -2;
+    if (s.eval_tex.empty()) return -4;
+    for (uint32_t i = 0; i < n; i++)
+        if (xy[2 * i] >= W || xy[2 * i + 1] >= H) return -1;
+    if (threads > 0) omp_set_num_threads(threads);
+    s.watch.clear();
+    for (uint32_t k = 0; k < n_watch; k++) s.watch[(uint64_t(watch_tex[k]) << 32) | watch_texel[k]] = k;
+    std::vector<std::vector<uint32_t>> pairs;
+#pragma omp parallel
+    {
+        std::vector<uint32_t> local, touched;
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t i = 0; i < int64_t(n); i++) {
+            touched.clear();
+            Tracer<double> tr{s, *ubo, nullptr};
+            tr.touched = &touched;
+            (void)tr.pixel(xy[2 * i], xy[2 * i + 1], W, H);
+            std::sort(touched.begin(), touched.end());
+            touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+            for (uint32_t w : touched) {
+                local.push_back(w);
+                local.push_back(uint32_t(i));
+            }
+        }
+#pragma omp critical
+        pairs.push_back(std::move(local));
+    }
+    s.watch.clear();
+    uint64_t total = 0;
+    for (auto& v : pairs) total += v.size() / 2;
+    *n_out = total;
+    if (total > capacity) return -3;
+    uint64_t o = 0;
+    for (auto& v : pairs)
+        for (size_t k = 0; k < v.size(); k++) out_pairs[o++] = v[k];
     return 0;
 }
 

@@ -117,6 +117,12 @@ void orc_scene_destroy(void* scene);
  * lobe choices, directions, densities and Russian roulette keep the scene's
  * textures.  n = 0 switches it off.  The arrays must outlive the renders. */
 void orc_scene_set_eval_textures(void* scene, const rt_texture* tex, uint32_t n);
+/* Detached evaluation of light intensities: the NEE term (rchit:347-348) uses
+ * plight_intensity[i] for point light i, and the miss shader (rmiss:18-26)
+ * dlight_intensity[i] for directional light i, while the RIS choice and weights keep
+ * the scene's intensities.  NULL (or n = 0) switches either off.  Copied. */
+void orc_scene_set_eval_lights(void* scene, const float* plight_intensity, uint32_t np,
+                               const float* dlight_intensity, uint32_t nd);
 /* rays: n x {ox,oy,oz,tmin,dx,dy,dz,tmax} (float).  hits: n x rt_hit (t,u,v in
  * float for layout compatibility) plus n x 3 doubles of (t,u,v) when hits_tuv is
  * non-NULL.  brute != 0 tests every triangle of every instance.  Without
@@ -129,6 +135,15 @@ int orc_trace_rays(void* scene, const float* rays, uint32_t n, uint32_t ray_flag
 int orc_render_pixels(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
                       const uint32_t* xy, uint32_t n, int precision, int threads, double* out,
                       orc_stats* stats);
+/* Which pixels read which watched texels in the detached evaluation: renders the n
+ * pixels xy (FP64) with the eval textures set (else -4) and writes (entry, i) pairs,
+ * entry k being texel watch_texel[k] of texture watch_tex[k], for every pixel i whose
+ * eval texture fetches read it (bilinear taps included).  Pixels that read none of
+ * them do not depend on them.  Returns -3 when more than `capacity` pairs exist
+ * (*n_out is then the number needed). */
+int orc_texel_footprints(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height, const uint32_t* xy,
+                         uint32_t n, const uint32_t* watch_tex, const uint32_t* watch_texel, uint32_t n_watch,
+                         uint32_t* out_pairs, uint64_t capacity, uint64_t* n_out, int threads);
 /* Primary hits of sample 0 (camera ray and jitter as RayTracing.rgen computes them). */
 int orc_trace_primary(void* scene, const rt_ubo* ubo, uint32_t width, uint32_t height,
                       rt_hit* hits, double* hits_tuv, int threads);

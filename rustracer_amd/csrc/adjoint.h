@@ -5,15 +5,23 @@
 
 namespace rt {
 
-// Bounces one adjoint lane records per sample (paths are at most this deep).
-constexpr uint32_t kMaxAdjointBounces = 16;
+// Deepest path the adjoint replays: one bit per bounce records the forward's shadow
+// results in a 64-bit mask.
+constexpr uint32_t kMaxAdjointBounces = 64;
 
-// Accumulates into d_grads[i] (device buffers in the layout of RGBA32F texture i, or
-// NULL) the gradient of sum(grad_accum * accum) over the frame `fa` describes.
-// d_grads is a device array of num_textures pointers.  `rec` (or NULL) holds the
-// frame's path records from the forward trace; the adjoint then replays them
-// instead of tracing the paths again.
-int launch_adjoint(const FrameArgs& fa, const float* grad_accum, float* const* d_grads, uint32_t num_textures,
-                   const PathRecord* rec, hipStream_t st, bool any_hit);
+// Where the adjoint writes (all device memory; each entry may be NULL).
+struct AdjointOut {
+    float* const* grads;      // device array of num_textures pointers: gradient buffer of
+                              // RGBA32F texture i (its texel layout), or NULL
+    uint32_t num_textures;
+    float* grad_plight;       // d/d intensity of point light i (num_plights floats)
+    float* grad_dlight;       // d/d intensity of directional light i (num_dlights floats)
+};
+
+// Accumulates into `out` the gradient of sum(grad_accum * accum) over the frame `fa`
+// describes.  `table_bounces`: contributions of vertices at bounces below this are
+// summed per wave in LDS before their atomics (see adjoint.hip).
+int launch_adjoint(const FrameArgs& fa, const float* grad_accum, const AdjointOut& out, uint32_t table_bounces,
+                   hipStream_t st, bool any_hit, bool stats);
 
 }  // namespace rt

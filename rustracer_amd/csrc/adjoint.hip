@@ -1,27 +1,41 @@
 // adjoint.hip — path-replay adjoint of the trace: the gradient of
 // sum(grad_accum * accum) with respect to the texels of RGBA32F parameter textures
 // (base colour RGB; metallic-roughness G = roughness, B = metallic) of
-// metallic-roughness materials.
+// metallic-roughness materials, and to the intensities of the point and
+// directional lights.
 //
-// One lane per pixel, the megakernel's tiling.  Per sample:
-//   pass 1  the forward path (traversal, shading, shadow rays), recording per
-//           bounce the hit, the shadow result, the radiance the bounce adds per
-//           unit throughput (E_k) and the factor to the next throughput (M_k) --
-//           or these records read back from the forward trace that wrote them
-//           (RT_OPTION_RECORD_PATHS);
-//   back    R_k = E_k + M_k R_{k+1}: the radiance from bounce k on per unit of
-//           throughput at k;
-//   pass 2  the path replayed from the recorded hits (no traversal): at each
-//           vertex whose material reads a parameter texture, the Jacobians of the
-//           NEE term and of M_k with respect to (base colour, roughness, metallic)
-//           come from dual numbers (brdf_dual.h), and
-//             dL/dp = T_k [unoccluded dNEE/dp + dM_k/dp R_{k+1}]
-//           is scattered to the texels of the bilinear footprint with atomics.
-// Lobe choices, directions, sampling densities and Russian roulette are held at
-// their forward values (detached sampling), so the replay sees the forward paths
-// exactly.  Follows the raygen loop of
-// crates/examples/gltf_viewer/shaders/RayTracing.rgen (:85-128) and the closest-hit
-// shading of RayTracing.rchit (:136-416).
+// Path replay backpropagation (Vicini, Speierer & Jakob, SIGGRAPH 2021): nothing of
+// a path is stored between its two passes but its radiance and one bit per bounce.
+// One lane per pixel, the megakernel's tiling; per sample:
+//   pass 1  the forward path (traversal, shading, shadow rays) gives the sample's
+//           radiance L and the shadow results (one bit per bounce);
+//   pass 2  the same path traced and shaded again from the same RNG state.  At
+//           vertex k with throughput T_k, radiance added E_k and throughput factor
+//           M_k (T_{k+1} = T_k M_k), the radiance still to come is
+//             L_k = L - sum_{j<=k} T_j E_j = T_{k+1} R_{k+1},
+//           so the vertex's gradient  T_k dE_k/dp + T_k R_{k+1} dM_k/dp  is
+//             T_k dE_k/dp + L_k (dM_k/dp) / M_k      (per colour channel).
+// The Jacobians of the NEE term and of M_k with respect to (base colour, roughness,
+// metallic) come from dual numbers (brdf_dual.h); those with respect to a light's
+// intensity are the NEE and miss terms over that intensity.  A colour channel whose
+// M_k is exactly zero carries no radiance past vertex k, and its dM_k/dp term is
+// dropped (the known limit of path replay).
+//
+// Lobe choices, directions, sampling densities (the RIS light weight included) and
+// Russian roulette are held at their forward values (detached sampling), so the
+// replay sees the forward paths exactly.
+//
+// Scatter.  A vertex's contributions (texel x channel, or light) are summed per
+// wave in an LDS table keyed by the target address before any float atomic: the
+// camera-ray vertices of a wave's 8x8 pixels and 16 samples share a few hundred
+// texels, and every lane's NEE adds to the same few lights.  The table is drained
+// into the gradient buffers with one atomic per key when it is half full at the end
+// of a sample, and at the end of the kernel; a key that finds no free slot within
+// a few probes goes to the atomics directly.
+//
+// Follows the raygen loop of crates/examples/gltf_viewer/shaders/RayTracing.rgen
+// (:85-128), the closest-hit shading of RayTracing.rchit (:136-416, NEE :338-350)
+// and the miss shader RayTracing.rmiss (:14-45).
 #include "adjoint.h"
 #include "brdf_dual.h"
 #include "path.h"
@@ -31,20 +45,78 @@ namespace rt {
 namespace {
 
 constexpr int kAdjBlock = 128;
+constexpr int kAdjWaves = kAdjBlock / 64;
+constexpr int kSlots = 256;  // LDS table entries per wave
+constexpr int kProbe = 4;    // slots a key tries before its atomics go out directly
 
 struct AdjView {
     const float4* grad_accum;  // dL/d accum per pixel
-    float* const* grads;       // per texture: gradient buffer (RGBA32F layout) or NULL
-    uint32_t num_textures;
-    const PathRecord* rec;     // the forward's path records (pass 1 skipped), or NULL
+    AdjointOut out;
+    uint32_t table_bounces;    // vertices at bounces below this sum texel taps in LDS
 };
 
-struct BounceRec {
-    HitInfo h;
-    uint32_t wb;     // stream-B counter after the traversal (any-hit draws)
-    uint32_t flags;  // 1: hit, 2: shadow ray unoccluded
-    D3 e, m;         // E_k, M_k
-    double rr;       // Russian-roulette probability applied at this bounce (1: none)
+struct GradCounts {
+    uint32_t taps = 0, atomics = 0;
+};
+
+// A wave's table of pending gradient sums: key = the target's address (its first
+// float), value = the sums for the target's first three floats.
+struct WaveTable {
+    unsigned long long* keys;
+    float4* vals;
+    uint32_t* used;
+    uint32_t lane;
+
+    // every lane of the wave
+    __device__ void clear() {
+        for (int s = int(lane); s < kSlots; s += 64) {
+            keys[s] = 0ull;
+            vals[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (lane == 0) *used = 0u;
+        __threadfence_block();
+    }
+    // any lane: false when no slot was free within kProbe probes
+    __device__ bool add(float* a, float v0, float v1, float v2) {
+        const unsigned long long key = reinterpret_cast<unsigned long long>(a);
+        const uint32_t h = uint32_t(((key >> 2) * 0x9E3779B97F4A7C15ull) >> 56);
+        for (int i = 0; i < kProbe; i++) {
+            const uint32_t s = (h + uint32_t(i)) & uint32_t(kSlots - 1);
+            const unsigned long long old = atomicCAS(&keys[s], 0ull, key);
+            if (old == 0ull) atomicAdd(used, 1u);
+            if (old == 0ull || old == key) {
+                if (v0 != 0.f) atomicAdd(&vals[s].x, v0);
+                if (v1 != 0.f) atomicAdd(&vals[s].y, v1);
+                if (v2 != 0.f) atomicAdd(&vals[s].z, v2);
+                return true;
+            }
+        }
+        return false;
+    }
+    __device__ uint32_t occupancy() {
+        __threadfence_block();
+        return *used;
+    }
+    // every lane of the wave: one float atomic per non-zero sum, table cleared
+    __device__ uint32_t drain() {
+        __threadfence_block();
+        uint32_t n = 0;
+        for (int s = int(lane); s < kSlots; s += 64) {
+            const unsigned long long k = keys[s];
+            if (k == 0ull) continue;
+            float* a = reinterpret_cast<float*>(k);
+            const float4 v = vals[s];
+            if (v.x != 0.f) { atomicAdd(a, v.x); n++; }
+            if (v.y != 0.f) { atomicAdd(a + 1, v.y); n++; }
+            if (v.z != 0.f) { atomicAdd(a + 2, v.z); n++; }
+            keys[s] = 0ull;
+            vals[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __threadfence_block();
+        if (lane == 0) *used = 0u;
+        __threadfence_block();
+        return n;
+    }
 };
 
 // Bilinear footprint of sample_texture (same coordinates and wrap): texel indices
@@ -76,32 +148,59 @@ __device__ inline int tex_taps(const rt_texture& t, double u, double v, uint32_t
 
 // Gradient buffer of texture `index` if it is a parameter texture.
 __device__ inline float* param_grad(const SceneView& sv, const AdjView& av, int32_t index) {
-    if (index < 0 || uint32_t(index) >= av.num_textures || uint32_t(index) >= sv.num_textures) return nullptr;
+    if (index < 0 || uint32_t(index) >= av.out.num_textures || uint32_t(index) >= sv.num_textures) return nullptr;
     if (sv.textures[index].format != RT_FMT_RGBA32F) return nullptr;
-    return av.grads[index];
+    return av.out.grads[index];
 }
 
 __device__ inline D4 tex_coords(rt_texture_info ti, D4 uv) {
     return ti.tex_coord == 1 ? D4{uv.z, uv.w, 0.0, 0.0} : D4{uv.x, uv.y, 0.0, 0.0};
 }
 
-// Pass-2 observer: the vertex's contribution to the parameter gradient.
-struct GradTap {
+// Adds (v0, v1, v2) to a[0..2]: through the wave's table when `table`, else (or when
+// the table has no room) with float atomics.
+__device__ inline void scatter(WaveTable& tab, bool table, GradCounts& gc, float* a, double v0, double v1, double v2) {
+    gc.taps++;
+    const float f0 = float(v0), f1 = float(v1), f2 = float(v2);
+    if (table && tab.add(a, f0, f1, f2)) return;
+    if (f0 != 0.f) { atomicAdd(a, f0); gc.atomics++; }
+    if (f1 != 0.f) { atomicAdd(a + 1, f1); gc.atomics++; }
+    if (f2 != 0.f) { atomicAdd(a + 2, f2); gc.atomics++; }
+}
+
+// Pass-2 observer of one shaded vertex.
+struct ReplayTap {
     const FrameArgs* fa;
     const AdjView* av;
-    D3 gT;       // dL/d radiance at this bounce: pixel gradient * throughput T_k
-    D3 r_next;   // R_{k+1}
-    bool unocc;
-    double rr;
-    uint32_t atomics;
+    WaveTable* tab;
+    GradCounts* gc;
+    D3 g;        // dLoss/d radiance of the pixel
+    D3 thr;      // T_k
+    D3 l_rem;    // radiance of the sample not yet accounted for before this vertex
+    bool unocc;  // the forward's shadow ray reached the light
+    bool cont;   // the forward path went on past this vertex
+    bool table;  // texel taps go through the wave's table
 
-    __device__ void vertex(const ShadeState& ss, const Payload& ray) {
+    __device__ void vertex(const ShadeState& ss, const Payload& ray, D3 f, D3 nee) {
+        // L_k: the forward added thr * (emittance + NEE) when unoccluded (same expression)
+        const D3 rem = l_rem - (unocc ? thr * (ray.emittance + nee) : thr * ray.emittance);
+        const D3 gT = g * thr;
+        const SceneView& sv = fa->sv;
+        // point-light intensity: NEE = f * colour * (w * I) with the RIS weight w held
+        if (ss.has_light && unocc && av->out.grad_plight) {
+            const rt_light& l = sv.plights[ss.light_index];
+            const double v = (gT.x * f.x * double(l.color[0]) + gT.y * f.y * double(l.color[1]) +
+                              gT.z * f.z * double(l.color[2])) * ss.light_w;
+            scatter(*tab, true, *gc, av->out.grad_plight + ss.light_index, v, 0.0, 0.0);
+        }
         const rt_material_raw& mat = *ss.mat;
         if (mat.workflow == RT_WORKFLOW_SPECULAR_GLOSSINESS) return;
-        const SceneView& sv = fa->sv;
         float* gb = param_grad(sv, *av, mat.base_color_texture.index);
         float* gm = param_grad(sv, *av, mat.metallic_roughness_texture.index);
         if (!gb && !gm) return;
+        const bool nee_on = ss.has_light && unocc;
+        const bool m_on = cont && ray.need_scatter && (rem.x != 0.0 || rem.y != 0.0 || rem.z != 0.0);
+        if (!nee_on && !m_on) return;
         ad::MatIn in;
         in.base[0] = ss.m.base_color.x;
         in.base[1] = ss.m.base_color.y;
@@ -119,15 +218,19 @@ struct GradTap {
         const ad::V3 V = ad::v3(ss.V.x, ss.V.y, ss.V.z);
         double q[ad::kParams];
         for (int k = 0; k < ad::kParams; k++) q[k] = 0.0;
-        if (ss.has_light && unocc) {
-            ad::Dv3 f = ad::eval_brdf_d(N, ad::v3(ss.light_dir.x, ss.light_dir.y, ss.light_dir.z), V, md);
+        if (nee_on) {
+            ad::Dv3 fd = ad::eval_brdf_d(N, ad::v3(ss.light_dir.x, ss.light_dir.y, ss.light_dir.z), V, md);
             const double sx = gT.x * ss.light_scale.x, sy = gT.y * ss.light_scale.y, sz = gT.z * ss.light_scale.z;
-            for (int k = 0; k < ad::kParams; k++) q[k] += sx * f.x.d[k] + sy * f.y.d[k] + sz * f.z.d[k];
+            for (int k = 0; k < ad::kParams; k++) q[k] += sx * fd.x.d[k] + sy * fd.y.d[k] + sz * fd.z.d[k];
         }
-        if (ray.need_scatter && (r_next.x != 0.0 || r_next.y != 0.0 || r_next.z != 0.0)) {
+        if (m_on) {
+            // T_k R_{k+1} dM_k/dp = L_k (dw/dp) / w: M_k is the sampled weight w over
+            // factors that do not depend on p (Russian roulette, Beer attenuation)
             ad::Dv3 w = ad::sample_weight_d(int(ss.lobe), N, ad::v3(ray.scatter_dir.x, ray.scatter_dir.y, ray.scatter_dir.z),
                                             V, md, ss.p_lobe, ss.delta);
-            const double sx = gT.x * r_next.x / rr, sy = gT.y * r_next.y / rr, sz = gT.z * r_next.z / rr;
+            const double sx = w.x.v != 0.0 ? g.x * rem.x / w.x.v : 0.0;
+            const double sy = w.y.v != 0.0 ? g.y * rem.y / w.y.v : 0.0;
+            const double sz = w.z.v != 0.0 ? g.z * rem.z / w.z.v : 0.0;
             for (int k = 0; k < ad::kParams; k++) q[k] += sx * w.x.d[k] + sy * w.y.d[k] + sz * w.z.d[k];
         }
         uint32_t idx[4];
@@ -140,13 +243,8 @@ struct GradTap {
             const double s0 = q[ad::kBaseR] * ss.vcol.x * double(mat.base_color[0]);
             const double s1 = q[ad::kBaseG] * ss.vcol.y * double(mat.base_color[1]);
             const double s2 = q[ad::kBaseB] * ss.vcol.z * double(mat.base_color[2]);
-            for (int i = 0; i < n; i++) {
-                float* g = gb + 4 * size_t(idx[i]);
-                atomicAdd(g + 0, float(s0 * tw[i]));
-                atomicAdd(g + 1, float(s1 * tw[i]));
-                atomicAdd(g + 2, float(s2 * tw[i]));
-            }
-            atomics += 3u * uint32_t(n);
+            for (int i = 0; i < n; i++)
+                scatter(*tab, table, *gc, gb + 4 * size_t(idx[i]), s0 * tw[i], s1 * tw[i], s2 * tw[i]);
         }
         if (gm) {
             // roughness = factor * texel.g, metallic = factor * texel.b (rchit:204-213)
@@ -155,109 +253,132 @@ struct GradTap {
             int n = tex_taps(t, c.x, c.y, idx, tw);
             const double sr = q[ad::kRough] * double(mat.roughness_factor);
             const double sm = q[ad::kMetal] * double(mat.metallic_factor);
-            for (int i = 0; i < n; i++) {
-                float* g = gm + 4 * size_t(idx[i]);
-                atomicAdd(g + 1, float(sr * tw[i]));
-                atomicAdd(g + 2, float(sm * tw[i]));
-            }
-            atomics += 2u * uint32_t(n);
+            for (int i = 0; i < n; i++)
+                scatter(*tab, table, *gc, gm + 4 * size_t(idx[i]), 0.0, sr * tw[i], sm * tw[i]);
         }
     }
 };
 
-template <bool kAnyHit>
+// Directional-light intensities at a miss after the first bounce (rmiss:18-26): the
+// emittance holds max(0, -cos_i) colour_i I_i, doubled under a sky (rmiss:27-33).
+__device__ inline void miss_light_grads(const SceneView& sv, const AdjView& av, D3 dir, D3 gT, WaveTable& tab,
+                                        GradCounts& gc) {
+    const D3 rd = dnormalize(dir);
+    const double k = sv.has_sky ? 2.0 : 1.0;
+    for (uint32_t i = 0; i < sv.num_dlights; i++) {
+        const rt_light& li = sv.dlights[i];
+        const double c = ddot(dnormalize(ld3(li.transform)), rd);
+        if (c >= 0.0) continue;
+        const double v = -c * k * (gT.x * double(li.color[0]) + gT.y * double(li.color[1]) + gT.z * double(li.color[2]));
+        scatter(tab, true, gc, av.out.grad_dlight + i, v, 0.0, 0.0);
+    }
+}
+
+template <bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av) {
     __shared__ uint32_t s_stack[kStackDepth * kAdjBlock];
+    __shared__ unsigned long long s_keys[kAdjWaves][kSlots];
+    __shared__ float4 s_vals[kAdjWaves][kSlots];
+    __shared__ uint32_t s_used[kAdjWaves];
     const uint32_t tiles_x = (fa.width + 15) / 16;
     const uint32_t bid = blockIdx.x;
     const uint32_t tx = bid % tiles_x, ty = bid / tiles_x;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t x = tx * 16 + wave * 8 + (lane & 7);
     const uint32_t y = ty * 8 + (lane >> 3);
+    WaveTable tab{s_keys[wave], s_vals[wave], &s_used[wave], lane};
+    tab.clear();
     PathCounters pc;
-    if (x >= fa.width || y >= fa.height) return;
-    const float4 g4 = av.grad_accum[size_t(y) * fa.width + x];
-    const D3 g = d3(g4.x, g4.y, g4.z);
+    GradCounts gc;
+    // every lane stays to the end (the table is drained by the whole wave); lanes
+    // outside the image or with a zero pixel gradient do no work
+    D3 g = d3(0.0);
+    if (x < fa.width && y < fa.height) {
+        const float4 g4 = av.grad_accum[size_t(y) * fa.width + x];
+        g = d3(g4.x, g4.y, g4.z);
+    }
+    const bool live = (g.x != 0.0 || g.y != 0.0 || g.z != 0.0) && fa.bounces != 0u;
     uint32_t* stack = s_stack + threadIdx.x;
     const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
-    BounceRec rec[kMaxAdjointBounces];
-    D3 R[kMaxAdjointBounces + 1];
-    uint32_t atomics = 0;
+    const bool dl_grads = av.out.grad_dlight != nullptr && fa.sv.num_dlights != 0u;
     PathState p;
     init_path(fa, x, y, p);
-    for (; p.sample < fa.samples; p.sample++) {
-        begin_sample(fa, x, y, p);
-        if (fa.bounces == 0u) continue;
-        const PathState p0 = p;
-        uint32_t nb = 0;
-        if (av.rec) {
-            // pass 1 from the records the forward trace wrote
-            const PathRecord* pr = av.rec + ((size_t(y) * fa.width + x) * fa.samples + p.sample) * fa.bounces;
+    for (uint32_t s = 0; s < fa.samples; s++) {
+        if (live) {
+            p.sample = s;
+            begin_sample(fa, x, y, p);
+            const PathState p0 = p;
+            // pass 1: the sample's radiance and shadow results
+            D3 L = d3(0.0);
+            uint64_t vis = 0;
+            uint32_t nb = 0;
             for (;;) {
-                const PathRecord& q = pr[nb];
-                BounceRec& br = rec[nb];
-                br.h = HitInfo{q.t, q.u, q.v, q.inst, q.prim, q.rec};
-                br.wb = q.wb;
-                br.flags = q.flags & (kRecHit | kRecUnoccluded);
-                br.e = d3(q.e[0], q.e[1], q.e[2]);
-                br.m = d3(q.m[0], q.m[1], q.m[2]);
-                br.rr = q.rr;
+                if (kStats) { if (p.bounce == 0u) pc.primary++; else pc.extension++; }
+                Rng4 rb{x, y, fa.seed_b, p.wb};
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
+                HitInfo h;
+                bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(p.o), to_f3(p.d), RayFromF64{p.o, p.d},
+                                                        float(kTMin), float(p.tfar), flags, ah, stack, kAdjBlock, h, pc.tr);
+                p.wb = rb.w;
+                BounceOut r;
+                shade_bounce<false>(fa, x, y, p, found, h, r, pc);
+                const bool unocc = r.shadow && shadow_unoccluded<kAnyHit, kStats>(fa, x, y, r, stack, kAdjBlock, pc);
+                L = L + (unocc ? r.pend_with : r.pend_without);
+                if (unocc) vis |= 1ull << nb;
                 nb++;
-                if ((q.flags & kRecLast) || nb == kMaxAdjointBounces || nb == fa.bounces) break;
+                if (!r.cont) break;
+            }
+            // pass 2: the same path again, differentiated vertex by vertex
+            PathState q = p0;
+            ReplayTap tap{&fa, &av, &tab, &gc, g, d3(0.0), L, false, false, false};
+            for (uint32_t k = 0; k < nb; k++) {
+                if (kStats) { if (k == 0u) pc.primary++; else pc.extension++; }
+                Rng4 rb{x, y, fa.seed_b, q.wb};
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
+                HitInfo h;
+                bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(q.o), to_f3(q.d), RayFromF64{q.o, q.d},
+                                                        float(kTMin), float(q.tfar), flags, ah, stack, kAdjBlock, h, pc.tr);
+                q.wb = rb.w;
+                tap.thr = q.thr;
+                tap.unocc = ((vis >> k) & 1ull) != 0ull;
+                tap.cont = k + 1u < nb;
+                tap.table = k < av.table_bounces;
+                const D3 dir = q.d;
+                BounceOut r;
+                shade_bounce<false>(fa, x, y, q, found, h, r, pc, tap);
+                if (!found && k > 0u && dl_grads) miss_light_grads(fa.sv, av, dir, g * tap.thr, tab, gc);
+                tap.l_rem = tap.l_rem - (tap.unocc ? r.pend_with : r.pend_without);
             }
         }
-        // pass 1 without records: the forward path traced again
-        while (!av.rec) {
-            Rng4 rb{x, y, fa.seed_b, p.wb};
-            AlphaAnyHit ah{&fa.sv, &fa.tlas, &rb};
-            HitInfo h;
-            bool found = trace_ray<kAnyHit, false>(fa.tlas, to_f3(p.o), to_f3(p.d), RayFromF64{p.o, p.d}, float(kTMin),
-                                                   float(p.tfar), flags, ah, stack, kAdjBlock, h, pc.tr);
-            p.wb = rb.w;
-            BounceRec& br = rec[nb];
-            br.h = h;
-            br.wb = p.wb;
-            BounceOut r;
-            shade_bounce<false>(fa, x, y, p, found, h, r, pc);
-            bool unocc = r.shadow && shadow_unoccluded<kAnyHit, false>(fa, x, y, r, stack, kAdjBlock, pc);
-            br.flags = (found ? 1u : 0u) | (unocc ? 2u : 0u);
-            br.e = unocc ? r.emit + r.nee : r.emit;
-            br.m = r.mult;
-            br.rr = r.rr;
-            nb++;
-            if (!r.cont || nb == kMaxAdjointBounces) break;
-        }
-        // radiance from each bounce on, per unit throughput there
-        R[nb] = d3(0.0);
-        for (int k = int(nb) - 1; k >= 0; k--) R[k] = rec[k].e + rec[k].m * R[k + 1];
-        // pass 2: replay the shading from the recorded hits
-        p = p0;
-        GradTap tap{&fa, &av, d3(0.0), d3(0.0), false, 1.0, 0u};
-        for (uint32_t k = 0; k < nb; k++) {
-            const BounceRec& br = rec[k];
-            p.wb = br.wb;
-            tap.gT = g * p.thr;
-            tap.r_next = R[k + 1];
-            tap.unocc = (br.flags & 2u) != 0u;
-            tap.rr = br.rr;
-            BounceOut r;
-            shade_bounce<false>(fa, x, y, p, (br.flags & 1u) != 0u, br.h, r, pc, tap);
-            if (!r.cont) break;
-        }
-        atomics += tap.atomics;
+        // a converged point: drain the table before it fills
+        if (tab.occupancy() > uint32_t(kSlots / 2)) gc.atomics += tab.drain();
     }
-    (void)atomics;
+    gc.atomics += tab.drain();
+    if (kStats) {
+        flush_counters(fa.counters, pc);
+        const unsigned long long t = wave_sum<unsigned long long>(gc.taps);
+        const unsigned long long a = wave_sum<unsigned long long>(gc.atomics);
+        if (lane == 0) {
+            if (t) atomicAdd(&fa.counters[kCntGradTaps], t);
+            if (a) atomicAdd(&fa.counters[kCntGradAtomics], a);
+        }
+    }
 }
 
 }  // namespace
 
-int launch_adjoint(const FrameArgs& fa, const float* grad_accum, float* const* d_grads, uint32_t num_textures,
-                   const PathRecord* rec, hipStream_t st, bool any_hit) {
+int launch_adjoint(const FrameArgs& fa, const float* grad_accum, const AdjointOut& out, uint32_t table_bounces,
+                   hipStream_t st, bool any_hit, bool stats) {
     uint32_t tiles = ((fa.width + 15) / 16) * ((fa.height + 7) / 8);
     if (tiles == 0) return 0;
-    AdjView av{reinterpret_cast<const float4*>(grad_accum), d_grads, num_textures, rec};
-    if (any_hit) hipLaunchKernelGGL((k_adjoint<true>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
-    else hipLaunchKernelGGL((k_adjoint<false>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+    AdjView av{reinterpret_cast<const float4*>(grad_accum), out, table_bounces};
+    if (stats) {
+        if (any_hit) hipLaunchKernelGGL((k_adjoint<true, true>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+        else hipLaunchKernelGGL((k_adjoint<false, true>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+    } else {
+        if (any_hit) hipLaunchKernelGGL((k_adjoint<true, false>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+        else hipLaunchKernelGGL((k_adjoint<false, false>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+    }
     return int(hipGetLastError());
 }
 

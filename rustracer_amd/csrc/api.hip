@@ -26,7 +26,10 @@ namespace {
 
 constexpr uint32_t kAutotuneSamples = 8;  // samples per pixel of the autotuner's probe frames
 constexpr double kAutotuneMargin = 0.015;  // relative gain a probe needs to replace the incumbent
-constexpr uint64_t kMaxRecordBytes = 32ull << 30;  // path records of one frame (RT_OPTION_RECORD_PATHS)
+// Adjoint: vertices at bounces below this sum their texel taps per wave in LDS
+// before the atomics (camera-ray vertices of neighbouring pixels and samples share
+// texels; deeper vertices scatter over the scene).
+constexpr uint32_t kAdjointTableBounces = 1;
 constexpr uint32_t kBlasLeafMax = 2;  // default triangles per leaf (RT_OPTION_BLAS_LEAF_SIZE)
 constexpr uint32_t kTlasLeafMax = 1;
 
@@ -117,17 +120,10 @@ struct rt_ctx {
     bool out_bgra = false;   // RT_OPTION_OUTPUT_BGRA
     int streams = 2;         // RT_OPTION_WAVEFRONT_STREAMS
     uint32_t leaf_max = kBlasLeafMax;  // RT_OPTION_BLAS_LEAF_SIZE
+    uint32_t collapse = kCollapseSah;  // RT_OPTION_BVH_COLLAPSE
     uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
-    bool record = false;             // RT_OPTION_RECORD_PATHS
-    PathRecord* d_rec = nullptr;     // path records of the last recording trace
-    size_t rec_cap = 0;              // records d_rec holds
-    struct {
-        bool valid = false;
-        rt_tlas tlas = 0;
-        uint32_t width = 0, height = 0, samples = 0, bounces = 0, seed = 0, frame = 0;
-    } rec_frame;  // the frame the records belong to
 };
 
 namespace {
@@ -361,12 +357,21 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
     if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
     if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
-    if (ctx->d_rec) (void)hipFree(ctx->d_rec);
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_textures) (void)hipFree(ctx->d_textures);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     delete ctx;
+}
+
+// The status and message of a set of kErr* flags raised by kernels (RT_OK for none).
+static int flags_error(rt_ctx* ctx, uint32_t flags) {
+    if (flags & kErrStackOverflow)
+        return fail(ctx, RT_ERR_STACK_OVERFLOW, "a traversal exceeded its stack restart budget; results are incomplete");
+    if (flags & kErrBadSkin)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a vertex's skin index or joint is out of range (left unskinned)");
+    if (flags) return fail(ctx, RT_ERR_HIP, "a kernel raised an unknown error flag");
+    return RT_OK;
 }
 
 int rt_sync(rt_ctx* ctx) {
@@ -377,10 +382,7 @@ int rt_sync(rt_ctx* ctx) {
     RT_HIP(ctx, hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost), "read error flags");
     if (flags) {
         RT_HIP(ctx, hipMemset(ctx->d_err, 0, sizeof(uint32_t)), "reset error flags");
-        if (flags & kErrStackOverflow)
-            return fail(ctx, RT_ERR_STACK_OVERFLOW, "a traversal exceeded its stack restart budget; results are incomplete");
-        if (flags & kErrBadSkin)
-            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a vertex's skin index or joint is out of range (left unskinned)");
+        return flags_error(ctx, flags);
     }
     return RT_OK;
 }
@@ -419,7 +421,7 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     LbvhResult res;
     std::string err;
     rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), n, (const uint32_t*)d_seg_start.get(),
-                    seg_start, ctx->leaf_max, &res, &err);
+                    seg_start, ctx->leaf_max, ctx->collapse, &res, &err);
     if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "BLAS build: " + err);
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
     auto pool = std::make_shared<BlasPool>();
@@ -649,7 +651,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
     LbvhResult res;
     std::string err;
     rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), nt, (const uint32_t*)d_seg.get(),
-                    seg_start, kTlasLeafMax, &res, &err);
+                    seg_start, kTlasLeafMax, ctx->collapse, &res, &err);
     if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "TLAS build: " + err);
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
     te.nodes.reset(res.nodes);
@@ -700,7 +702,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
         if (rc) return hip_fail(ctx, hipError_t(rc), "gather_flat");
         LbvhResult fres;
         rc = lbvh_build(st, (const float4*)d_flo.get(), (const float4*)d_fhi.get(), n, (const uint32_t*)d_fseg.get(),
-                        fseg, ctx->leaf_max, &fres, &err);
+                        fseg, ctx->leaf_max, ctx->collapse, &fres, &err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "flattened BVH build: " + err);
         DevPtr fperm(fres.perm), fseg_bounds(fres.seg_bounds);
         te.flat_nodes.reset(fres.nodes);
@@ -812,6 +814,8 @@ static int read_stats(rt_ctx* ctx, rt_trace_stats* stats) {
     stats->shadow_node_visits = c[kCntShadowNodes];
     stats->shadow_tri_tests = c[kCntShadowTris];
     stats->shadow_instance_visits = c[kCntShadowInstances];
+    stats->grad_taps = c[kCntGradTaps];
+    stats->grad_atomics = c[kCntGradAtomics];
     if (c[kCntBadIndex])
         return fail(ctx, RT_ERR_INVALID_ARGUMENT, "a hit referenced an out-of-range prim info, material, index or vertex");
     return RT_OK;
@@ -843,27 +847,9 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     fa.o.out = reinterpret_cast<uint32_t*>(out_rgba8);
     fa.o.bgra = ctx->out_bgra ? 1u : 0u;
     const bool any_hit = !fa.fully_opaque;
-    // the counting kernels also carry the debug == 1 materials and the path records
-    const bool counting = stats != nullptr || fa.debug != 0u || ctx->record;
+    // the counting kernels also carry the debug == 1 materials
+    const bool counting = stats != nullptr || fa.debug != 0u;
     hipStream_t st = ctx->stream;
-    ctx->rec_frame.valid = false;
-    if (ctx->record) {
-        if (mode != RT_MODE_MEGAKERNEL)
-            return fail(ctx, RT_ERR_UNSUPPORTED, "RT_OPTION_RECORD_PATHS records in the megakernel mode");
-        const uint64_t n = uint64_t(width) * height * b->ubo->number_of_samples * b->ubo->number_of_bounces;
-        if (n * sizeof(PathRecord) > kMaxRecordBytes)
-            return fail(ctx, RT_ERR_INVALID_ARGUMENT, "the path records of this frame exceed 32 GiB");
-        if (n > ctx->rec_cap) {
-            if (ctx->d_rec) (void)hipFree(ctx->d_rec);
-            ctx->d_rec = nullptr;
-            ctx->rec_cap = 0;
-            hipError_t e;
-            ctx->d_rec = dev_alloc<PathRecord>(size_t(n), &e);
-            if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc path records");
-            ctx->rec_cap = size_t(n);
-        }
-        fa.rec_out = ctx->d_rec;
-    }
     if (stats) {
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
         RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
@@ -887,17 +873,6 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "wavefront: " + err);
     }
     if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
-    if (ctx->record) {
-        const rt_ubo& u = *b->ubo;
-        ctx->rec_frame.tlas = tlas;
-        ctx->rec_frame.width = width;
-        ctx->rec_frame.height = height;
-        ctx->rec_frame.samples = u.number_of_samples;
-        ctx->rec_frame.bounces = u.number_of_bounces;
-        ctx->rec_frame.seed = u.random_seed;
-        ctx->rec_frame.frame = u.frame_count;
-        ctx->rec_frame.valid = true;
-    }
     if (stats) return read_stats(ctx, stats);
     return RT_OK;
 }
@@ -907,7 +882,6 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (count && !items) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "items is NULL");
     if (mode != RT_MODE_MEGAKERNEL && mode != RT_MODE_WAVEFRONT) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "unknown mode");
-    ctx->rec_frame.valid = false;  // batches do not record paths
     size_t ntex = 0;
     for (uint32_t i = 0; i < count; i++) {
         const rt_batch_item& it = items[i];
@@ -986,7 +960,8 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
 }
 
 int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const float* d_grad_accum,
-                          uint32_t width, uint32_t height, float* const* grad_textures, rt_trace_stats* stats) {
+                          uint32_t width, uint32_t height, float* const* grad_textures, float* d_grad_point_intensity,
+                          float* d_grad_directional_intensity, rt_trace_stats* stats) {
     if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
     int rc = validate_bindings(ctx, b);
     if (rc) return rc;
@@ -996,7 +971,7 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     auto it = ctx->tlas.find(tlas);
     if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
     if (b->ubo->number_of_bounces > kMaxAdjointBounces)
-        return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint replays at most 16 bounces");
+        return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint replays at most 64 bounces");
     if (b->ubo->debug == 1u)
         return fail(ctx, RT_ERR_UNSUPPORTED, "the adjoint differentiates the PBR materials, not debug == 1");
     const uint32_t nt = b->num_textures;
@@ -1005,9 +980,9 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
         if (grad_textures[i] && b->textures[i].format != RT_FMT_RGBA32F)
             return fail(ctx, RT_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has a gradient buffer but is not RGBA32F");
     DeviceGuard g(ctx->device);
+    hipStream_t st = ctx->stream;
     rc = upload_textures(ctx, b);
     if (rc) return rc;
-    hipStream_t st = ctx->stream;
     if (nt > ctx->grad_cap) {
         if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
         ctx->d_grad_ptrs = nullptr;
@@ -1018,30 +993,21 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     if (nt) {
         // the pointer table must stay valid until the kernel has read it
         RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
-        RT_HIP(ctx, hipMemcpy(ctx->d_grad_ptrs, grad_textures, sizeof(float*) * nt, hipMemcpyHostToDevice), "copy gradient pointers");
+        RT_HIP(ctx, hipMemcpy(ctx->d_grad_ptrs, grad_textures, sizeof(float*) * nt, hipMemcpyHostToDevice),
+               "copy gradient pointers");
     }
+    float** d_ptrs = ctx->d_grad_ptrs;
     FrameArgs fa;
     make_frame(ctx, it->second, b, nullptr, width, height, &fa);
     if (stats) {
         RT_HIP(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * kNumCounters, st), "memset");
         RT_HIP(ctx, hipEventRecord(ctx->ev0, st), "hipEventRecord");
     }
-    // the last recording trace's paths, when it traced this frame
-    const auto& rf = ctx->rec_frame;
-    const rt_ubo& u = *b->ubo;
-    const bool use_rec = rf.valid && rf.tlas == tlas && rf.width == width && rf.height == height &&
-                         rf.samples == u.number_of_samples && rf.bounces == u.number_of_bounces &&
-                         rf.seed == u.random_seed && rf.frame == u.frame_count;
-    rc = launch_adjoint(fa, d_grad_accum, ctx->d_grad_ptrs, nt, use_rec ? ctx->d_rec : nullptr, st, !fa.fully_opaque);
+    AdjointOut out{d_ptrs, nt, b->num_plights ? d_grad_point_intensity : nullptr,
+                   b->num_dlights ? d_grad_directional_intensity : nullptr};
+    rc = launch_adjoint(fa, d_grad_accum, out, kAdjointTableBounces, st, !fa.fully_opaque, stats != nullptr);
     if (rc) return hip_fail(ctx, hipError_t(rc), "adjoint launch");
-    if (stats) {
-        RT_HIP(ctx, hipEventRecord(ctx->ev1, st), "hipEventRecord");
-        RT_HIP(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
-        float ms = 0.f;
-        RT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
-        std::memset(stats, 0, sizeof(*stats));
-        stats->kernel_ms = ms;
-    }
+    if (stats) return read_stats(ctx, stats);
     return RT_OK;
 }
 
@@ -1065,7 +1031,10 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         uint32_t flags = 0;
         RT_HIP(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         RT_HIP(ctx, hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost), "read error flags");
-        if (flags) return fail(ctx, RT_ERR_STACK_OVERFLOW, "an earlier traversal exceeded its stack restart budget");
+        if (flags) {
+            RT_HIP(ctx, hipMemset(ctx->d_err, 0, sizeof(uint32_t)), "reset error flags");
+            return flags_error(ctx, flags);  // raised by earlier work: reported here, as rt_sync would
+        }
     }
     const bool timing = ctx->stage_timing;
     ctx->stage_timing = false;
@@ -1074,8 +1043,6 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
     const int keep_streams = ctx->streams;
     const uint32_t keep_finish = ctx->finish;
-    const bool keep_record = ctx->record;
-    ctx->record = false;  // the probe frames are wavefront traces
     // one probe: the faster of two timed frames of the current settings (after a
     // warm-up frame), or < 0 when a traversal gave up
     auto probe = [&](double& ms_out) -> int {
@@ -1181,7 +1148,6 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
         ctx->variant = chosen;
     }
     ctx->stage_timing = timing;
-    ctx->record = keep_record;
     if (rc) {
         ctx->variant = keep_variant;
         ctx->use_graph = keep_graph;
@@ -1227,9 +1193,9 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
         ctx->use_graph = value != 0;
         return RT_OK;
     }
-    if (option == RT_OPTION_RECORD_PATHS) {
-        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_RECORD_PATHS takes 0 or 1");
-        ctx->record = value != 0;  // the last trace's records stay until the next trace
+    if (option == RT_OPTION_BVH_COLLAPSE) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_BVH_COLLAPSE takes 0 or 1");
+        ctx->collapse = uint32_t(value);
         return RT_OK;
     }
     if (option == RT_OPTION_BLAS_LEAF_SIZE) {

@@ -37,12 +37,19 @@ struct LbvhResult {
     std::vector<uint32_t> level_start;  // nodes of tree level L: [level_start[L], level_start[L + 1])
 };
 
+// How the binary radix tree becomes 4-wide nodes.
+enum : uint32_t {
+    kCollapseArea = 0,  // greedy: open the largest-surface-area child until four are gathered
+    kCollapseSah = 1    // least surface-area-heuristic cost (dynamic program over the binary tree)
+};
+
 // Builds the wide BVH of `n` primitives with boxes (lo[i], hi[i]) split into
 // `nseg` segments (d_seg_start has nseg + 1 entries, on device and host).
-// Segment s's root is node s.  Returns a hipError_t-compatible code.
+// Segment s's root is node s.  Leaves hold at most leaf_max primitives.  Returns a
+// hipError_t-compatible code.
 int lbvh_build(hipStream_t stream, const float4* d_lo, const float4* d_hi, uint32_t n,
                const uint32_t* d_seg_start, const std::vector<uint32_t>& h_seg_start,
-               uint32_t leaf_max, LbvhResult* out, std::string* err);
+               uint32_t leaf_max, uint32_t collapse, LbvhResult* out, std::string* err);
 
 // Triangle preparation for a BLAS batch: reads the caller's index and vertex
 // buffers, writes triangle records (input order) and boxes.  Sets *d_bad to 1 if an

@@ -7,9 +7,18 @@
 //      LDS-staged stable scatter that ranks keys with wave ballots
 //   4. Karras (HPG 2012) radix-tree emission, one thread per internal node
 //   5. bottom-up box fit with acquire/release counters
-//   6. top-down collapse into 4-wide nodes, one kernel launch per level: each
-//      task opens the largest-surface-area child that is too big for a leaf until
-//      four children are gathered
+//   6. top-down collapse into 4-wide nodes, one kernel launch per level.  With the
+//      SAH collapse (default), the box-fit pass also solves, bottom-up, for every
+//      binary node n and every number of slots i = 1..4 it may take in its parent's
+//      wide node, the least surface-area-heuristic cost of its subtree
+//        C(n, 1) = min(leaf: c_tri |n| A(n) if |n| <= leaf size,
+//                      wide node: c_node A(n) + min_{i=2..4} D(n, i))
+//        D(n, i) = min_{k=1..i-1} C(left, k) + C(right, i - k)
+//        C(n, i) = min(C(n, i - 1), D(n, i))
+//      (the dynamic program of Ylitie, Karras & Laine, HPG 2017, for 4-wide nodes),
+//      and the collapse follows the decisions; the greedy form opens the
+//      largest-surface-area child that is too big for a leaf until four children
+//      are gathered
 #include <string>
 
 #include "lbvh.h"
@@ -486,23 +495,93 @@ __device__ inline void load_box(uint32_t ref, const float4* lo, const float4* hi
     }
 }
 
+__device__ inline float half_area(float3 l, float3 h) {
+    float3 e = h - l;
+    return e.x * e.y + e.y * e.z + e.z * e.x;
+}
+
+// Surface-area-heuristic costs of the 4-wide collapse: a wide node's visit against
+// a triangle test, per unit of (half) surface area.
+constexpr float kSahNodeCost = 1.0f;
+constexpr float kSahTriCost = 0.6f;
+
+// Decision word of one binary node (SAH collapse):
+//   bit 0       C(n, 1) is a wide node (else a leaf)
+//   bits 2i..   for i = 2..4 (2 bits each, at 2, 4, 6): k of D(n, i), 0 = C(n, i - 1)
+//   bits 8..9   i* - 2 of the node's own wide node, bits 10..11 its k*
+__device__ inline uint32_t dec_slots(uint32_t d, int i) { return (d >> (2 * (i - 1))) & 3u; }
+
+struct SahArgs {
+    float4* cost;     // per binary node: C(n, 1..4)
+    uint32_t* dec;    // per binary node: decision word
+    const uint32_t* first;
+    const uint32_t* last;
+    uint32_t leaf_max;
+};
+
+// C(r, 1..4) of a child reference: a primitive costs one triangle test wherever it goes
+__device__ inline float4 child_cost(uint32_t r, const SahArgs& sa, float3 l, float3 h) {
+    if (r & kLeafBit) {
+        float c = kSahTriCost * half_area(l, h);
+        return make_float4(c, c, c, c);
+    }
+    return sa.cost[r];
+}
+
+__device__ inline void sah_node(uint32_t node, uint32_t a, uint32_t b, float3 la, float3 ha, float3 lb, float3 hb,
+                                float3 l, float3 h, const SahArgs& sa) {
+    const float4 ca4 = child_cost(a, sa, la, ha), cb4 = child_cost(b, sa, lb, hb);
+    const float ca[5] = {0.f, ca4.x, ca4.y, ca4.z, ca4.w}, cb[5] = {0.f, cb4.x, cb4.y, cb4.z, cb4.w};
+    float dist[5];
+    uint32_t kbest[5];
+    for (int i = 2; i <= 4; i++) {
+        dist[i] = INFINITY;
+        kbest[i] = 1;
+        for (int k = 1; k < i; k++) {
+            const float c = ca[k] + cb[i - k];
+            if (c < dist[i]) { dist[i] = c; kbest[i] = uint32_t(k); }
+        }
+    }
+    int istar = 2;
+    for (int i = 3; i <= 4; i++)
+        if (dist[i] < dist[istar]) istar = i;
+    const float area = half_area(l, h);
+    const float c_int = kSahNodeCost * area + dist[istar];
+    const uint32_t cnt = sa.last[node] - sa.first[node] + 1u;
+    const float c_leaf = cnt <= sa.leaf_max ? kSahTriCost * float(cnt) * area : INFINITY;
+    float c[5];
+    uint32_t d = 0;
+    if (c_int < c_leaf) { c[1] = c_int; d |= 1u; } else { c[1] = c_leaf; }
+    for (int i = 2; i <= 4; i++) {
+        if (dist[i] < c[i - 1]) { c[i] = dist[i]; d |= kbest[i] << (2 * (i - 1)); }
+        else c[i] = c[i - 1];
+    }
+    d |= uint32_t(istar - 2) << 8;
+    d |= kbest[istar] << 10;
+    sa.cost[node] = make_float4(c[1], c[2], c[3], c[4]);
+    sa.dec[node] = d;
+}
+
+template <bool kSah>
 __global__ void k_fit(const float4* lo, const float4* hi, const uint32_t* perm, int n, const uint32_t* left,
                       const uint32_t* right, const uint32_t* parent_int, const uint32_t* parent_leaf,
-                      const uint32_t* is_root, uint32_t* visits, float4* nlo, float4* nhi) {
+                      const uint32_t* is_root, uint32_t* visits, float4* nlo, float4* nhi, SahArgs sa) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint32_t node = parent_leaf[p];
     while (node != kEmpty) {
-        // the second child to arrive computes the union; acq_rel makes the first
-        // child's box visible
+        // the second child to arrive computes the union (and the SAH costs); acq_rel
+        // makes the first child's box and costs visible
         uint32_t prev = __hip_atomic_fetch_add(&visits[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == 0u) return;
         float3 l0, h0, l1, h1;
-        load_box(left[node], lo, hi, perm, nlo, nhi, l0, h0);
-        load_box(right[node], lo, hi, perm, nlo, nhi, l1, h1);
+        const uint32_t a = left[node], b = right[node];
+        load_box(a, lo, hi, perm, nlo, nhi, l0, h0);
+        load_box(b, lo, hi, perm, nlo, nhi, l1, h1);
         float3 l = fmin3(l0, l1), h = fmax3(h0, h1);
         nlo[node] = make_float4(l.x, l.y, l.z, 0.f);
         nhi[node] = make_float4(h.x, h.y, h.z, 0.f);
+        if (kSah) sah_node(node, a, b, l0, h0, l1, h1, l, h, sa);
         if (is_root[node]) return;
         node = parent_int[node];
     }
@@ -527,6 +606,7 @@ struct CollapseArgs {
     uint32_t* node_counter;
     uint32_t max_nodes;     // capacity of nodes and of each task array
     uint32_t* overflow;     // set when a capacity would be exceeded
+    const uint32_t* dec;    // SAH decision words (NULL: the greedy largest-area collapse)
 };
 
 __device__ inline uint32_t ref_count(const CollapseArgs& a, uint32_t r) {
@@ -535,20 +615,43 @@ __device__ inline uint32_t ref_count(const CollapseArgs& a, uint32_t r) {
 __device__ inline uint32_t ref_first(const CollapseArgs& a, uint32_t r) {
     return (r & kLeafBit) ? (r & ~kLeafBit) : a.first[r];
 }
-__device__ inline float half_area(float3 l, float3 h) {
-    float3 e = h - l;
-    return e.x * e.y + e.y * e.z + e.z * e.x;
-}
-
 // Collapse of one level, in two passes so node numbering is deterministic:
 // gather picks the (up to 4) children of every task and counts the ones that become
 // internal wide nodes; after an exclusive scan of the counts, emit writes the nodes
 // and numbers the new internal nodes level by level in task order.
 // task = (binary ref or kEmpty, wide node index, segment)
+// SAH collapse: the children of the wide node made from binary node b, following
+// the decisions (each (ref, slots) pair is expanded until it takes one slot).
+__device__ inline int gather_children_sah(const CollapseArgs& a, uint32_t b, uint32_t c[4]) {
+    const uint32_t d = a.dec[b];
+    const uint32_t istar = ((d >> 8) & 3u) + 2u, kstar = (d >> 10) & 3u;
+    uint32_t sr[4], ss[4];
+    int sp = 0, nc = 0;
+    sr[sp] = a.right[b]; ss[sp++] = istar - kstar;
+    sr[sp] = a.left[b]; ss[sp++] = kstar;
+    while (sp > 0 && nc < 4) {
+        const uint32_t r = sr[--sp];
+        uint32_t j = ss[sp];
+        if (!(r & kLeafBit)) {
+            while (j > 1u && dec_slots(a.dec[r], int(j)) == 0u) j--;  // C(r, j) = C(r, j - 1)
+        }
+        if ((r & kLeafBit) || j == 1u) {
+            c[nc++] = r;
+            continue;
+        }
+        const uint32_t k = dec_slots(a.dec[r], int(j));
+        if (sp + 2 > 4) { c[nc++] = r; continue; }  // cannot happen: slots add up to at most 4
+        sr[sp] = a.right[r]; ss[sp++] = j - k;
+        sr[sp] = a.left[r]; ss[sp++] = k;
+    }
+    return nc;
+}
+
 __device__ inline int gather_children(const CollapseArgs& a, uint32_t b, uint32_t c[4], float3 cl[4], float3 ch[4]) {
     int nc;
     if (b == kEmpty) return 0;
     if (b & kLeafBit) { c[0] = b; nc = 1; }
+    else if (a.dec) return gather_children_sah(a, b, c);
     else { c[0] = a.left[b]; c[1] = a.right[b]; nc = 2; }
     for (int k = 0; k < nc; k++) load_box(c[k], a.lo, a.hi, a.perm, a.nlo, a.nhi, cl[k], ch[k]);
     while (nc < 4) {
@@ -571,7 +674,9 @@ __device__ inline int gather_children(const CollapseArgs& a, uint32_t b, uint32_
 }
 
 __device__ inline bool becomes_internal(const CollapseArgs& a, uint32_t r) {
-    return !(r & kLeafBit) && ref_count(a, r) > a.leaf_max;
+    if (r & kLeafBit) return false;
+    if (a.dec) return (a.dec[r] & 1u) != 0u;
+    return ref_count(a, r) > a.leaf_max;
 }
 
 __global__ void k_collapse_gather(CollapseArgs a, const uint4* tasks, uint32_t n, uint4* refs, uint32_t* counts,
@@ -881,7 +986,8 @@ struct Scratch {
 }  // namespace
 
 int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t n, const uint32_t* d_seg_start,
-               const std::vector<uint32_t>& h_seg_start, uint32_t leaf_max, LbvhResult* out, std::string* err) {
+               const std::vector<uint32_t>& h_seg_start, uint32_t leaf_max, uint32_t collapse, LbvhResult* out,
+               std::string* err) {
     const uint32_t nseg = uint32_t(h_seg_start.size()) - 1;
     if (leaf_max < 1 || leaf_max > uint32_t(kMaxLeafPrims)) { *err = "leaf size out of range"; return int(hipErrorInvalidValue); }
     if (n >= (1u << kLeafCountShift)) { *err = "too many primitives in one build"; return int(hipErrorInvalidValue); }
@@ -978,12 +1084,22 @@ int lbvh_build(hipStream_t st, const float4* d_lo, const float4* d_hi, uint32_t 
         hipLaunchKernelGGL(k_single_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, d_seg_start, nseg,
                            seg_root);
         hipLaunchKernelGGL(k_mark_roots, dim3(blocks_for(nseg, kBlock)), dim3(kBlock), 0, st, seg_root, nseg, is_root);
-        hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_lo, d_hi, v0, int(n), left, right,
-                           parent_int, parent_leaf, is_root, visits, nlo, nhi);
+        const bool sah = collapse == kCollapseSah;
+        SahArgs sa{nullptr, nullptr, first, last, leaf_max};
+        if (sah) {
+            sa.cost = s.alloc<float4>(n);
+            sa.dec = s.alloc<uint32_t>(n);
+            if (!sa.cost || !sa.dec) return fail(int(hipErrorOutOfMemory), "out of device memory");
+            hipLaunchKernelGGL(k_fit<true>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_lo, d_hi, v0, int(n),
+                               left, right, parent_int, parent_leaf, is_root, visits, nlo, nhi, sa);
+        } else {
+            hipLaunchKernelGGL(k_fit<false>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, d_lo, d_hi, v0, int(n),
+                               left, right, parent_int, parent_leaf, is_root, visits, nlo, nhi, sa);
+        }
         if (hipGetLastError() != hipSuccess) return fail(int(hipErrorLaunchFailure), "launch k_fit");
 
         CollapseArgs a{d_lo, d_hi, v0, nlo, nhi, left, right, first, last, d_seg_start, nseg, pad, leaf_max,
-                       nodes_tmp, nc, max_nodes, nc + 3};
+                       nodes_tmp, nc, max_nodes, nc + 3, sa.dec};
         uint4* ta = s.alloc<uint4>(max_nodes);
         uint4* tb = s.alloc<uint4>(max_nodes);
         uint4* refs = s.alloc<uint4>(max_nodes);

@@ -33,7 +33,7 @@ __device__ inline void flush_counters(unsigned long long* c, const PathCounters&
                                 pc.tr.tris + pc.trs.tris, pc.tr.instances + pc.trs.instances, pc.shade,
                                 pc.tr.anyhits + pc.trs.anyhits, pc.tex, pc.lights,
                                 pc.tr.overflows + pc.trs.overflows, pc.bad, pc.trs.nodes, pc.trs.tris,
-                                pc.trs.instances, 0};
+                                pc.trs.instances, 0, 0};
 #pragma unroll
     for (int k = 0; k < kNumCounters; k++) {
         unsigned long long s = wave_sum<unsigned long long>(v[k]);
@@ -190,6 +190,8 @@ struct ShadeState {
     D3 light_dir;
     double light_dist;
     D3 light_scale;   // light_weight * intensity * color
+    double light_w;   // light_weight (the RIS weight, rchit:119)
+    uint32_t light_index;  // the chosen point light
     Rng4 rng;         // local stream after the RIS draws
     double displacement;
     // read by the adjoint only (dead in the forward kernels)
@@ -408,6 +410,8 @@ __device__ inline bool shade_begin(const FrameArgs& fa, Payload& ray, const HitI
             s.light_dist = dlength(lv);
             s.light_dir = dnormalize(lv);
             s.light_scale = ld3(l.color) * (w * double(l.intensity));
+            s.light_w = w;
+            s.light_index = sel;
             s.has_light = true;
         }
     }
@@ -539,15 +543,13 @@ struct BounceOut {
     double sdist;
     uint32_t shadow_wb;  // stream-B counter of the shadow payload's copy (rchit:45)
     bool shadow, cont;
-    // read by the adjoint only
-    D3 emit, nee;     // emittance, NEE term (before the shadow test)
-    D3 mult;          // factor from this bounce's throughput to the next one's
-    double rr;        // Russian-roulette probability applied (1: none)
 };
 
-// Closest-hit observer of shade_bounce (the adjoint records the shaded vertex).
+// Closest-hit observer of shade_bounce: vertex(state, payload, brdf, nee) sees each
+// shaded vertex after its scatter, with the NEE BRDF value and term (zero without a
+// light).  The adjoint's replay differentiates there; the forward kernels pass this.
 struct NoTap {
-    __device__ void vertex(const ShadeState&, const Payload&) {}
+    __device__ void vertex(const ShadeState&, const Payload&, D3, D3) {}
 };
 
 // The shaders for the traced result of p's current ray (closest-hit with NEE set-up
@@ -566,7 +568,7 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
     ray.need_scatter = false;
     ray.hit_value = d3(0.0);
     ray.scatter_dir = d3(0.0);
-    D3 nee = d3(0.0);
+    D3 nee = d3(0.0), brdf = d3(0.0);
     r.shadow = false;
     if (found) {
         double ht, bu, bv;
@@ -579,23 +581,20 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
                 r.sd = ss.light_dir;
                 r.sdist = ss.light_dist;
                 r.shadow_wb = ray.rng.w;
-                nee = eval_combined_brdf(ss.outward, ss.light_dir, ss.V, ss.m) * ss.light_scale;
+                brdf = eval_combined_brdf(ss.outward, ss.light_dir, ss.V, ss.m);
+                nee = brdf * ss.light_scale;
             }
             shade_end(ray, ss);
             // the counting kernels carry the debug == 1 materials (the host runs them
             // for such frames), so the timed kernels keep their registers
             if (kStats && fa.debug) debug_scatter(ray, ss, p.d);
-            tap.vertex(ss, ray);
+            tap.vertex(ss, ray, brdf, nee);
         }
     } else {
         miss_shader(fa.sv, ray, p.d);
     }
     r.pend_without = p.thr * ray.emittance;
     r.pend_with = p.thr * (ray.emittance + nee);
-    r.emit = ray.emittance;
-    r.nee = nee;
-    r.mult = d3(0.0);
-    r.rr = 1.0;
     p.hit_point = ray.hit_point;
     p.volume_dis = ray.volume_dis;
     p.wb = ray.rng.w;
@@ -613,8 +612,6 @@ __device__ inline void shade_bounce(const FrameArgs& fa, uint32_t x, uint32_t y,
         p.thr = p.thr / rr;
     }
     p.thr = p.thr * ray.hit_value;
-    r.mult = ray.hit_value / rr;
-    r.rr = rr;
     if (!ray.need_scatter || ray.t < 0.0) return;
     p.o = ray.hit_point;
     p.d = ray.scatter_dir;
@@ -667,27 +664,10 @@ __device__ inline void run_samples(const FrameArgs& fa, uint32_t x, uint32_t y, 
             bool found = trace_ray<kAnyHit, kStats>(fa.tlas, to_f3(p.o), to_f3(p.d), RayFromF64{p.o, p.d}, float(kTMin),
                                                     float(p.tfar), flags, ah, stack, stride, h, pc.tr);
             p.wb = rb.w;
-            const uint32_t bounce = p.bounce, wb = p.wb;
             BounceOut r;
             shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
             bool unocc = r.shadow && shadow_unoccluded<kAnyHit, kStats>(fa, x, y, r, stack, stride, pc);
             p.rad = p.rad + (unocc ? r.pend_with : r.pend_without);
-            // the counting kernels also record the path for the adjoint's replay
-            if (kStats && fa.rec_out) {
-                PathRecord& q = fa.rec_out[((size_t(y) * fa.width + x) * fa.samples + p.sample) * fa.bounces + bounce];
-                q.inst = found ? h.inst : kEmpty;
-                q.prim = h.prim;
-                q.rec = h.rec;
-                q.wb = wb;
-                q.t = h.t;
-                q.u = h.u;
-                q.v = h.v;
-                q.flags = (found ? kRecHit : 0u) | (unocc ? kRecUnoccluded : 0u) | (r.cont ? 0u : kRecLast);
-                const D3 e = unocc ? r.emit + r.nee : r.emit;
-                q.e[0] = e.x; q.e[1] = e.y; q.e[2] = e.z;
-                q.m[0] = r.mult.x; q.m[1] = r.mult.y; q.m[2] = r.mult.z;
-                q.rr = r.rr;
-            }
             if (!r.cont) break;
         }
     }

@@ -9,19 +9,6 @@
 
 namespace rt {
 
-// One bounce of a recorded forward path (RT_OPTION_RECORD_PATHS): what the adjoint's
-// replay needs, at ((y * width + x) * samples + sample) * bounces + bounce.
-struct PathRecord {
-    uint32_t inst, prim, rec;  // the hit (inst = kEmpty: the ray missed)
-    uint32_t wb;               // stream-B counter after the traversal (any-hit draws)
-    float t, u, v;             // fp32 hit (hit refinement's fallback)
-    uint32_t flags;            // kRecHit, kRecUnoccluded, kRecLast
-    double e[3], m[3];         // radiance added per unit throughput, factor to the next throughput
-    double rr;                 // Russian-roulette probability applied (1: none)
-};
-static_assert(sizeof(PathRecord) == 88, "PathRecord is 88 bytes");
-enum : uint32_t { kRecHit = 1u, kRecUnoccluded = 2u, kRecLast = 4u };
-
 // Per-launch arguments: the UBO fields the pipeline reads, the scene bindings and
 // the acceleration structure.  Passed by value (kernel arguments).
 struct FrameArgs {
@@ -37,13 +24,14 @@ struct FrameArgs {
     uint32_t samples, bounces, antialiasing, mapping, frame_count, fully_opaque;
     uint32_t debug;        // ubo.debug == 1: the legacy materials of RayTracing.rchit:437-475
     OutputArgs o;                  // display image (o.out NULL: none)
-    PathRecord* rec_out;           // path records of this frame (counting kernels), or NULL
 };
 
 enum Counter : int {
     kCntPrimary = 0, kCntExtension, kCntShadow, kCntNodes, kCntTris, kCntInstances, kCntShade, kCntAnyHit,
     kCntTex, kCntLights, kCntOverflow, kCntBadIndex, kCntShadowNodes, kCntShadowTris, kCntShadowInstances,
-    kNumCounters = 16
+    kCntGradTaps,     // adjoint: gradient contributions (texel or light) before the per-wave sums
+    kCntGradAtomics,  // adjoint: float atomics issued to the gradient buffers
+    kNumCounters
 };
 
 // Per-stage event timing of a trace (enabled by RT_OPTION_STAGE_TIMING).

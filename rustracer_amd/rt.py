@@ -84,7 +84,8 @@ class TraceStats(C.Structure):
                                             "tri_tests", "instance_visits", "shade_events", "anyhit_calls",
                                             "tex_fetches", "light_candidates", "stack_overflows")] + \
                [("kernel_ms", C.c_double)] + \
-               [(n, C.c_uint64) for n in ("shadow_node_visits", "shadow_tri_tests", "shadow_instance_visits")]
+               [(n, C.c_uint64) for n in ("shadow_node_visits", "shadow_tri_tests", "shadow_instance_visits",
+                                           "grad_taps", "grad_atomics")]
 
 
 OPTION_STAGE_TIMING = 1
@@ -97,7 +98,13 @@ OPTION_OUTPUT_BGRA = 7
 OPTION_WAVEFRONT_STREAMS = 8
 OPTION_WAVEFRONT_FINISH = 9
 OPTION_BLAS_LEAF_SIZE = 10
-OPTION_RECORD_PATHS = 11
+OPTION_BVH_COLLAPSE = 12
+
+# The value every option has on a new context (rt_create).
+DEFAULT_OPTIONS = {OPTION_STAGE_TIMING: 0, OPTION_FLATTEN_IDENTITY_INSTANCES: 1, OPTION_TRAVERSAL_VARIANT: 0,
+                   OPTION_GRAPH: 0, OPTION_EXTEND_REFILL: 40, OPTION_SHADOW_REFILL: 1, OPTION_OUTPUT_BGRA: 0,
+                   OPTION_WAVEFRONT_STREAMS: 2, OPTION_WAVEFRONT_FINISH: 65536, OPTION_BLAS_LEAF_SIZE: 2,
+                   OPTION_BVH_COLLAPSE: 1}
 TONE_MAP_DEFAULT, TONE_MAP_UNCHARTED, TONE_MAP_HEJL_RICHARD, TONE_MAP_ACES, TONE_MAP_SRGB = 0, 1, 2, 3, 4
 NUM_TRAVERSAL_VARIANTS = 4
 
@@ -139,7 +146,7 @@ def load_library(path=LIB_PATH):
         "rt_blas_info": (i, [vp, u64, vp]), "rt_tlas_info": (i, [vp, u64, vp]),
         "rt_set_option": (i, [vp, u, u64]), "rt_get_stage_times": (i, [vp, vp]),
         "rt_tlas_flat_info": (i, [vp, u64, vp]),
-        "rt_trace_rays_adjoint": (i, [vp, u64, vp, vp, u, u, vp, vp]),
+        "rt_trace_rays_adjoint": (i, [vp, u64, vp, vp, u, u, vp, vp, vp, vp]),
         "rt_trace_rays_batch": (i, [vp, vp, u, u, vp]),
         "rt_autotune": (i, [vp, u64, vp, u, u, vp]),
     }
@@ -199,6 +206,11 @@ class Context:
     def set_option(self, option, value):
         """rt_set_option (OPTION_STAGE_TIMING: event timing of every pipeline stage)."""
         self.check(self.lib.rt_set_option(self.handle, C.c_uint32(option), C.c_uint64(value)))
+
+    def reset_options(self):
+        """Every option back to its value on a new context."""
+        for k, v in DEFAULT_OPTIONS.items():
+            self.set_option(k, v)
 
     def stage_times(self):
         """rt_get_stage_times: device milliseconds per stage of the last trace."""
@@ -425,18 +437,29 @@ def trace_rays_batch(ctx, items, mode=MODE_WAVEFRONT, stats=False):
     return {n: getattr(st, n) for n, _ in TraceStats._fields_}
 
 
-def trace_rays_adjoint(ctx, tlas, buffers, ubo, grad_accum, width, height, grads, stats=False):
+def trace_rays_adjoint(ctx, tlas, buffers, ubo, grad_accum, width, height, grads, stats=False,
+                       grad_point_intensity=None, grad_directional_intensity=None):
     """rt_trace_rays_adjoint: adds to grads[k] (float32 device tensors shaped like
     RGBA32F texture k) the gradient of sum(grad_accum * accum) of the trace with the
-    same buffers and UBO.  `grads` maps texture index -> tensor."""
+    same buffers and UBO.  `grads` maps texture index -> tensor.  The optional
+    float32 device tensors grad_point_intensity (one entry per point light) and
+    grad_directional_intensity (one per directional light) receive the gradient with
+    respect to the lights' intensities."""
     b = buffers.bindings(ubo)
     n = len(buffers.scene.textures)
     ptrs = (C.c_void_p * max(1, n))()
     for k, g in grads.items():
         ptrs[k] = g.data_ptr()
+    for t, cnt in ((grad_point_intensity, len(buffers.scene.plights)),
+                   (grad_directional_intensity, len(buffers.scene.dlights))):
+        if t is not None and (t.dtype != _torch().float32 or not t.is_contiguous() or t.numel() < cnt):
+            raise ValueError("light-intensity gradients need a contiguous float32 tensor with one entry per light")
     st = TraceStats() if stats else None
-    ctx.check(ctx.lib.rt_trace_rays_adjoint(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(grad_accum.data_ptr()),
-                                            width, height, ptrs, C.byref(st) if st is not None else None))
+    ctx.check(ctx.lib.rt_trace_rays_adjoint(
+        ctx.handle, tlas.handle, C.byref(b), C.c_void_p(grad_accum.data_ptr()), width, height, ptrs,
+        C.c_void_p(grad_point_intensity.data_ptr()) if grad_point_intensity is not None else None,
+        C.c_void_p(grad_directional_intensity.data_ptr()) if grad_directional_intensity is not None else None,
+        C.byref(st) if st is not None else None))
     if st is None:
         return None
     return {n_: getattr(st, n_) for n_, _ in TraceStats._fields_}

@@ -99,3 +99,26 @@ def explain_mismatches(scene, rays_o, rays_d, gpu, orc, orc_t, tol_bary=2e-5, to
 
 def rel_rmse(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def unexplained_primary_mismatches(gpu, orc, orc_tuv, tol_bary=2e-5, tol_t=1e-5):
+    """Camera-ray hits where GPU and oracle name different (instance, primitive) and
+    the difference is not a documented sub-ulp tie.  A tie is: both candidates hit
+    within tol_bary of one of their triangle's edges (barycentrics of each side's own
+    hit record), or their t values agree to tol_t (coincident surfaces), or one side
+    misses and the other hits within tol_bary of an edge (a silhouette).  Returns the
+    pixel indices left unexplained."""
+    diff = np.nonzero((gpu["instance_id"] != orc["instance_id"]) | (gpu["primitive_id"] != orc["primitive_id"]))[0]
+    bad = []
+    for i in diff:
+        g_hit, o_hit = gpu["primitive_id"][i] != MISS, orc["primitive_id"][i] != MISS
+        gb = min(1.0 - float(gpu["u"][i]) - float(gpu["v"][i]), float(gpu["u"][i]), float(gpu["v"][i])) if g_hit else None
+        ob = min(1.0 - orc_tuv[i, 1] - orc_tuv[i, 2], orc_tuv[i, 1], orc_tuv[i, 2]) if o_hit else None
+        if g_hit and o_hit:
+            ok = (abs(float(gpu["t"][i]) - orc_tuv[i, 0]) <= tol_t * max(1.0, abs(orc_tuv[i, 0]))
+                  or (abs(gb) < tol_bary and abs(ob) < tol_bary))
+        else:
+            ok = abs(gb if g_hit else ob) < tol_bary
+        if not ok:
+            bad.append(int(i))
+    return bad

@@ -6,6 +6,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import torch
 
 from rustracer_amd import layouts as L, rt
 from tests import gpu_kit as G
@@ -84,22 +85,18 @@ def test_trace_argument_checks(gpu_ctx):
     assert lib.rt_trace_rays(h, ds.tlas.handle, C.byref(b3), C.c_void_p(acc.data_ptr()), None, 8, 8, 0, None) == -1
 
 
-def test_refit_and_record_argument_checks(gpu_ctx):
+def test_refit_and_option_argument_checks(gpu_ctx):
     lib, h = gpu_ctx.lib, gpu_ctx.handle
     assert lib.rt_refit_tlas(h, 987654) == -7  # unknown TLAS handle
-    assert lib.rt_set_option(h, rt.OPTION_RECORD_PATHS, 2) == -1
+    assert lib.rt_set_option(h, 11, 1) == -1  # no such option
     ds = G.device_scene(gpu_ctx, "cornell_sphere")
     sc = ds.scene
-    big = sc.ubo.copy()
-    big["number_of_samples"] = big["total_number_of_samples"] = 1 << 16  # 256^2 x 65536 x 4 x 88 B > 32 GiB
-    acc = rt.new_accumulation(sc.width, sc.height)
-    gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 1)
-    try:
-        with pytest.raises(rt.RtError) as e:
-            rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, big, acc, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL)
-        assert e.value.code == -1 and "32 GiB" in str(e.value)
-    finally:
-        gpu_ctx.set_option(rt.OPTION_RECORD_PATHS, 0)
+    deep = sc.ubo.copy()
+    deep["number_of_bounces"] = 65  # the adjoint keeps one shadow bit per bounce in 64 bits
+    g = torch.zeros((sc.height, sc.width, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(rt.RtError) as e:
+        rt.trace_rays_adjoint(gpu_ctx, ds.tlas, ds.buffers, deep, g, sc.width, sc.height, {})
+    assert e.value.code == -6 and "64 bounces" in str(e.value)
 
 
 def test_out_of_range_material_is_reported_not_read(gpu_ctx):

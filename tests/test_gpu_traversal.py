@@ -83,6 +83,7 @@ def test_primary_ids_cornell(gpu_ctx):
     oh, otuv = B.OracleScene(sc).trace_primary()
     same = (gh["instance_id"] == oh["instance_id"]) & (gh["primitive_id"] == oh["primitive_id"])
     assert (~same).sum() <= 3, np.nonzero(~same)[0][:10]
+    assert G.unexplained_primary_mismatches(gh, oh, otuv) == []
 
 
 def test_primary_ids_displaced_sphere(gpu_ctx):
@@ -94,6 +95,9 @@ def test_primary_ids_displaced_sphere(gpu_ctx):
     # camera rays are computed in fp32 on the GPU and fp64 in the oracle: only
     # near-edge pixels may differ
     assert (~same).mean() < 2e-4, (~same).sum()
+    bad = G.unexplained_primary_mismatches(gh, oh, otuv)
+    print("primary-id mismatches", int((~same).sum()), "unexplained", bad[:10])
+    assert bad == []
     hit = same & (oh["primitive_id"] != MISS)
     assert hit.sum() > 0.25 * len(hit)
 
