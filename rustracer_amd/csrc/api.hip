@@ -121,6 +121,8 @@ struct rt_ctx {
     int streams = 2;         // RT_OPTION_WAVEFRONT_STREAMS
     uint32_t leaf_max = kBlasLeafMax;  // RT_OPTION_BLAS_LEAF_SIZE
     uint32_t collapse = kCollapseSah;  // RT_OPTION_BVH_COLLAPSE
+    bool mega_persistent = true;       // RT_OPTION_MEGAKERNEL_PERSISTENT
+    uint32_t* d_band = nullptr;        // pixel counters of the persistent megakernel (8 bands)
     uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
@@ -327,6 +329,7 @@ int rt_create(rt_ctx** out, int hip_device, void* hip_stream) {
     if (e == hipSuccess) ctx->d_srgb = dev_alloc<float>(256, &e);
     if (e == hipSuccess) ctx->d_err = dev_alloc<uint32_t>(1, &e);
     if (e == hipSuccess) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
+    if (e == hipSuccess) ctx->d_band = dev_alloc<uint32_t>(8, &e);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) {
@@ -358,6 +361,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
     if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
     if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->d_band) (void)hipFree(ctx->d_band);
     if (ctx->d_textures) (void)hipFree(ctx->d_textures);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -857,7 +861,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     StageTimer* tm = ctx->stage_timing ? &ctx->timer : nullptr;
     ctx->timer.used = 0;
     if (mode == RT_MODE_MEGAKERNEL) {
-        rc = launch_megakernel(fa, st, any_hit, counting, tm);
+        rc = launch_megakernel(fa, st, any_hit, counting, tm, ctx->mega_persistent ? ctx->d_band : nullptr);
     } else {
         if (!ctx->wavefront) {
             ctx->wavefront = wavefront_create();
@@ -933,7 +937,8 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
     int rc = 0;
     if (mode == RT_MODE_MEGAKERNEL) {
         for (uint32_t i = 0; i < count && rc == 0; i++)
-            rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, counting, tm);
+            rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, counting, tm,
+                                   ctx->mega_persistent ? ctx->d_band : nullptr);
         if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
     } else {
         if (!ctx->wavefront) {
@@ -1191,6 +1196,11 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (option == RT_OPTION_GRAPH) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_GRAPH takes 0 or 1");
         ctx->use_graph = value != 0;
+        return RT_OK;
+    }
+    if (option == RT_OPTION_MEGAKERNEL_PERSISTENT) {
+        if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_MEGAKERNEL_PERSISTENT takes 0 or 1");
+        ctx->mega_persistent = value != 0;
         return RT_OK;
     }
     if (option == RT_OPTION_BVH_COLLAPSE) {

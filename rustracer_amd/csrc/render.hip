@@ -6,6 +6,17 @@
 // workgroup order is remapped so each of the 8 XCDs (which receive workgroups
 // round-robin) renders one horizontal band of the image and its L2 sees the part
 // of the scene that band looks at.
+//
+// Persistent form (default): a grid of as many workgroups as can be resident, each
+// lane taking its next pixel from a counter when its pixel is done (the pixel's
+// samples stay in order in one lane, so the image is the same).  A wave then no
+// longer lasts as long as its slowest pixel.  Pixels are numbered tile by tile
+// (8x8) within 8 horizontal bands; workgroup b starts on band b mod 8 (its XCD's)
+// and moves on to the next band when that one runs out.
+#include <algorithm>
+#include <mutex>
+#include <unordered_map>
+
 #include "path.h"
 
 namespace rt {
@@ -22,6 +33,55 @@ __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
     return (bid % nx) * per + bid / nx;
 }
 
+// One pixel of the megakernel: the raygen loop, then the accumulation and display
+// image writes (RayTracing.rgen:25-167).
+template <bool kAnyHit, bool kStats>
+__device__ inline void mega_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, PathCounters& pc) {
+    const bool heat = fa.o.out && fa.o.mapping == RT_MAPPING_HEAT;
+    const uint64_t t0 = heat ? uint64_t(clock64()) : 0u;  // clockARB(), rgen:27
+    float last_t;
+    D3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, stack, kMegaBlock, pc, last_t);
+    float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
+    float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
+    const float4 sum =
+        make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
+    *acc = sum;
+    if (fa.o.out) write_output(fa.o, fa.width, x, y, sum, last_t, heat ? uint64_t(clock64()) - t0 : 0u);
+}
+
+constexpr uint32_t kBands = 8;
+
+// Band of 8x8 tile rows [r0, r1) of band b.
+__device__ inline void band_rows(uint32_t tiles_y, uint32_t b, uint32_t& r0, uint32_t& r1) {
+    r0 = tiles_y * b / kBands;
+    r1 = tiles_y * (b + 1) / kBands;
+}
+
+template <bool kAnyHit, bool kStats>
+__global__ __launch_bounds__(kMegaBlock) void k_megakernel_persistent(FrameArgs fa, uint32_t* band_next) {
+    __shared__ uint32_t s_stack[kStackDepth * kMegaBlock];
+    const uint32_t tiles_x = (fa.width + 7) / 8;
+    const uint32_t tiles_y = (fa.height + 7) / 8;
+    PathCounters pc;
+    uint32_t band = blockIdx.x % kBands;
+    for (uint32_t tried = 0; tried < kBands;) {
+        uint32_t r0, r1;
+        band_rows(tiles_y, band, r0, r1);
+        const uint32_t items = (r1 - r0) * tiles_x * 64u;
+        const uint32_t it = items ? atomicAdd(&band_next[band], 1u) : items;
+        if (it >= items) {  // this band is done: the next one
+            band = (band + 1) % kBands;
+            tried++;
+            continue;
+        }
+        const uint32_t tile = it >> 6, within = it & 63u;
+        const uint32_t x = (tile % tiles_x) * 8 + (within & 7u);
+        const uint32_t y = (r0 + tile / tiles_x) * 8 + (within >> 3);
+        if (x < fa.width && y < fa.height) mega_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, pc);
+    }
+    if (kStats) flush_counters(fa.counters, pc);
+}
+
 template <bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kMegaBlock) void k_megakernel(FrameArgs fa) {
     __shared__ uint32_t s_stack[kStackDepth * kMegaBlock];
@@ -33,18 +93,7 @@ __global__ __launch_bounds__(kMegaBlock) void k_megakernel(FrameArgs fa) {
     const uint32_t x = tx * 16 + wave * 8 + (lane & 7);
     const uint32_t y = ty * 8 + (lane >> 3);
     PathCounters pc;
-    if (x < fa.width && y < fa.height) {
-        const bool heat = fa.o.out && fa.o.mapping == RT_MAPPING_HEAT;
-        const uint64_t t0 = heat ? uint64_t(clock64()) : 0u;  // clockARB(), rgen:27
-        float last_t;
-        D3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, kMegaBlock, pc, last_t);
-        float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
-        float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
-        const float4 sum =
-            make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
-        *acc = sum;
-        if (fa.o.out) write_output(fa.o, fa.width, x, y, sum, last_t, heat ? uint64_t(clock64()) - t0 : 0u);
-    }
+    if (x < fa.width && y < fa.height) mega_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, pc);
     if (kStats) flush_counters(fa.counters, pc);
 }
 
@@ -118,14 +167,49 @@ __global__ __launch_bounds__(kMegaBlock) void k_primary(FrameArgs fa, rt_hit* hi
 
 }  // namespace
 
-int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm) {
+// Resident workgroups of kernel f across the device (occupancy x CUs), per kernel.
+static uint32_t resident_grid(const void* f) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, uint32_t> grids;
+    std::lock_guard<std::mutex> lk(mu);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const void* key = static_cast<const char*>(f) + dev;  // per kernel and device
+    auto it = grids.find(key);
+    if (it != grids.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kMegaBlock, 0) != hipSuccess || per_cu < 1) per_cu = 2;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    const uint32_t g = uint32_t(cus) * uint32_t(per_cu);
+    grids.emplace(key, g);
+    return g;
+}
+
+int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* tm,
+                      uint32_t* band_next) {
     uint32_t tiles = ((fa.width + 15) / 16) * ((fa.height + 7) / 8);
     if (tiles == 0) return 0;
     if (tm) {
         hipError_t e = tm->mark(st, -1);
         if (e != hipSuccess) return int(e);
     }
-    if (any_hit) {
+    if (band_next) {
+        hipError_t e = hipMemsetAsync(band_next, 0, sizeof(uint32_t) * kBands, st);
+        if (e != hipSuccess) return int(e);
+        const void* f = any_hit ? (stats ? (const void*)k_megakernel_persistent<true, true>
+                                         : (const void*)k_megakernel_persistent<true, false>)
+                                : (stats ? (const void*)k_megakernel_persistent<false, true>
+                                         : (const void*)k_megakernel_persistent<false, false>);
+        // no more workgroups than there are pixel tiles to start on
+        const uint32_t g = std::min(resident_grid(f), std::max(tiles, kBands));
+        if (any_hit) {
+            if (stats) hipLaunchKernelGGL((k_megakernel_persistent<true, true>), dim3(g), dim3(kMegaBlock), 0, st, fa, band_next);
+            else hipLaunchKernelGGL((k_megakernel_persistent<true, false>), dim3(g), dim3(kMegaBlock), 0, st, fa, band_next);
+        } else {
+            if (stats) hipLaunchKernelGGL((k_megakernel_persistent<false, true>), dim3(g), dim3(kMegaBlock), 0, st, fa, band_next);
+            else hipLaunchKernelGGL((k_megakernel_persistent<false, false>), dim3(g), dim3(kMegaBlock), 0, st, fa, band_next);
+        }
+    } else if (any_hit) {
         if (stats) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
         else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(tiles), dim3(kMegaBlock), 0, st, fa);
     } else {

@@ -7,7 +7,10 @@ frame) for the generated scenes in rustracer_amd.scenes.WORKLOADS.
 
 Ray counts come from one counting frame before the timed frames.  --autotune runs
 rt_autotune first; --ppm writes the last frame's accumulation image, divided by the
-sample count and clamped to [0, 1], as a binary PPM.
+sample count and clamped to [0, 1], as a binary PPM.  --adjoint also times the
+path-replay adjoint of the frame (rt_trace_rays_adjoint) for a seeded random image
+gradient, with gradients for every RGBA32F texture and the point lights'
+intensities.
 """
 import argparse
 import json
@@ -24,6 +27,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--ppm", default=None)
+    ap.add_argument("--adjoint", action="store_true")
     args = ap.parse_args(argv)
     if args.frames < 1 or args.warmup < 0:
         ap.error("--frames must be at least 1 and --warmup at least 0")
@@ -55,6 +59,8 @@ def main(argv=None):
             "samples": int(sc.ubo["number_of_samples"]), "bounces": int(sc.ubo["number_of_bounces"]),
             "rays": int(st["rays"]), "ms_per_frame": round(ms, 3), "mrays_per_s": round(st["rays"] / ms / 1e3, 1),
             "autotune": tuned}
+    if args.adjoint:
+        line["adjoint"] = _adjoint(ctx, sc, buf, tlas, args.frames, args.warmup)
     print(json.dumps(line), flush=True)
     if args.ppm:
         img = acc.cpu().numpy()[..., :3] / float(sc.ubo["total_number_of_samples"])
@@ -64,6 +70,30 @@ def main(argv=None):
             f.write(px.tobytes())
     del blases, tlas, buf
     return 0
+
+
+def _adjoint(ctx, sc, buf, tlas, frames, warmup):
+    import numpy as np
+    import torch
+    from rustracer_amd import layouts as L
+    from rustracer_amd import rt
+    W, H = sc.width, sc.height
+    g = torch.from_numpy(np.random.default_rng(0).normal(size=(H, W, 4)).astype(np.float32)).cuda()
+    grads = {k: torch.zeros(t.data.shape, dtype=torch.float32, device="cuda")
+             for k, t in enumerate(sc.textures) if t.format == L.FMT_RGBA32F}
+    gp = torch.zeros(max(1, len(sc.plights)), dtype=torch.float32, device="cuda")
+    st = rt.trace_rays_adjoint(ctx, tlas, buf, sc.ubo, g, W, H, grads, stats=True, grad_point_intensity=gp)
+    for _ in range(warmup):
+        rt.trace_rays_adjoint(ctx, tlas, buf, sc.ubo, g, W, H, grads, grad_point_intensity=gp)
+    ctx.sync()
+    times = []
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        rt.trace_rays_adjoint(ctx, tlas, buf, sc.ubo, g, W, H, grads, grad_point_intensity=gp)
+        ctx.sync()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return {"ms": round(min(times), 3), "rays": int(st["rays"]), "grad_contributions": int(st["grad_taps"]),
+            "grad_atomics": int(st["grad_atomics"])}
 
 
 if __name__ == "__main__":

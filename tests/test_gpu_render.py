@@ -5,6 +5,7 @@ and determinism.  Restates crates/examples/gltf_viewer/shaders/RayTracing.rgen
 (:25-167), RayTracing.rchit (:136-477) and RayTracing.rmiss (:14-45)."""
 import numpy as np
 import pytest
+import torch
 
 from oracle import binding as B
 from rustracer_amd import rt
@@ -289,3 +290,30 @@ def test_autotune_on_a_deep_bvh_leaves_no_error(gpu_ctx):
         gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
         gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
         gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)
+
+
+@pytest.mark.parametrize("name", ["cornell_sphere", "textured_small"])
+def test_persistent_megakernel_equals_one_lane_per_pixel(gpu_ctx, name):
+    """RT_OPTION_MEGAKERNEL_PERSISTENT: lanes that fetch their next pixel give the
+    same images (and display images) as one lane per pixel; HEAT frames too keep
+    the accumulation."""
+    if name == "textured_small":
+        ds = G.device_scene(gpu_ctx, "textured_small_mk", lambda: S.textured_instances(
+            width=100, height=60, spp=2, bounces=4, num_instances=60, tex_size=64, tri_range=(2000, 20000)))
+    else:
+        ds = G.device_scene(gpu_ctx, name)
+    sc = ds.scene
+    out = []
+    for persistent in (0, 1):
+        gpu_ctx.set_option(rt.OPTION_MEGAKERNEL_PERSISTENT, persistent)
+        acc = rt.new_accumulation(sc.width, sc.height)
+        disp = torch.zeros((sc.height, sc.width, 4), dtype=torch.uint8, device="cuda")
+        st = rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, acc, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL,
+                           stats=True, out=disp)
+        acc2 = rt.new_accumulation(sc.width, sc.height)
+        rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, acc2, sc.width, sc.height, mode=rt.MODE_MEGAKERNEL)
+        gpu_ctx.sync()
+        assert torch.equal(acc, acc2)
+        out.append((acc.cpu().numpy(), disp.cpu().numpy(), st["rays"]))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
