@@ -28,10 +28,22 @@ Follows crates/libs/asset_loader:
   used as normal, metallic-roughness, transmission or specular textures are linear,
   the others sRGB.
 
-Not ingested (ValueError): skins (the reference duplicates skinned meshes per skin
-before its compute pass), sparse accessors, primitives of other modes are skipped as
-the reference skips them.  Tangents are not generated for normal-mapped primitives
-without TANGENT (the reference runs mikktspace there).
+- scene_graph.rs:191-230, skinning.rs, acceleration_structures.rs:146-152: a mesh of
+  a node with a skin is skinned by that skin (every vertex of its primitives gets
+  skin_index = the skin; one skin per mesh, and a skinned mesh may not also be drawn
+  unskinned); its instances take the identity transform, because the skinning pass
+  writes world-space vertices.  JOINTS_0 / WEIGHTS_0 fill joints / weights.  The
+  scene's `skins` are the joint matrices world(joint) * inverseBindMatrix (column-
+  major, 256 per skin, identity beyond the skin's joints) of the bind pose, as
+  Doc::get_skins gives them: the input of rt_skin_vertices (AnimationCompute.comp).
+
+- geometry.rs:192-211, 296-349: a primitive without TANGENT whose material has a
+  normal texture gets MikkTSpace tangents (the `mikktspace` 0.3.0 crate, restated in
+  `mikktspace_tangents`), with the reference's sign: w = -1 where the uv mapping
+  preserves orientation, +1 where it mirrors.
+
+Not ingested (ValueError): sparse accessors; primitives of other modes are skipped as
+the reference skips them.
 """
 import base64
 import json
@@ -201,6 +213,107 @@ def _flat_normals(pos, idx):
     return n
 
 
+def _safe_normalize(v):
+    n = np.linalg.norm(v, axis=-1, keepdims=True)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(n > 0, v / np.where(n > 0, n, 1), 0.0)
+
+
+def mikktspace_tangents(pos, nrm, uv, idx):
+    """Per-vertex tangents (x, y, z, w) of an indexed triangle list by the MikkTSpace
+    method (M. Mikkelsen, "Simulation of Wrinkled Surfaces Revisited", 2008; the
+    `mikktspace` 0.3.0 crate the reference calls from geometry.rs:192-211):
+    - vertices with equal position, normal and uv are one vertex;
+    - per triangle, the direction of growing u, dS = (t31.v d1 - t21.v d2) / area of
+      the edges d1 = p1 - p0, d2 = p2 - p0, uv edges t21, t31 and uv area
+      t21.u t31.v - t21.v t31.u, normalised; the area's sign is its orientation;
+    - at each vertex, the triangles around it that share an edge and have the same
+      orientation form a group; the group's tangent is the normalised sum of dS
+      projected on the vertex's tangent plane, weighted by the triangle's angle at
+      the vertex (its two edges also projected on that plane);
+    - each triangle corner is written in triangle order, so a vertex takes the group
+      of the last triangle that uses it (the reference's set_tangent_encoded writes
+      tangents[indices[3 f + v]]);
+    - w = -1 for an orientation-preserving group and +1 otherwise (the reference's
+      set_tangent, geometry.rs:336-349, inverts the crate's convention).
+    Triangles of zero uv area contribute no direction and join the group of a
+    neighbour; corners without any direction get (1, 0, 0)."""
+    pos = np.asarray(pos, np.float64)[:, :3]
+    nrm = np.asarray(nrm, np.float64)[:, :3]
+    uv = np.asarray(uv, np.float64)[:, :2]
+    tri = np.asarray(idx, np.int64).reshape(-1, 3)
+    out = np.zeros((len(pos), 4), np.float32)
+    out[:, 0] = 1.0
+    if len(tri) == 0:
+        return out
+    # weld equal (position, normal, uv) vertices
+    key = np.concatenate([pos, nrm, uv], 1)
+    _, weld = np.unique(key, axis=0, return_inverse=True)
+    weld = weld.reshape(-1)
+    wt = weld[tri]
+    p0, p1, p2 = pos[tri[:, 0]], pos[tri[:, 1]], pos[tri[:, 2]]
+    t0, t1, t2 = uv[tri[:, 0]], uv[tri[:, 1]], uv[tri[:, 2]]
+    d1, d2 = p1 - p0, p2 - p0
+    t21, t31 = t1 - t0, t2 - t0
+    area = t21[:, 0] * t31[:, 1] - t21[:, 1] * t31[:, 0]
+    orient = area > 0
+    # d p / d u = (t31.v d1 - t21.v d2) / area: normalised, with the sign of the area
+    dS = np.where(orient[:, None], 1.0, -1.0) * _safe_normalize(t31[:, 1:2] * d1 - t21[:, 1:2] * d2)
+    degen_uv = area == 0
+    # neighbours across edges (an edge a->b meets b->a of another triangle)
+    edge_owner = {}
+    for t in range(len(tri)):
+        for k in range(3):
+            edge_owner.setdefault((wt[t, k], wt[t, (k + 1) % 3]), []).append(t)
+    # union-find over corners (t, k): corners at the same vertex of neighbouring
+    # triangles of the same orientation (or of zero uv area) join
+    parent = np.arange(3 * len(tri))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+    for t in range(len(tri)):
+        for k in range(3):
+            a, b = wt[t, k], wt[t, (k + 1) % 3]
+            for u in edge_owner.get((b, a), ()):
+                if u == t or not (orient[t] == orient[u] or degen_uv[t] or degen_uv[u]):
+                    continue
+                for kk in range(3):  # the shared edge's two vertices
+                    v = wt[u, kk]
+                    if v == a or v == b:
+                        ka = k if v == a else (k + 1) % 3
+                        ra, rb = find(3 * t + ka), find(3 * u + kk)
+                        if ra != rb:
+                            parent[ra] = rb
+    roots = np.array([find(c) for c in range(3 * len(tri))])
+    # angle-weighted sum of the projected directions per group
+    acc = {}
+    for t in range(len(tri)):
+        for k in range(3):
+            v = tri[t, k]
+            n = nrm[v]
+            pc, pp, pn = pos[tri[t, k]], pos[tri[t, (k + 2) % 3]], pos[tri[t, (k + 1) % 3]]
+            e1, e2 = pp - pc, pn - pc
+            e1 = _safe_normalize(e1 - n * (n @ e1))
+            e2 = _safe_normalize(e2 - n * (n @ e2))
+            ang = np.arccos(np.clip(e1 @ e2, -1.0, 1.0))
+            ds = _safe_normalize(dS[t] - n * (n @ dS[t]))
+            r = roots[3 * t + k]
+            s_, o_ = acc.get(r, (np.zeros(3), None))
+            acc[r] = (s_ + ang * ds, o_ if (o_ is not None or degen_uv[t]) else bool(orient[t]))
+    for t in range(len(tri)):
+        for k in range(3):
+            s_, o_ = acc[roots[3 * t + k]]
+            tg = _safe_normalize(s_)
+            if not np.any(tg):
+                tg = np.array([1.0, 0.0, 0.0])
+            out[tri[t, k], :3] = tg
+            out[tri[t, k], 3] = -1.0 if o_ else 1.0
+    return out
+
+
 def _local(node):
     """Node::get_local_transform: `matrix` (column-major) or T * R * S."""
     if "matrix" in node:
@@ -279,8 +392,6 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
     becomes binding 12 and sets ubo.HasSky."""
     doc = Document(path)
     js = doc.json
-    if js.get("skins"):
-        raise ValueError("skinned glTF scenes are not ingested")
     b = S.Builder()
     mats = [material_raw(m) for m in js.get("materials", [])] or [L.default_material()]
     for m in mats:
@@ -301,7 +412,15 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
             v["pos"][:, :3] = pos
             v["normal"] = (np.concatenate([doc.floats(attr["NORMAL"])[:, :3], np.zeros((n, 1), np.float32)], 1)
                            if "NORMAL" in attr else _flat_normals(pos.astype(np.float32), idx))
-            v["tangent"] = doc.floats(attr["TANGENT"]) if "TANGENT" in attr else (1.0, 0.0, 0.0, 0.0)
+            mid = int(prim.get("material", 0))
+            if "TANGENT" in attr:
+                v["tangent"] = doc.floats(attr["TANGENT"])
+            elif mid < len(js.get("materials", [])) and "normalTexture" in js["materials"][mid]:
+                # geometry.rs:197-211: normal map without tangents (uv 0 when absent)
+                uv0 = doc.floats(attr["TEXCOORD_0"]) if "TEXCOORD_0" in attr else np.zeros((n, 2), np.float32)
+                v["tangent"] = mikktspace_tangents(pos, v["normal"], uv0, idx)
+            else:
+                v["tangent"] = (1.0, 0.0, 0.0, 0.0)
             if "COLOR_0" in attr:
                 c = doc.floats(attr["COLOR_0"])
                 v["color"] = c if c.shape[1] == 4 else np.concatenate([c, np.ones((n, 1), np.float32)], 1)
@@ -316,7 +435,6 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
             if "JOINTS_0" in attr:
                 v["joints"] = doc.accessor(attr["JOINTS_0"])[0].astype(np.uint32)
             v["skin_index"] = -1
-            mid = int(prim.get("material", 0))
             if mid >= len(mats):
                 raise ValueError(f"primitive material {mid} out of range")
             geos.append(b.add_primitive(v, idx, mid))
@@ -357,10 +475,13 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
     # depth-first walk: instances and lights
     light_defs = js.get("extensions", {}).get("KHR_lights_punctual", {}).get("lights", [])
     entries, dl, pl = [], [], []
+    world_of = {}
+    mesh_skin = {}  # mesh -> skin of the nodes that draw it (None: drawn unskinned)
 
     def walk(i, parent):
         nd = nodes[i]
         world = parent @ _local(nd)
+        world_of[i] = world
         li = nd.get("extensions", {}).get("KHR_lights_punctual", {}).get("light")
         if li is not None:
             d = light_defs[li]
@@ -374,8 +495,14 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
             l["intensity"] = d.get("intensity", 1.0)
             (dl if kind == "directional" else pl).append(l)
         if "mesh" in nd:
-            for g in mesh_geos[nd["mesh"]]:
-                entries.append((world, g))
+            skin = nd.get("skin")
+            m = nd["mesh"]
+            if m in mesh_skin and mesh_skin[m] != skin:
+                raise ValueError(f"mesh {m} is drawn with different skins (or skinned and unskinned)")
+            mesh_skin[m] = skin
+            for g in mesh_geos[m]:
+                # a skinned node's instance is the identity: its vertices are skinned in world space
+                entries.append((np.eye(4) if skin is not None else world, g))
         for c in nd.get("children", []):
             walk(c, world)
     for r in roots:
@@ -410,6 +537,29 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
                                   _WRAP.get(smp.get("wrapT", 10497), L.WRAP_REPEAT),
                                   L.FILTER_NEAREST if smp.get("magFilter") == 9728 else L.FILTER_LINEAR,
                                   L.FILTER_NEAREST if smp.get("minFilter") in _NEAREST else L.FILTER_LINEAR))
+    # skinned meshes: skin index on their vertices, joint matrices of the bind pose
+    for m, skin in mesh_skin.items():
+        if skin is None:
+            continue
+        for g in mesh_geos[m]:
+            b.vertices[g]["skin_index"] = int(skin)  # Builder.vertices[g]: geometry g's vertices
+    skins = None
+    if js.get("skins"):
+        skins = np.zeros((len(js["skins"]), 256, 4, 4), np.float32)
+        skins[:] = np.eye(4, dtype=np.float32)
+        for k, sk in enumerate(js["skins"]):
+            joints = sk["joints"]
+            if "inverseBindMatrices" in sk:
+                ibm = doc.floats(sk["inverseBindMatrices"]).astype(np.float64).reshape(-1, 4, 4).transpose(0, 2, 1)
+            else:
+                ibm = np.repeat(np.eye(4)[None], len(joints), 0)
+            if len(ibm) != len(joints):
+                raise ValueError(f"skin {k}: {len(ibm)} inverse bind matrices for {len(joints)} joints")
+            for j, node in enumerate(joints[:256]):  # MAX_JOINTS (skinning.rs:9), the rest dropped
+                w = world_of.get(node)
+                if w is None:
+                    raise ValueError(f"skin {k}: joint node {node} is not in the scene")
+                skins[k, j] = (w @ ibm[j]).T  # column-major (Mat4::to_cols_array)
     inst = S.make_instances(entries)
     fully_opaque = all(int(m["alpha_mode"]) == L.ALPHA_OPAQUE for m in mats)
     view = S.look_at_rh(eye, target, (0, 1, 0))
@@ -417,6 +567,8 @@ def load(path, width=256, height=256, spp=4, bounces=4, eye=(0.0, 0.0, 1.0), tar
     ubo = S.make_ubo(view, proj, samples=spp, bounces=bounces, random_seed=seed, fully_opaque=fully_opaque)
     name = os.path.splitext(os.path.basename(path))[0]
     sc = b.finish(name, inst, pl, dl, ubo, width, height, textures)
+    sc.skins = skins
+    sc.normalisation = top  # Doc::aabb_trans: the transform above the root nodes
     if skybox is not None:
         sc.skybox = load_skybox(skybox) if isinstance(skybox, str) else skybox
         sc.ubo["has_sky"] = 1

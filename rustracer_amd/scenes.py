@@ -101,6 +101,8 @@ class Scene:
     height: int
     textures: list = field(default_factory=list)
     skybox: object = None       # Texture with data (6, w, w, 4): faces +x, -x, +y, -y, +z, -z
+    skins: object = None        # float32 (num_skins, 256, 4, 4) column-major joint matrices, or None
+    normalisation: object = None  # the transform the loader put above the root nodes (glTF scenes)
 
     @property
     def num_triangles(self):

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from rustracer_amd import gltf, layouts as L
+from tests import gltf_kit as K
 
 REF_CORNELL = "/root/reference/assets/models/CornellBox/cornellBox.gltf"
 REF_SKYBOX = "/root/reference/assets/skyboxs/Yokohama"
@@ -239,7 +240,7 @@ def test_reference_skybox_directory():
 
 
 def test_unsupported_inputs_raise_and_extensions_map(tmp_path):
-    """Skins and sparse accessors are refused; KHR_materials_unlit and
+    """Sparse accessors are refused; KHR_materials_unlit and
     KHR_materials_pbrSpecularGlossiness map to MaterialRaw's unlit flag and SG workflow
     (material.rs:162-190, 289-330)."""
     import json as _json
@@ -254,10 +255,6 @@ def test_unsupported_inputs_raise_and_extensions_map(tmp_path):
     assert sc.materials[0]["unlit"] == 1 and sc.materials[1]["workflow"] == 1
     assert np.allclose(sc.materials[1]["sg_specular_glossiness_factor"], [0.2, 0.2, 0.2, 0.6])
     assert tuple(sc.materials[1]["sg_diffuse_texture"]) == (1, 0)
-    skinned = dict(js, skins=[{"joints": [0]}])
-    open(p, "w").write(_json.dumps(skinned))
-    with pytest.raises(ValueError):
-        gltf.load(p)
     js["accessors"][0]["sparse"] = {"count": 1}
     open(p, "w").write(_json.dumps(js))
     with pytest.raises(ValueError):
@@ -276,3 +273,164 @@ def test_reference_cornell_box_renders_in_the_oracle():
     assert st["anyhit_calls"] > 0 and st["shadow_rays"] == 0
     img = o[..., :3] / 2
     assert (img > 0.011).mean() > 0.02 and img.max() > 1.0
+
+
+def _trs(t=(0, 0, 0), q=(0, 0, 0, 1), sc=(1, 1, 1)):
+    x, y, z, w = q
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    m = np.eye(4)
+    m[:3, :3] = r * np.asarray(sc, np.float64)[None, :]
+    m[:3, 3] = t
+    return m
+
+
+def _skinned_file(tmp_path, glb=False):
+    """A box skinned by two joints under a translated root: the lower half of its
+    vertices follows joint A (rotated 30 degrees about z), the upper half joint B
+    (translated), each with weight 1, and the middle blends them; the inverse bind
+    matrices are the joints' inverse world transforms at the rest pose (identity
+    rotation, B at (0, 1, 0))."""
+    w = K.Writer()
+    pos, nrm, uv, idx = K.box()
+    p = w.primitive(pos, idx, nrm, uv)
+    up = pos[:, 1] > 0
+    joints = np.zeros((len(pos), 4), np.uint16)
+    joints[:, 1] = 1
+    weights = np.zeros((len(pos), 4), np.float32)
+    weights[:, 0] = np.where(up, 0.0, 1.0)
+    weights[:, 1] = np.where(up, 1.0, 0.0)
+    weights[0] = (0.25, 0.75, 0, 0)  # a blended vertex
+    p["attributes"]["JOINTS_0"] = w.accessor(joints, 5123, "VEC4")
+    p["attributes"]["WEIGHTS_0"] = w.accessor(weights, 5126, "VEC4")
+    m = w.mesh([p])
+    root_t = (1.0, 2.0, 3.0)
+    qa = (0.0, 0.0, np.sin(np.pi / 12), np.cos(np.pi / 12))
+    ja = w.node(root=False, rotation=list(qa))
+    jb = w.node(root=False, translation=[0.0, 1.5, 0.0])
+    sk = w.node(root=False, mesh=m, skin=0, translation=[50.0, 0.0, 0.0])  # its own transform is ignored
+    root = w.node(translation=list(root_t), children=[ja, jb, sk])
+    rest = [_trs(root_t), _trs(root_t) @ _trs((0.0, 1.0, 0.0))]
+    ibm = np.stack([np.linalg.inv(r) for r in rest]).transpose(0, 2, 1).reshape(-1, 16).astype(np.float32)
+    w.js["skins"] = [{"joints": [ja, jb], "inverseBindMatrices": w.accessor(ibm, 5126, "MAT4")}]
+    worlds = [_trs(root_t) @ _trs(q=qa), _trs(root_t) @ _trs((0.0, 1.5, 0.0))]
+    joint_mats = [wd @ np.linalg.inv(r) for wd, r in zip(worlds, rest)]
+    path = w.save(str(tmp_path / ("skinned.glb" if glb else "skinned.gltf")), glb=glb)
+    return path, pos, joints, weights, joint_mats
+
+
+@pytest.mark.parametrize("glb", [False, True])
+def test_skinned_mesh_ingestion(tmp_path, glb):
+    """scene_graph.rs:191-230 (skin index on the skinned mesh's vertices),
+    acceleration_structures.rs:146-152 (identity instance), skinning.rs (joint matrix
+    = world(joint) * inverse bind matrix, MAX_JOINTS 256): the loaded vertices skinned
+    by the loaded matrices (the oracle's AnimationCompute.comp restatement) land where
+    the joints' transforms put them, normalised like the rest of the scene."""
+    from oracle import binding as OB
+    path, pos, joints, weights, joint_mats = _skinned_file(tmp_path, glb)
+    sc = gltf.load(path)
+    v = sc.vertices
+    assert np.all(v["skin_index"] == 0)
+    assert np.array_equal(v["joints"], joints.astype(np.uint32)) and np.array_equal(v["weights"], weights)
+    assert np.array_equal(sc.instances["transform"][0], np.eye(4)[:3].astype(np.float32))
+    assert sc.skins.shape == (1, 256, 4, 4)
+    assert np.array_equal(sc.skins[0, 2:], np.broadcast_to(np.eye(4, dtype=np.float32), (254, 4, 4)))
+    out, bad = OB.skin_vertices(sc.vertices, sc.skins)
+    assert bad == 0
+    # normalisation of the loader: the scene box (the mesh's box under the skinned node's
+    # local transform, scene_graph.rs:338-352) centred and scaled to 10
+    top = np.asarray(sc.normalisation, np.float64)
+    p4 = np.concatenate([pos, np.ones((len(pos), 1))], 1)
+    blend = np.einsum("vk,kij->vij", weights[:, :2].astype(np.float64), np.stack(joint_mats))
+    want = np.einsum("ij,vjk,vk->vi", top, blend, p4)[:, :3]
+    assert np.allclose(out["pos"][:, :3], want, rtol=0, atol=2e-5), np.abs(out["pos"][:, :3] - want).max()
+
+
+def test_a_mesh_drawn_with_two_skins_is_refused(tmp_path):
+    import json as _json
+    path, *_ = _skinned_file(tmp_path)
+    js = _json.load(open(path))
+    js["skins"].append(dict(js["skins"][0]))
+    js["nodes"].append({"mesh": 0, "skin": 1})
+    js["scenes"][0]["nodes"].append(len(js["nodes"]) - 1)
+    open(path, "w").write(_json.dumps(js))
+    with pytest.raises(ValueError):
+        gltf.load(path)
+
+
+# ---------------------------------------------------------------- tangents
+def _grid(nx=4, ny=3, flip_u=False):
+    """A planar grid in z = 0 with uv = (x, y) (or (-x, y)), normal +z."""
+    xs, ys = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1))
+    pos = np.stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)], 1)
+    uv = np.stack([(-1.0 if flip_u else 1.0) * xs.ravel(), ys.ravel()], 1)
+    idx = []
+    for j in range(ny):
+        for i in range(nx):
+            a = j * (nx + 1) + i
+            idx += [a, a + 1, a + nx + 2, a, a + nx + 2, a + nx + 1]
+    return pos, np.tile([0.0, 0.0, 1.0], (len(pos), 1)), uv, np.array(idx)
+
+
+def test_mikktspace_plane_closed_form():
+    """On a plane with u growing along +x the tangent is +x; the mapping preserves
+    orientation (uv area > 0), which the reference writes as w = -1
+    (geometry.rs:336-349); mirrored u gives -x and w = +1."""
+    pos, nrm, uv, idx = _grid()
+    t = gltf.mikktspace_tangents(pos, nrm, uv, idx)
+    assert np.allclose(t, [1, 0, 0, -1], atol=1e-7)
+    pos, nrm, uv, idx = _grid(flip_u=True)
+    t = gltf.mikktspace_tangents(pos, nrm, uv, idx)
+    assert np.allclose(t, [-1, 0, 0, 1], atol=1e-7)
+
+
+def test_mikktspace_cylinder_is_azimuthal_and_orthogonal():
+    """A cylinder with u = angle / 2 pi and v = height: the tangent at every vertex is
+    the azimuthal direction (the angle-weighted mean of the faces' +u directions,
+    which are symmetric about it), unit length, orthogonal to the normal."""
+    n_a, n_h = 24, 4
+    ang = 2 * np.pi * np.arange(n_a + 1) / n_a  # seam duplicated: u 0 and 1
+    h = np.linspace(0, 2, n_h + 1)
+    A, Hh = np.meshgrid(ang, h)
+    pos = np.stack([np.cos(A).ravel(), np.sin(A).ravel(), Hh.ravel()], 1)
+    nrm = np.stack([np.cos(A).ravel(), np.sin(A).ravel(), np.zeros(A.size)], 1)
+    uv = np.stack([A.ravel() / (2 * np.pi), Hh.ravel() / 2], 1)
+    idx = []
+    for j in range(n_h):
+        for i in range(n_a):
+            a = j * (n_a + 1) + i
+            idx += [a, a + 1, a + n_a + 2, a, a + n_a + 2, a + n_a + 1]
+    t = gltf.mikktspace_tangents(pos, nrm, uv, np.array(idx))
+    want = np.stack([-np.sin(A).ravel(), np.cos(A).ravel(), np.zeros(A.size)], 1)
+    assert np.allclose(t[:, :3], want, atol=1e-6)
+    assert np.allclose(np.abs(t[:, 3]), 1) and np.allclose((t[:, :3] * nrm).sum(1), 0, atol=1e-6)
+
+
+def test_mikktspace_groups_split_by_orientation_and_last_face_wins():
+    """Two triangles sharing an edge, the second with mirrored uv: their shared
+    vertices belong to two groups (one per orientation) and take the tangent of the
+    group of the later triangle."""
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float64)
+    nrm = np.tile([0.0, 0.0, 1.0], (4, 1))
+    uv = np.array([[0, 0], [1, 0], [0, 1], [0.0, 0.0]])  # vertex 3 folds the uv back: the 2nd triangle mirrors
+    idx = np.array([0, 1, 2, 2, 1, 3])
+    t = gltf.mikktspace_tangents(pos, nrm, uv, idx)
+    # triangle 0 alone: +x, preserving (w = -1) at vertex 0; vertices 1, 2 are rewritten
+    # by triangle 1, whose orientation is reversed (w = +1)
+    assert np.allclose(t[0], [1, 0, 0, -1], atol=1e-7)
+    assert t[1, 3] == 1 and t[2, 3] == 1 and t[3, 3] == 1
+
+
+def test_loader_generates_tangents_only_for_normal_mapped_primitives(tmp_path):
+    w = K.Writer()
+    pos, nrm, uv, idx = _grid(2, 2)
+    tex = w.texture(np.full((2, 2, 4), 128, np.uint8))
+    m_nm = w.material({"normalTexture": {"index": tex}})
+    m_plain = w.material({})
+    w.node(mesh=w.mesh([w.primitive(pos, idx, nrm, uv, material=m_nm)]))
+    w.node(mesh=w.mesh([w.primitive(pos + 2, idx, nrm, uv, material=m_plain)]))
+    sc = gltf.load(w.save(str(tmp_path / "nm.gltf")))
+    n = len(pos)
+    assert np.allclose(sc.vertices["tangent"][:n], [1, 0, 0, -1], atol=1e-7)
+    assert np.all(sc.vertices["tangent"][n:] == [1, 0, 0, 0])
