@@ -78,5 +78,5 @@ def test_adjoint_refuses_debug(gpu_ctx):
     b = ds.buffers.bindings(sc.ubo)
     lib, h = gpu_ctx.lib, gpu_ctx.handle
     rc = lib.rt_trace_rays_adjoint(h, ds.tlas.handle, C.byref(b), C.c_void_p(grad.data_ptr()), sc.width, sc.height,
-                                   None, None)
+                                   None, None, None, None)
     assert rc == -6 and b"debug" in lib.rt_last_error(h)
