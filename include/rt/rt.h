@@ -387,6 +387,21 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
                   uint8_t* out_rgba8, uint32_t width, uint32_t height, uint32_t mode,
                   rt_trace_stats* stats);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * during a trace (tight serving loops, harnesses that forbid hipMalloc; it reads
+ * queue lengths back and synchronises, so it cannot itself be stream-captured).  Same arguments and the same image as
+ * rt_trace_rays(..., RT_MODE_WAVEFRONT, NULL), but calls no hipMalloc / hipFree:
+ * the path state and the texture descriptors live in one 64 MiB device array per
+ * GPU, reserved when the library is loaded and kept until the process ends (its
+ * address is looked up by rt_create).  Frames larger than the array's slots are
+ * traced in passes of whole 8x8 tiles.  Limits: at most 1024 textures
+ * (RT_ERR_INVALID_ARGUMENT beyond), no stats, no stage timing, no graph replay,
+ * ubo->debug == 1 is RT_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several contexts or streams) race.
+ * Synchronises the context's stream. */
+int rt_trace_rays_noalloc(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
+                          uint8_t* out_rgba8, uint32_t width, uint32_t height);
+
 /* ---------------------------------------------------- ray queries (parity) */
 
 /* Ray flags (gl_RayFlags*EXT values). */

@@ -331,6 +331,8 @@ int rt_create(rt_ctx** out, int hip_device, void* hip_stream) {
     if (e == hipSuccess) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
     if (e == hipSuccess) ctx->d_band = dev_alloc<uint32_t>(8, &e);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    // the device address of the no-allocation scratch array (rt_trace_rays_noalloc)
+    if (e == hipSuccess) e = hipError_t(wavefront_noalloc_init(hip_device));
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) {
         float lut[256];
@@ -878,6 +880,50 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     }
     if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
     if (stats) return read_stats(ctx, stats);
+    return RT_OK;
+}
+
+int rt_trace_rays_noalloc(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum, uint8_t* out_rgba8,
+                          uint32_t width, uint32_t height) {
+    if (!ctx) return fail(nullptr, RT_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (b && b->num_textures > kNoAllocMaxTextures)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT,
+                    "rt_trace_rays_noalloc takes at most " + std::to_string(kNoAllocMaxTextures) + " textures");
+    int rc = validate_bindings(ctx, b);
+    if (rc) return rc;
+    if (!accum) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "accum is NULL");
+    if (width == 0 || height == 0 || uint64_t(width) * height > (1ull << 31))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "width/height out of range");
+    if (out_rgba8 && (reinterpret_cast<uintptr_t>(out_rgba8) & 3u))
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "out_rgba8 must be 4-byte aligned");
+    if (out_rgba8 && b->ubo->total_number_of_samples == 0)
+        return fail(ctx, RT_ERR_INVALID_ARGUMENT, "the display image needs total_number_of_samples > 0");
+    if (b->ubo->debug == 1u)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "the debug == 1 materials need rt_trace_rays (counting kernels)");
+    auto it = ctx->tlas.find(tlas);
+    if (it == ctx->tlas.end()) return fail(ctx, RT_ERR_INVALID_HANDLE, "unknown TLAS handle");
+    DeviceGuard g(ctx->device);
+    FrameArgs fa;
+    make_frame(ctx, it->second, b, accum, width, height, &fa);
+    fa.sv.textures = nullptr;  // the descriptors go to the scratch array
+    fa.o.out = reinterpret_cast<uint32_t*>(out_rgba8);
+    fa.o.bgra = ctx->out_bgra ? 1u : 0u;
+    const bool any_hit = !fa.fully_opaque;
+    hipStream_t st = ctx->stream;
+    if (out_rgba8 && b->ubo->mapping == RT_MAPPING_HEAT) {
+        // one thread per pixel, no scratch (see rt_trace_rays); the descriptors still
+        // come from the scratch array
+        std::string err;
+        rc = launch_wavefront_noalloc(ctx->device, fa, b->textures, st, any_hit, ctx->variant, ctx->refill_ext,
+                                      ctx->refill_sh, ctx->finish, &err, /*megakernel=*/true);
+        if (rc) return fail(ctx, RT_ERR_HIP, "noalloc: " + err);
+        return RT_OK;
+    }
+    std::string err;
+    rc = launch_wavefront_noalloc(ctx->device, fa, b->textures, st, any_hit, ctx->variant, ctx->refill_ext,
+                                  ctx->refill_sh, ctx->finish, &err, false);
+    if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "noalloc: " + err);
+    if (rc) return fail(ctx, RT_ERR_HIP, "noalloc: " + err + ": " + hipGetErrorString(hipError_t(rc)));
     return RT_OK;
 }
 

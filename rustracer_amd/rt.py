@@ -38,7 +38,8 @@ EXPORTS = ["rt_create", "rt_destroy", "rt_last_error", "rt_sync", "rt_version", 
            "rt_build_blas", "rt_blas_device_address", "rt_destroy_blas", "rt_build_tlas", "rt_destroy_tlas",
            "rt_trace_rays", "rt_trace_ray_queries", "rt_trace_primary", "rt_blas_info", "rt_tlas_info",
            "rt_set_option", "rt_get_stage_times", "rt_tlas_flat_info", "rt_trace_rays_adjoint",
-           "rt_trace_rays_batch", "rt_autotune", "rt_skin_vertices", "rt_refit_blas_batch", "rt_refit_tlas"]
+           "rt_trace_rays_batch", "rt_autotune", "rt_skin_vertices", "rt_refit_blas_batch", "rt_refit_tlas",
+           "rt_trace_rays_noalloc"]
 
 
 class RtError(RuntimeError):
@@ -139,6 +140,7 @@ def load_library(path=LIB_PATH):
         "rt_blas_device_address": (i, [vp, u64, C.POINTER(u64)]), "rt_destroy_blas": (None, [vp, u64]),
         "rt_build_tlas": (i, [vp, vp, u, C.POINTER(u64)]), "rt_destroy_tlas": (None, [vp, u64]),
         "rt_trace_rays": (i, [vp, u64, vp, vp, vp, u, u, u, vp]),
+        "rt_trace_rays_noalloc": (i, [vp, u64, vp, vp, vp, u, u]),
         "rt_skin_vertices": (i, [vp, vp, vp, u, vp, u]),
         "rt_refit_blas_batch": (i, [vp, vp, u, C.POINTER(u64)]),
         "rt_refit_tlas": (i, [vp, u64]),
@@ -421,6 +423,17 @@ def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERN
     if st is None:
         return None
     return {n: getattr(st, n) for n, _ in TraceStats._fields_}
+
+
+def trace_rays_noalloc(ctx, tlas, buffers, ubo, accum, width, height, out=None):
+    """rt_trace_rays_noalloc: the wavefront trace of trace_rays with no device
+    allocation (path state in the library's per-GPU scratch array).  Same image as
+    trace_rays(..., mode=MODE_WAVEFRONT)."""
+    b = buffers.bindings(ubo)
+    if out is not None and (out.dtype != _torch().uint8 or not out.is_contiguous() or out.numel() != width * height * 4):
+        raise ValueError("out must be a contiguous uint8 tensor of width * height * 4 elements")
+    ctx.check(ctx.lib.rt_trace_rays_noalloc(ctx.handle, tlas.handle, C.byref(b), C.c_void_p(accum.data_ptr()),
+                                            C.c_void_p(out.data_ptr()) if out is not None else None, width, height))
 
 
 def trace_rays_batch(ctx, items, mode=MODE_WAVEFRONT, stats=False):

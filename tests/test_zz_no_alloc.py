@@ -1,0 +1,68 @@
+"""rt_trace_rays_noalloc: the wavefront trace with no device allocation (path state
+in the library's per-GPU scratch array).  Its images must equal those of
+rt_trace_rays(RT_MODE_WAVEFRONT) bit for bit, on two calls in a row (the second
+smaller, so it runs over what the first left in the array), including a frame
+larger than one pass of the array's slots."""
+import numpy as np
+import pytest
+import torch
+
+from rustracer_amd import layouts as L
+from rustracer_amd import rt
+from rustracer_amd import scenes as S
+from tests import gpu_kit as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(ctx, ds, ubo, w, h, out=False):
+    a_ref = rt.new_accumulation(w, h)
+    a_new = torch.full((h, w, 4), float("nan"), dtype=torch.float32, device="cuda")
+    o_ref = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda") if out else None
+    o_new = torch.full((h, w, 4), 7, dtype=torch.uint8, device="cuda") if out else None
+    rt.trace_rays(ctx, ds.tlas, ds.buffers, ubo, a_ref, w, h, mode=rt.MODE_WAVEFRONT, out=o_ref)
+    ctx.sync()
+    rt.trace_rays_noalloc(ctx, ds.tlas, ds.buffers, ubo, a_new, w, h, out=o_new)
+    ctx.sync()
+    return a_ref.cpu().numpy(), a_new.cpu().numpy(), (o_ref.cpu().numpy() if out else None), \
+        (o_new.cpu().numpy() if out else None)
+
+
+def test_noalloc_equals_wavefront_on_two_calls(gpu_ctx):
+    # call 1: 512x512 textured Cornell box (262,144 pixels: more than one pass)
+    ds = G.device_scene(gpu_ctx, "noalloc_params", factory=lambda: S.cornell_params(width=512, height=512, spp=4))
+    sc = ds.scene
+    print("noalloc call 1:", sc.width, "x", sc.height, "spp", sc.ubo["number_of_samples"], "textures",
+          len(sc.textures), flush=True)
+    r, n, ro, no = _both(gpu_ctx, ds, sc.ubo, sc.width, sc.height, out=True)
+    assert np.isfinite(n).all()
+    assert np.array_equal(r, n)
+    assert np.array_equal(ro, no)
+    assert float(np.abs(r[..., :3]).sum()) > 0.0
+    # call 2: a smaller frame of another scene, ragged against the 8x8 tiles
+    ds2 = G.device_scene(gpu_ctx, "noalloc_cornell", factory=lambda: S.cornell_sphere(width=93, height=61, spp=2,
+                                                                                      bounces=3))
+    sc2 = ds2.scene
+    print("noalloc call 2:", sc2.width, "x", sc2.height, flush=True)
+    r2, n2, _, _ = _both(gpu_ctx, ds2, sc2.ubo, sc2.width, sc2.height)
+    assert np.isfinite(n2).all()
+    assert np.array_equal(r2, n2)
+
+
+def test_noalloc_rejects_too_many_textures(gpu_ctx):
+    ds = G.device_scene(gpu_ctx, "noalloc_cornell", factory=lambda: S.cornell_sphere(width=93, height=61, spp=2,
+                                                                                     bounces=3))
+    sc = ds.scene
+    b = ds.buffers.bindings(sc.ubo)
+    # 1025 valid descriptors (copies of one small texture): only the limit rejects them
+    texel = torch.ones(4 * 4 * 4, dtype=torch.float32, device="cuda")
+    tex = (rt.Texture * 1025)()
+    for k in range(1025):
+        tex[k] = rt.Texture(data=texel.data_ptr(), width=4, height=4, format=L.FMT_RGBA32F)
+    b.textures = rt.C.cast(tex, rt.C.c_void_p)
+    b.num_textures = 1025
+    acc = rt.new_accumulation(sc.width, sc.height)
+    rc = gpu_ctx.lib.rt_trace_rays_noalloc(gpu_ctx.handle, ds.tlas.handle, rt.C.byref(b), rt.C.c_void_p(acc.data_ptr()),
+                                           None, sc.width, sc.height)
+    assert rc == -1
+    assert b"at most 1024 textures" in gpu_ctx.lib.rt_last_error(gpu_ctx.handle)
