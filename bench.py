@@ -497,7 +497,15 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else int(traffic), "kernel": kernel_name,
                      "launch_ms": round(launch_ms, 4), "algorithmic_bytes": int(bytes_per_launch),
-                     "stage_ms": {k: round(v, 3) for k, v in stage.items()}},
+                     "stage_ms": {k: round(v, 3) for k, v in stage.items()},
+                     # where the figures come from: launch_ms and achieved from HIP events in
+                     # the stage-timed frames after the timed ones (one slot set, no finishing
+                     # launch: every traversal runs in these launches); traffic from the
+                     # rocprofv3 FETCH_SIZE / WRITE_SIZE profile of the same frame shape
+                     "measured_in": "stage-timed frames of this run: one slot set, no finishing launch",
+                     "traffic_source": None if traffic is None else
+                     "profiles/k_wf_trace_displaced_sphere_pmc.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                     "stage-timed frames, scaled by a copy kernel)"},
     }
     if not args.no_dynamic:
         line["config"]["dynamic"] = dynamic_step(ctx, sc, buf, blases, 3)
