@@ -153,6 +153,8 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* c) {
 struct OneFrame {
     FrameArgs fa;
     uint32_t tile_step = 1, tile_first = 0;  // the frame's tiles this slot set holds
+    __device__ uint32_t frame_of(uint32_t) const { return 0u; }
+    __device__ const FrameArgs& frame(uint32_t) const { return fa; }
     __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
         slot_pixel(fa, s, tile_step, tile_first, x, y);
         return fa;
@@ -164,12 +166,17 @@ struct ManyFrames {
     const uint32_t* base;  // n + 1 slot offsets
     uint32_t n;
     unsigned long long* cnt;
-    __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
+    __device__ uint32_t frame_of(uint32_t s) const {
         uint32_t lo = 0, hi = n;
         while (hi - lo > 1) {
             uint32_t m = (lo + hi) >> 1;
             if (base[m] <= s) lo = m; else hi = m;
         }
+        return lo;
+    }
+    __device__ const FrameArgs& frame(uint32_t f) const { return fas[f]; }
+    __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
+        const uint32_t lo = frame_of(s);
         const FrameArgs& fa = fas[lo];
         slot_pixel(fa, s - base[lo], 1u, 0u, x, y);
         return fa;
@@ -258,7 +265,7 @@ void k_wf_trace(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_e_in
     PathCounters pc;
     TraceStats ts;
     Traversal<kAnyHit, kStats, kDepth> tr;
-    uint32_t s = 0, x = 0, y = 0;
+    uint32_t s = 0, x = 0, y = 0, f = 0;  // f: the slot's frame (a batch's frames)
     Rng4 rng{0u, 0u, 0u, 0u};
     bool active = false, exhausted = false, shadow = false;
     for (;;) {
@@ -268,6 +275,7 @@ void k_wf_trace(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_e_in
                 shadow = i >= n_e;
                 s = shadow ? (qs[i - n_e] & ~kRetireBit) : q[i];
                 const FrameArgs& fa = fr.at(s, x, y);
+                f = fr.frame_of(s);
                 const uint32_t opq = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
                 if (shadow) {
                     // castShadowRay, rchit:35-59: tMin 0.1, tMax the light distance
@@ -290,8 +298,7 @@ void k_wf_trace(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_e_in
         const uint32_t keep = exhausted ? 1u : a.refill_ext;
         do {
             if (active) {
-                uint32_t xx, yy;
-                const FrameArgs& fa = fr.at(s, xx, yy);
+                const FrameArgs& fa = fr.frame(f);  // found once per ray, not per step
                 AlphaAnyHit ah{&fa.sv, &fa.tlas, &rng};
                 if (tr.step(fa.tlas, RayFromSlotOrShadow{a, s, shadow}, ah, stack, kWfBlock, ts)) {
                     if (shadow) {
