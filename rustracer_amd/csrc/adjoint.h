@@ -20,8 +20,9 @@ struct AdjointOut {
 
 // Accumulates into `out` the gradient of sum(grad_accum * accum) over the frame `fa`
 // describes.  `table_bounces`: contributions of vertices at bounces below this are
-// summed per wave in LDS before their atomics (see adjoint.hip).
+// summed per wave in LDS before their atomics (see adjoint.hip).  `next_pixel`: one
+// device counter the persistent kernel takes pixels from (zeroed on `st` here).
 int launch_adjoint(const FrameArgs& fa, const float* grad_accum, const AdjointOut& out, uint32_t table_bounces,
-                   hipStream_t st, bool any_hit, bool stats);
+                   hipStream_t st, bool any_hit, bool stats, uint32_t* next_pixel);
 
 }  // namespace rt

@@ -6,7 +6,7 @@
 //
 // Path replay backpropagation (Vicini, Speierer & Jakob, SIGGRAPH 2021): nothing of
 // a path is stored between its two passes but its radiance and one bit per bounce.
-// One lane per pixel, the megakernel's tiling; per sample:
+// One lane per pixel at a time (persistent lanes taking pixels in turn); per sample:
 //   pass 1  the forward path (traversal, shading, shadow rays) gives the sample's
 //           radiance L and the shadow results (one bit per bounce);
 //   pass 2  the same path traced and shaded again from the same RNG state.  At
@@ -36,6 +36,8 @@
 // Follows the raygen loop of crates/examples/gltf_viewer/shaders/RayTracing.rgen
 // (:85-128), the closest-hit shading of RayTracing.rchit (:136-416, NEE :338-350)
 // and the miss shader RayTracing.rmiss (:14-45).
+#include <algorithm>
+
 #include "adjoint.h"
 #include "brdf_dual.h"
 #include "path.h"
@@ -274,37 +276,68 @@ __device__ inline void miss_light_grads(const SceneView& sv, const AdjView& av, 
     }
 }
 
+// Persistent: as many workgroups as can be resident; each lane takes a pixel, runs
+// its samples one per loop turn and then takes the next pixel (one atomic per wave
+// on `next_pixel` for the lanes that need one; pixels numbered 8x8 tile by tile, so
+// the lanes a wave serves together sit in one or two tiles).  A wave no longer
+// lasts as long as its slowest pixel.  Each pixel's samples stay in order in one
+// lane, so the gradients are the same up to the order of the float sums.
 template <bool kAnyHit, bool kStats>
-__global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av) {
+__global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av, uint32_t* next_pixel) {
     __shared__ uint32_t s_stack[kStackDepth * kAdjBlock];
     __shared__ unsigned long long s_keys[kAdjWaves][kSlots];
     __shared__ float4 s_vals[kAdjWaves][kSlots];
     __shared__ uint32_t s_used[kAdjWaves];
-    const uint32_t tiles_x = (fa.width + 15) / 16;
-    const uint32_t bid = blockIdx.x;
-    const uint32_t tx = bid % tiles_x, ty = bid / tiles_x;
+    const uint32_t tiles_x = (fa.width + 7) / 8;
+    const uint32_t npix = tiles_x * ((fa.height + 7) / 8) * 64u;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t x = tx * 16 + wave * 8 + (lane & 7);
-    const uint32_t y = ty * 8 + (lane >> 3);
     WaveTable tab{s_keys[wave], s_vals[wave], &s_used[wave], lane};
     tab.clear();
     PathCounters pc;
     GradCounts gc;
-    // every lane stays to the end (the table is drained by the whole wave); lanes
-    // outside the image or with a zero pixel gradient do no work
-    D3 g = d3(0.0);
-    if (x < fa.width && y < fa.height) {
-        const float4 g4 = av.grad_accum[size_t(y) * fa.width + x];
-        g = d3(g4.x, g4.y, g4.z);
-    }
-    const bool live = (g.x != 0.0 || g.y != 0.0 || g.z != 0.0) && fa.bounces != 0u;
     uint32_t* stack = s_stack + threadIdx.x;
     const uint32_t flags = fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE;
     const bool dl_grads = av.out.grad_dlight != nullptr && fa.sv.num_dlights != 0u;
+    // the lane's pixel, its next sample and its path state between samples
+    bool have = false, exhausted = false;
+    uint32_t x = 0, y = 0, s = 0;
+    D3 g = d3(0.0);
     PathState p;
-    init_path(fa, x, y, p);
-    for (uint32_t s = 0; s < fa.samples; s++) {
-        if (live) {
+    for (;;) {
+        if (!exhausted) {
+            // lanes without a pixel take the next ones: one atomic per wave
+            const uint64_t m = __ballot(!have);
+            if (m) {
+                const uint32_t first = __ffsll((unsigned long long)m) - 1;
+                uint32_t base = 0;
+                if (lane == first) base = atomicAdd(next_pixel, uint32_t(__popcll(m)));
+                base = __shfl(base, first, 64);
+                if (base + uint32_t(__popcll(m)) >= npix) exhausted = true;
+                const uint32_t i = base + uint32_t(__popcll(m & lanemask_lt()));
+                if (!have && i < npix) {
+                    const uint32_t tile = i >> 6, within = i & 63u;
+                    x = (tile % tiles_x) * 8 + (within & 7u);
+                    y = (tile / tiles_x) * 8 + (within >> 3);
+                    g = d3(0.0);
+                    if (x < fa.width && y < fa.height) {
+                        const float4 g4 = av.grad_accum[size_t(y) * fa.width + x];
+                        g = d3(g4.x, g4.y, g4.z);
+                    }
+                    // a pixel outside the image or with a zero gradient does no work
+                    have = (g.x != 0.0 || g.y != 0.0 || g.z != 0.0) && fa.bounces != 0u && fa.samples != 0u;
+                    if (have) {
+                        init_path(fa, x, y, p);
+                        s = 0;
+                    }
+                }
+            }
+        }
+        if (!__ballot(have)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (have) {
+            // sample s of the lane's pixel
             p.sample = s;
             begin_sample(fa, x, y, p);
             const PathState p0 = p;
@@ -349,8 +382,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av)
                 if (!found && k > 0u && dl_grads) miss_light_grads(fa.sv, av, dir, g * tap.thr, tab, gc);
                 tap.l_rem = tap.l_rem - (tap.unocc ? r.pend_with : r.pend_without);
             }
+            if (++s == fa.samples) have = false;
         }
-        // a converged point: drain the table before it fills
+        // the whole wave: drain the table before it fills
         if (tab.occupancy() > uint32_t(kSlots / 2)) gc.atomics += tab.drain();
     }
     gc.atomics += tab.drain();
@@ -368,16 +402,26 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(FrameArgs fa, AdjView av)
 }  // namespace
 
 int launch_adjoint(const FrameArgs& fa, const float* grad_accum, const AdjointOut& out, uint32_t table_bounces,
-                   hipStream_t st, bool any_hit, bool stats) {
-    uint32_t tiles = ((fa.width + 15) / 16) * ((fa.height + 7) / 8);
+                   hipStream_t st, bool any_hit, bool stats, uint32_t* next_pixel) {
+    const uint32_t tiles = ((fa.width + 7) / 8) * ((fa.height + 7) / 8);
     if (tiles == 0) return 0;
     AdjView av{reinterpret_cast<const float4*>(grad_accum), out, table_bounces};
+    const void* f = any_hit ? (stats ? (const void*)k_adjoint<true, true> : (const void*)k_adjoint<true, false>)
+                            : (stats ? (const void*)k_adjoint<false, true> : (const void*)k_adjoint<false, false>);
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kAdjBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    // resident workgroups, no more than it takes to start every 8x8 tile at once
+    const uint32_t g = std::min(uint32_t(cus) * uint32_t(per_cu), (tiles + kAdjWaves - 1) / kAdjWaves);
+    hipError_t e = hipMemsetAsync(next_pixel, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) return int(e);
     if (stats) {
-        if (any_hit) hipLaunchKernelGGL((k_adjoint<true, true>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
-        else hipLaunchKernelGGL((k_adjoint<false, true>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+        if (any_hit) hipLaunchKernelGGL((k_adjoint<true, true>), dim3(g), dim3(kAdjBlock), 0, st, fa, av, next_pixel);
+        else hipLaunchKernelGGL((k_adjoint<false, true>), dim3(g), dim3(kAdjBlock), 0, st, fa, av, next_pixel);
     } else {
-        if (any_hit) hipLaunchKernelGGL((k_adjoint<true, false>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
-        else hipLaunchKernelGGL((k_adjoint<false, false>), dim3(tiles), dim3(kAdjBlock), 0, st, fa, av);
+        if (any_hit) hipLaunchKernelGGL((k_adjoint<true, false>), dim3(g), dim3(kAdjBlock), 0, st, fa, av, next_pixel);
+        else hipLaunchKernelGGL((k_adjoint<false, false>), dim3(g), dim3(kAdjBlock), 0, st, fa, av, next_pixel);
     }
     return int(hipGetLastError());
 }

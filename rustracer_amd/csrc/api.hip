@@ -1056,7 +1056,8 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     }
     AdjointOut out{d_ptrs, nt, b->num_plights ? d_grad_point_intensity : nullptr,
                    b->num_dlights ? d_grad_directional_intensity : nullptr};
-    rc = launch_adjoint(fa, d_grad_accum, out, kAdjointTableBounces, st, !fa.fully_opaque, stats != nullptr);
+    rc = launch_adjoint(fa, d_grad_accum, out, kAdjointTableBounces, st, !fa.fully_opaque, stats != nullptr,
+                        ctx->d_band);
     if (rc) return hip_fail(ctx, hipError_t(rc), "adjoint launch");
     if (stats) return read_stats(ctx, stats);
     return RT_OK;
