@@ -166,14 +166,8 @@ struct ManyFrames {
     const uint32_t* base;  // n + 1 slot offsets
     uint32_t n;
     unsigned long long* cnt;
-    __device__ uint32_t frame_of(uint32_t s) const {
-        uint32_t lo = 0, hi = n;
-        while (hi - lo > 1) {
-            uint32_t m = (lo + hi) >> 1;
-            if (base[m] <= s) lo = m; else hi = m;
-        }
-        return lo;
-    }
+    const uint32_t* slot_frame;  // frame of every slot (k_slot_frames)
+    __device__ uint32_t frame_of(uint32_t s) const { return slot_frame[s]; }
     __device__ const FrameArgs& frame(uint32_t f) const { return fas[f]; }
     __device__ const FrameArgs& at(uint32_t s, uint32_t& x, uint32_t& y) const {
         const uint32_t lo = frame_of(s);
@@ -505,6 +499,18 @@ WfArrays slice(const WfArrays& a, uint32_t base, uint32_t* counts) {
     r.sq_d += base;
     r.counts = counts;
     return r;
+}
+
+// The frame of every slot of a batch's slot set: the frame i with base[i] <= s < base[i + 1].
+__global__ void k_slot_frames(const uint32_t* base, uint32_t n, uint32_t nslots, uint32_t* out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (base[m] <= s) lo = m; else hi = m;
+    }
+    out[s] = lo;
 }
 
 __global__ void k_zero_counts(uint32_t* c) {
@@ -906,20 +912,23 @@ int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t
         set_off.push_back(set_base.size());
         for (uint32_t i = cut[g]; i <= cut[g + 1]; i++) set_base.push_back(base[i] - base[cut[g]]);
     }
-    const size_t fbytes = sizeof(FrameArgs) * count, bbytes = sizeof(uint32_t) * set_base.size();
-    if (fbytes + bbytes > ws->batch_cap) {
+    const size_t fbytes = (sizeof(FrameArgs) * count + 255) & ~size_t(255);
+    const size_t bbytes = (sizeof(uint32_t) * set_base.size() + 255) & ~size_t(255);
+    const size_t sbytes = sizeof(uint32_t) * size_t(total);
+    if (fbytes + bbytes + sbytes > ws->batch_cap) {
         if (ws->batch_mem) (void)hipFree(ws->batch_mem);
         ws->batch_mem = nullptr;
         ws->batch_cap = 0;
-        hipError_t e = hipMalloc(&ws->batch_mem, fbytes + bbytes + 256);
+        hipError_t e = hipMalloc(&ws->batch_mem, fbytes + bbytes + sbytes);
         if (e != hipSuccess) {
             *err = "cannot allocate the batch table";
             return int(e);
         }
-        ws->batch_cap = fbytes + bbytes;
+        ws->batch_cap = fbytes + bbytes + sbytes;
     }
     FrameArgs* d_frames = reinterpret_cast<FrameArgs*>(ws->batch_mem);
-    uint32_t* d_base = reinterpret_cast<uint32_t*>(ws->batch_mem + ((fbytes + 255) & ~size_t(255)));
+    uint32_t* d_base = reinterpret_cast<uint32_t*>(ws->batch_mem + fbytes);
+    uint32_t* d_slot_frame = reinterpret_cast<uint32_t*>(ws->batch_mem + fbytes + bbytes);
     // frames with no samples or bounces own no slots: the image reset still applies
     hipError_t e = hipMemcpyAsync(d_frames, frames, fbytes, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_base, set_base.data(), bbytes, hipMemcpyHostToDevice, st);
@@ -931,8 +940,18 @@ int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t
     for (uint32_t g = 0; g < nsets; g++) {
         const uint32_t f0 = cut[g], nf = cut[g + 1] - cut[g];
         const uint32_t first = base[f0], n = base[cut[g + 1]] - first;
-        sets.push_back(SlotSet<ManyFrames>{ManyFrames{d_frames + f0, d_base + set_off[g], nf, frames[0].counters},
-                                           slice(ws->a, first, ws->a.counts + 8 * g), n});
+        if (n) {
+            hipLaunchKernelGGL(k_slot_frames, dim3((n + 255) / 256), dim3(256), 0, st, d_base + set_off[g], nf, n,
+                               d_slot_frame + first);
+            e = hipGetLastError();
+            if (e != hipSuccess) {
+                *err = "launch k_slot_frames";
+                return int(e);
+            }
+        }
+        sets.push_back(SlotSet<ManyFrames>{
+            ManyFrames{d_frames + f0, d_base + set_off[g], nf, frames[0].counters, d_slot_frame + first},
+            slice(ws->a, first, ws->a.counts + 8 * g), n});
     }
     rc = run_wavefront(ws, sets, steps, st, any_hit, stats, variant, tm, err);
     // the host arrays the copies read must outlive them
