@@ -41,13 +41,18 @@ def test_traffic_only_for_the_profiled_variant():
 
 
 def test_trace_bytes_match_the_quoted_counts():
-    # counts quoted for the stage-timed 1080p / 64 spp frame: 2.30 G node visits,
-    # 432 M triangle tests, 173.5 M camera + extension rays, 40.4 M shadow rays, no
-    # instance entries (flattened), 248 iterations -> about 744 MB per launch
-    st = {"node_visits": 2.30e9, "tri_tests": 432e6, "instance_visits": 0, "primary_rays": 132.7e6,
-          "extension_rays": 173.5e6 - 132.7e6, "shadow_rays": 40.4e6}
+    # the counting frame of the 1080p / 64 spp workload (SAH-collapsed BVH): 2,291,441,269
+    # node visits, 360,892,479 triangle tests, 132,710,400 camera and 40,813,025 extension
+    # rays, 40,433,654 shadow rays, no instance entries (flattened), 248 iterations ->
+    # 727,137,615 B per launch, the `roofline.algorithmic_bytes` those counts gave
+    st = {"node_visits": 2291441269, "tri_tests": 360892479, "instance_visits": 0, "primary_rays": 132710400,
+          "extension_rays": 40813025, "shadow_rays": 40433654}
     per_launch = bench.trace_bytes(st) / 248
-    assert per_launch == pytest.approx(744e6, rel=5e-3)
+    assert int(per_launch) == 727137615
+    # by hand: 64 B per node, 48 B per triangle test, 84 B per camera or extension ray,
+    # 44 B per shadow ray
+    assert per_launch * 248 == (2291441269 * 64 + 360892479 * 48 + (132710400 + 40813025) * 84
+                                + 40433654 * 44)
     # units: one node is 64 B, one triangle 48 B, an extension ray 84 B, a shadow ray 44 B
     assert bench.trace_bytes({"node_visits": 1, "tri_tests": 0, "instance_visits": 0, "primary_rays": 0,
                               "extension_rays": 0, "shadow_rays": 0}) == 64
