@@ -122,7 +122,7 @@ struct rt_ctx {
     uint32_t leaf_max = kBlasLeafMax;  // RT_OPTION_BLAS_LEAF_SIZE
     uint32_t collapse = kCollapseSah;  // RT_OPTION_BVH_COLLAPSE
     bool mega_persistent = true;       // RT_OPTION_MEGAKERNEL_PERSISTENT
-    uint32_t* d_band = nullptr;        // pixel counters of the persistent megakernel (8 bands)
+    uint32_t* d_next_pixel = nullptr;  // pixel counter of the persistent megakernel and adjoint
     uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
     bool stage_timing = false;
     StageTimer timer;  // events of the last trace when stage_timing is set
@@ -329,7 +329,7 @@ int rt_create(rt_ctx** out, int hip_device, void* hip_stream) {
     if (e == hipSuccess) ctx->d_srgb = dev_alloc<float>(256, &e);
     if (e == hipSuccess) ctx->d_err = dev_alloc<uint32_t>(1, &e);
     if (e == hipSuccess) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
-    if (e == hipSuccess) ctx->d_band = dev_alloc<uint32_t>(8, &e);
+    if (e == hipSuccess) ctx->d_next_pixel = dev_alloc<uint32_t>(1, &e);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     // the device address of the no-allocation scratch array (rt_trace_rays_noalloc)
     if (e == hipSuccess) e = hipError_t(wavefront_noalloc_init(hip_device));
@@ -363,7 +363,7 @@ void rt_destroy(rt_ctx* ctx) {
     if (ctx->d_grad_ptrs) (void)hipFree(ctx->d_grad_ptrs);
     if (ctx->d_batch_tex) (void)hipFree(ctx->d_batch_tex);
     if (ctx->d_err) (void)hipFree(ctx->d_err);
-    if (ctx->d_band) (void)hipFree(ctx->d_band);
+    if (ctx->d_next_pixel) (void)hipFree(ctx->d_next_pixel);
     if (ctx->d_textures) (void)hipFree(ctx->d_textures);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -863,7 +863,7 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
     StageTimer* tm = ctx->stage_timing ? &ctx->timer : nullptr;
     ctx->timer.used = 0;
     if (mode == RT_MODE_MEGAKERNEL) {
-        rc = launch_megakernel(fa, st, any_hit, counting, tm, ctx->mega_persistent ? ctx->d_band : nullptr);
+        rc = launch_megakernel(fa, st, any_hit, counting, tm, ctx->mega_persistent ? ctx->d_next_pixel : nullptr);
     } else {
         if (!ctx->wavefront) {
             ctx->wavefront = wavefront_create();
@@ -984,7 +984,7 @@ int rt_trace_rays_batch(rt_ctx* ctx, const rt_batch_item* items, uint32_t count,
     if (mode == RT_MODE_MEGAKERNEL) {
         for (uint32_t i = 0; i < count && rc == 0; i++)
             rc = launch_megakernel(frames[i], st, !frames[i].fully_opaque, counting, tm,
-                                   ctx->mega_persistent ? ctx->d_band : nullptr);
+                                   ctx->mega_persistent ? ctx->d_next_pixel : nullptr);
         if (rc) return hip_fail(ctx, hipError_t(rc), "trace launch");
     } else {
         if (!ctx->wavefront) {
@@ -1057,7 +1057,7 @@ int rt_trace_rays_adjoint(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, const
     AdjointOut out{d_ptrs, nt, b->num_plights ? d_grad_point_intensity : nullptr,
                    b->num_dlights ? d_grad_directional_intensity : nullptr};
     rc = launch_adjoint(fa, d_grad_accum, out, kAdjointTableBounces, st, !fa.fully_opaque, stats != nullptr,
-                        ctx->d_band);
+                        ctx->d_next_pixel);
     if (rc) return hip_fail(ctx, hipError_t(rc), "adjoint launch");
     if (stats) return read_stats(ctx, stats);
     return RT_OK;

@@ -56,10 +56,10 @@ struct StageTimer {
     }
 };
 
-// band_next: 8 device counters for the persistent form (NULL: one workgroup per
-// 16x8 pixel tile, one pixel per lane).
+// next_pixel: one device counter the persistent form takes pixels from, zeroed on
+// `st` here (NULL: one workgroup per 16x8 pixel tile, one pixel per lane).
 int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool stats, StageTimer* timer = nullptr,
-                      uint32_t* band_next = nullptr);
+                      uint32_t* next_pixel = nullptr);
 int launch_ray_queries(const TlasView& tl, const SceneView& sv, const rt_ray* rays, uint32_t n, uint32_t flags,
                        rt_hit* hits, uint32_t seed_b, hipStream_t st, bool any_hit);
 int launch_primary(const FrameArgs& fa, rt_hit* hits, hipStream_t st, bool any_hit);
