@@ -1000,7 +1000,10 @@ namespace {
 uint32_t noalloc_slots() {
     const size_t room = kNoAllocBytes - ((kNoAllocTexBytes + 255) & ~size_t(255));
     WfArrays probe{};
-    uint32_t n = uint32_t(room / 292u) & ~63u;
+    // bytes per slot of the layout (its arrays are padded to 256 B, so measured on a
+    // large count), then down to a count that fits with the padding
+    const size_t per_slot = layout(probe, nullptr, 1u << 20) >> 20;
+    uint32_t n = uint32_t(room / per_slot) & ~63u;
     while (n > 64u && layout(probe, nullptr, n) > room) n -= 64u;
     return n;
 }
