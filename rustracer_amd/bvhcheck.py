@@ -27,31 +27,55 @@ def check_blases(ctx, blases, expect_prims=None):
     out = [None] * len(blases)
     for members in pools.values():
         info0 = infos[members[0]]
-        nodes = rt.read_nodes(ctx, info0)
+        pool = _Pool(rt.read_nodes(ctx, info0))
         n_total = max(infos[k].first_prim + infos[k].num_prims for k in members)
         tris = np.frombuffer(rt.read_device(info0.tris, 48 * max(1, n_total)), TRI)[:n_total] if n_total else np.zeros(0, TRI)
         for k in members:
             info = infos[k]
-            out[k] = _check(nodes, info.root, info.first_prim, info.num_prims, tris, "triangle",
+            out[k] = _check(pool, info.root, info.first_prim, info.num_prims, tris, "triangle",
                             None if expect_prims is None else expect_prims[k])
     return out
 
 
-def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
+class _Pool:
+    """The decoded nodes of one node pool, and the per-node scratch of a walk (visit
+    counts, path boxes), which each walk resets on the nodes it reached so that the
+    BLASes of one pool are walked without re-decoding or re-allocating the pool."""
+
+    def __init__(self, nodes):
+        self.n = n = len(nodes)
+        self.child = nodes["child"].astype(np.int64)
+        self.count = nodes["count"].astype(np.int64)
+        self.lo = np.stack([nodes["lo_x"], nodes["lo_y"], nodes["lo_z"]], -1)  # (n, 4, 3)
+        self.hi = np.stack([nodes["hi_x"], nodes["hi_y"], nodes["hi_z"]], -1)
+        self.visits = np.zeros(n, np.int64)
+        # intersection of the decoded boxes on the path to each reached node
+        self.path_lo = np.full((n, 3), -np.inf, np.float32)
+        self.path_hi = np.full((n, 3), np.inf, np.float32)
+
+    def reset(self, reached):
+        self.visits[reached] = 0
+        self.path_lo[reached] = -np.inf
+        self.path_hi[reached] = np.inf
+
+
+def _check(pool, root, first_prim, num_prims, tris, kind, expect_prims=None):
     """Level-by-level walk from `root` over all nodes of a level at once."""
-    errors = []
-    n = len(nodes)
+    n = pool.n
     if root >= n:
         return [f"node index {root} out of range {n}"]
-    child = nodes["child"].astype(np.int64)
-    count = nodes["count"].astype(np.int64)
-    lo = np.stack([nodes["lo_x"], nodes["lo_y"], nodes["lo_z"]], -1)  # (n, 4, 3)
-    hi = np.stack([nodes["hi_x"], nodes["hi_y"], nodes["hi_z"]], -1)
-    visits = np.zeros(n, np.int64)
+    reached = [np.array([root], np.int64)]
+    try:
+        return _walk(pool, root, first_prim, num_prims, tris, kind, expect_prims, reached)
+    finally:
+        pool.reset(np.concatenate(reached))
+
+
+def _walk(pool, root, first_prim, num_prims, tris, kind, expect_prims, reached):
+    errors = []
+    n, child, count, lo, hi = pool.n, pool.child, pool.count, pool.lo, pool.hi
+    visits, path_lo, path_hi = pool.visits, pool.path_lo, pool.path_hi
     visits[root] = 1
-    # intersection of the decoded boxes on the path to each reached node
-    path_lo = np.full((n, 3), -np.inf, np.float32)
-    path_hi = np.full((n, 3), np.inf, np.float32)
     frontier = np.array([root], np.int64)
     leaf_first, leaf_cnt, leaf_lo, leaf_hi = [], [], [], []
     while frontier.size:
@@ -69,6 +93,7 @@ def _check(nodes, root, first_prim, num_prims, tris, kind, expect_prims=None):
             ci, ilo, ihi = ci[keep], blo[internal][keep], bhi[internal][keep]
         else:
             ilo, ihi = blo[internal], bhi[internal]
+        reached.append(ci)
         np.add.at(visits, ci, 1)
         if np.any(visits[ci] > 1):
             errors.append(f"{int(np.sum(visits[ci] > 1))} nodes reached twice")
@@ -122,7 +147,7 @@ def check_tlas(ctx, tlas, num_instances):
     instances: every instance is reached exactly once, through one or the other."""
     info = tlas.info()
     nodes = rt.read_nodes(ctx, info)
-    errs = _check(nodes, info.root, 0, info.num_prims, None, "instance")
+    errs = _check(_Pool(nodes), info.root, 0, info.num_prims, None, "instance")
     inst = np.frombuffer(rt.read_device(info.tris, INST.itemsize * max(1, num_instances)), INST)[:num_instances]
     if num_instances and not np.array_equal(np.sort(inst["instance_id"]), np.arange(num_instances)):
         errs.append("instance ids are not a permutation")
@@ -134,7 +159,7 @@ def check_tlas(ctx, tlas, num_instances):
         return errs
     fnodes = rt.read_nodes(ctx, fi)
     ftris = np.frombuffer(rt.read_device(fi.tris, 48 * fi.num_prims), TRI)
-    errs += _check(fnodes, fi.root, 0, fi.num_prims, ftris, "flat")
+    errs += _check(_Pool(fnodes), fi.root, 0, fi.num_prims, ftris, "flat")
     bits = ftris["v1"][:, 3].copy().view("<u4")
     ii = bits & ~np.uint32(FLAT_OPAQUE)
     if np.any(ii < nt) or np.any(ii >= num_instances):

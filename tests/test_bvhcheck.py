@@ -39,37 +39,61 @@ def _tree():
     return nodes, _tris(5)
 
 
+def _chk(nodes, *args):
+    return V._check(V._Pool(nodes), *args)
+
+
 def test_valid_tree_passes():
     nodes, tris = _tree()
-    assert V._check(nodes, 0, 0, 5, tris, "triangle") == []
+    assert _chk(nodes, 0, 0, 5, tris, "triangle") == []
 
 
 def test_triangle_outside_an_ancestor_box_is_reported():
     nodes, tris = _tree()
     # the leaf box still contains triangle 2, the root's box of node 1 no longer does
     nodes[0]["hi_x"][0] = 2.4
-    assert any("not inside every box" in e for e in V._check(nodes, 0, 0, 5, tris, "triangle"))
+    assert any("not inside every box" in e for e in _chk(nodes, 0, 0, 5, tris, "triangle"))
 
 
 def test_record_in_two_leaves_and_missing_record():
     nodes, tris = _tree()
     nodes[1]["child"][1] = 1  # leaf [1,2) instead of [2,3): record 1 twice, record 2 never
-    errs = V._check(nodes, 0, 0, 5, tris, "triangle")
+    errs = _chk(nodes, 0, 0, 5, tris, "triangle")
     assert any("not in exactly one leaf" in e for e in errs)
 
 
 def test_node_reached_twice_and_out_of_range_child():
     nodes, tris = _tree()
     nodes[0]["child"][1], nodes[0]["count"][1] = 1, 0
-    assert any("reached twice" in e for e in V._check(nodes, 0, 0, 5, tris, "triangle"))
+    assert any("reached twice" in e for e in _chk(nodes, 0, 0, 5, tris, "triangle"))
     nodes, tris = _tree()
     nodes[0]["child"][0] = 7
-    assert any("out of range" in e for e in V._check(nodes, 0, 0, 5, tris, "triangle"))
+    assert any("out of range" in e for e in _chk(nodes, 0, 0, 5, tris, "triangle"))
 
 
 def test_inverted_box_and_foreign_leaf():
     nodes, tris = _tree()
     nodes[1]["lo_y"][0] = 2.0
-    assert any("inverted" in e for e in V._check(nodes, 0, 0, 5, tris, "triangle"))
+    assert any("inverted" in e for e in _chk(nodes, 0, 0, 5, tris, "triangle"))
     nodes, tris = _tree()
-    assert any("leaves outside" in e for e in V._check(nodes, 0, 1, 4, tris, "triangle"))
+    assert any("leaves outside" in e for e in _chk(nodes, 0, 1, 4, tris, "triangle"))
+
+
+def test_walks_of_one_pool_do_not_see_each_other():
+    """Two BLASes in one node pool (as a batched build leaves them), walked in turn
+    with the pool decoded once: a node that the first walk reached twice leaves no
+    visit count or path box behind for the second walk or for a repeat of the first."""
+    a, ta = _tree()
+    b, tb = _tree()
+    a[0]["child"][1], a[0]["count"][1] = 1, 0  # first BLAS: node 1 reached twice
+    real = b["child"] != E
+    b["child"][real & (b["count"] == 0)] += 2   # second BLAS: nodes 2, 3
+    b["child"][real & (b["count"] > 0)] += 5    # and records 5..9
+    nodes, tris = np.concatenate([a, b]), np.concatenate([ta, tb])
+    tris["prim"][5:] = np.arange(5)
+    pool = V._Pool(nodes)
+    first = V._check(pool, 0, 0, 5, tris, "triangle")
+    assert any("reached twice" in e for e in first)
+    assert V._check(pool, 2, 5, 5, tris, "triangle") == []
+    assert V._check(pool, 0, 0, 5, tris, "triangle") == first
+    assert not pool.visits.any() and np.all(pool.path_lo == -np.inf) and np.all(pool.path_hi == np.inf)
