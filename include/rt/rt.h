@@ -333,7 +333,7 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * (the analogue of the reference's GPU timestamp query pool,
  * crates/libs/vulkan/src/query.rs:8-73).  The megakernel reports all of its time
  * as extend_ms.  Synchronises the context stream. */
-/* Autotuner: traces the frame (the bindings' UBO with at most 8 samples per pixel)
+/* Autotuner: traces the frame (the bindings' UBO with at most 64 samples per pixel)
  * in wavefront mode and tunes, one after another, the traversal variant (stack
  * depth, register target), the number of concurrent slot sets, the live-path
  * count that hands the tail to one finishing launch, the refill threshold,
