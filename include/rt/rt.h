@@ -322,9 +322,10 @@ enum {
      * crates/libs/vulkan/src/ray_tracing/acceleration_structure.rs:105-118). */
     RT_OPTION_BVH_COLLAPSE = 12,
     /* The megakernel mode: 1 (default) a grid of as many workgroups as can be
-     * resident, each lane taking the next pixel (8x8 tiles within 8 horizontal
-     * bands of the image) when its pixel's samples are done, so no wave waits on its
-     * slowest pixel; 0 one lane per pixel in 16x8-pixel workgroups.  Same images. */
+     * resident, each lane taking the next pixel from one counter (pixels numbered
+     * 8x8 tile by tile, tiles in image order) when its pixel's samples are done, so
+     * no wave waits on its slowest pixel; 0 one lane per pixel in 16x8-pixel
+     * workgroups.  Same images. */
     RT_OPTION_MEGAKERNEL_PERSISTENT = 13
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
