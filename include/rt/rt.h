@@ -287,7 +287,7 @@ enum {
     RT_OPTION_GRAPH = 4,
     /* The wavefront traversal kernel fetches rays dynamically: a wave takes new
      * rays from the queue once fewer than this many of its 64 lanes are still
-     * traversing (1 = only when all have finished).  Same images.  Default 40.
+     * traversing (1 = only when all have finished).  Same images.  Default 16.
      * RT_OPTION_SHADOW_REFILL (1 .. 64) is accepted and has no effect: shadow rays
      * are fetched with the extension rays. */
     RT_OPTION_EXTEND_REFILL = 5,
@@ -335,12 +335,12 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * crates/libs/vulkan/src/query.rs:8-73).  The megakernel reports all of its time
  * as extend_ms.  Synchronises the context stream. */
 /* Autotuner: traces the frame (the bindings' UBO with at most 64 samples per pixel)
- * in wavefront mode and tunes, one after another, the traversal variant (stack
- * depth, register target), the number of concurrent slot sets, the live-path
- * count that hands the tail to one finishing launch, the refill threshold,
- * graph replay, and the traversal variant once more under those settings,
- * keeping the fastest settings on this context (the RT_OPTION_* values).  Each
- * probe times the faster of two frames after a warm-up frame, and a setting
+ * in wavefront mode and tunes, one after another, the refill threshold, the
+ * traversal variant (stack depth, register target), the number of concurrent slot
+ * sets, the live-path count that hands the tail to one finishing launch, graph
+ * replay, and the traversal variant once more under those settings, keeping the
+ * fastest settings on this context (the RT_OPTION_* values).  Each probe times the
+ * fastest of three frames after a warm-up frame, and a setting
  * replaces the one it is compared with only when it is more than 1.5 % faster.
  * A variant whose stack ring gives up on a ray of this frame (restart budget) is
  * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */

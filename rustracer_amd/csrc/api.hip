@@ -121,7 +121,7 @@ struct rt_ctx {
     uint32_t grad_cap = 0;
     int variant = 0;      // RT_OPTION_TRAVERSAL_VARIANT
     bool use_graph = false;  // RT_OPTION_GRAPH
-    uint32_t refill_ext = 40, refill_sh = 1;  // RT_OPTION_EXTEND_REFILL / RT_OPTION_SHADOW_REFILL
+    uint32_t refill_ext = 16, refill_sh = 1;  // RT_OPTION_EXTEND_REFILL / RT_OPTION_SHADOW_REFILL
     bool out_bgra = false;   // RT_OPTION_OUTPUT_BGRA
     int streams = 2;         // RT_OPTION_WAVEFRONT_STREAMS
     uint32_t leaf_max = kBlasLeafMax;  // RT_OPTION_BLAS_LEAF_SIZE
@@ -1100,14 +1100,14 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
     const int keep_streams = ctx->streams;
     const uint32_t keep_finish = ctx->finish;
-    // one probe: the faster of two timed frames of the current settings (after a
+    // one probe: the fastest of three timed frames of the current settings (after a
     // warm-up frame), or < 0 when a traversal gave up
     auto probe = [&](double& ms_out) -> int {
         int r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
         if (r) return r;
         hipError_t er = hipSuccess;
         float best_ms = -1.f;
-        for (int rep = 0; rep < 2 && er == hipSuccess; rep++) {
+        for (int rep = 0; rep < 3 && er == hipSuccess; rep++) {
             er = hipEventRecord(ctx->ev0, ctx->stream);
             if (er == hipSuccess)
                 r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT,
@@ -1130,44 +1130,11 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     // frame-to-frame noise, so repeated tunings of one scene choose alike
     double best = -1.0;
     auto consider = [&](double ms) { return ms >= 0.0 && (best < 0.0 || ms < best * (1.0 - kAutotuneMargin)); };
-    // 1. traversal variant (graph off, current refills)
     ctx->use_graph = false;
-    int best_v = -1;
-    for (int v = 0; v < wavefront_num_variants() && rc == RT_OK; v++) {
-        ctx->variant = v;
-        double ms = -1.0;
-        rc = probe(ms);
-        if (rc == RT_OK && consider(ms)) { best = ms; best_v = v; }
-    }
-    if (rc == RT_OK && best_v < 0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
-    if (rc == RT_OK) ctx->variant = best_v;
-    // 2. concurrent slot sets
-    if (rc == RT_OK) {
-        int chosen = ctx->streams;
-        for (int n = 1; n <= wavefront_max_streams() && rc == RT_OK; n++) {
-            if (n == chosen) continue;
-            ctx->streams = n;
-            double ms = -1.0;
-            rc = probe(ms);
-            if (rc == RT_OK && consider(ms)) { best = ms; chosen = n; }
-        }
-        ctx->streams = chosen;
-    }
-    // 3. live-path count that hands the tail to the finishing launch
-    if (rc == RT_OK) {
-        const uint32_t finish[] = {0u, 1u << 14, 1u << 16, 1u << 18};
-        uint32_t chosen = ctx->finish;
-        for (uint32_t f : finish) {
-            if (f == chosen) continue;
-            ctx->finish = f;
-            double ms = -1.0;
-            rc = probe(ms);
-            if (rc) break;
-            if (consider(ms)) { best = ms; chosen = f; }
-        }
-        ctx->finish = chosen;
-    }
-    // 4. refill threshold of the traversal kernel
+    // 1. refill threshold of the traversal kernel (the default variant; the other
+    //    settings are tuned under the refill chosen here), against the frame of the
+    //    settings the context has
+    rc = probe(best);
     if (rc == RT_OK) {
         const uint32_t refills[] = {1u, 16u, 32u, 48u, 64u};
         uint32_t chosen = ctx->refill_ext;
@@ -1180,6 +1147,45 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
             if (consider(ms)) { best = ms; chosen = r; }
         }
         ctx->refill_ext = chosen;
+    }
+    // 2. traversal variant (graph off): the refill probes' variant stands if its frames
+    //    completed
+    const int incumbent = ctx->variant;
+    int best_v = best >= 0.0 ? incumbent : -1;
+    for (int v = 0; v < wavefront_num_variants() && rc == RT_OK; v++) {
+        if (v == incumbent && best_v >= 0) continue;
+        ctx->variant = v;
+        double ms = -1.0;
+        rc = probe(ms);
+        if (rc == RT_OK && consider(ms)) { best = ms; best_v = v; }
+    }
+    if (rc == RT_OK && best_v < 0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
+    if (rc == RT_OK) ctx->variant = best_v;
+    // 3. concurrent slot sets
+    if (rc == RT_OK) {
+        int chosen = ctx->streams;
+        for (int n = 1; n <= wavefront_max_streams() && rc == RT_OK; n++) {
+            if (n == chosen) continue;
+            ctx->streams = n;
+            double ms = -1.0;
+            rc = probe(ms);
+            if (rc == RT_OK && consider(ms)) { best = ms; chosen = n; }
+        }
+        ctx->streams = chosen;
+    }
+    // 4. live-path count that hands the tail to the finishing launch
+    if (rc == RT_OK) {
+        const uint32_t finish[] = {0u, 1u << 14, 1u << 16, 1u << 17, 1u << 18};
+        uint32_t chosen = ctx->finish;
+        for (uint32_t f : finish) {
+            if (f == chosen) continue;
+            ctx->finish = f;
+            double ms = -1.0;
+            rc = probe(ms);
+            if (rc) break;
+            if (consider(ms)) { best = ms; chosen = f; }
+        }
+        ctx->finish = chosen;
     }
     // 5. graph replay
     if (rc == RT_OK) {

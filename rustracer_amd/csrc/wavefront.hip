@@ -414,43 +414,163 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, co
 
 // The tail of a trace: once few paths are alive, each iteration's launches last as
 // long as their slowest ray while most of the GPU idles.  This kernel takes, after a
-// shade, the paths of path queue q and the retiring slots of shadow queue qs, and
-// runs each one, in one thread, through its pending shadow ray and the rest of its
-// pixel's samples (the megakernel's loop from the slot's state), then writes the
-// pixel.
+// shade, the paths of path queue q and the retiring slots of shadow queue qs and runs
+// each one, in one lane, through its pending shadow ray and the rest of its pixel's
+// samples (the megakernel's loop from the slot's state), then writes the pixel.
+// A lane's traversals are stepped, not run to completion: the wave steps the lanes
+// that trace while at least half of its lanes that hold a pixel do, then the others
+// shade together; a lane that finishes its pixel takes the next queue entry (one
+// atomic per wave, the trace cursor).  No lane waits each bounce for the slowest
+// traversal of its wave, and each lane's operations are those of the loop, in order.
 constexpr int kFinishBlock = 128;
+// A finishing lane's state: no pixel, tracing the path's ray, tracing a shadow ray,
+// shading, taking the next ray / sample / pixel, writing a retiring pixel.
+enum : uint32_t { kFinIdle = 0, kFinExt, kFinShadow, kFinShade, kFinNext, kFinWrite };
+enum : uint32_t { kAfterExt = 0, kAfterDone, kAfterNext };  // what follows a shadow ray
+
+// The ray of a finishing lane's traversal: the path's fp64 ray, or the fp32 shadow ray.
+struct FinishRay {
+    const PathState& p;
+    float3 so, sd;
+    bool shadow;
+    __device__ void get(D3& O, D3& Dd) const {
+        if (shadow) {
+            O = d3(so);
+            Dd = d3(sd);
+        } else {
+            O = p.o;
+            Dd = p.d;
+        }
+    }
+};
+
 template <class Frames, bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kFinishBlock) void k_wf_finish(Frames fr, WfArrays a, const uint32_t* q,
                                                             const uint32_t* n_e_in, const uint32_t* qs,
                                                             const uint32_t* n_s_in) {
     __shared__ uint32_t s_stack[kStackDepth * kFinishBlock];
+    uint32_t* const stack = s_stack + threadIdx.x;
     const uint32_t n_e = *n_e_in, n = n_e + *n_s_in;
     PathCounters pc;
-    for (uint32_t i = blockIdx.x * kFinishBlock + threadIdx.x; i < n; i += gridDim.x * kFinishBlock) {
-        const uint32_t entry = i < n_e ? q[i] : qs[i - n_e];
-        if (i >= n_e && !(entry & kRetireBit)) continue;  // a path queue entry as well
-        const bool retire = i >= n_e;
-        const uint32_t s = entry & ~kRetireBit;
-        const uint32_t f = a.flags[s];
-        uint32_t x, y;
-        const FrameArgs& fa = fr.at(s, x, y);
-        PathState p;
-        load_full(a, s, p);
-        if (f & kFlagShadow) {
-            // the last shaded bounce's shadow ray, as the trace kernel would take it
-            const float4 o = a.sq_o[s], d = a.sq_d[s];
-            BounceOut r;
-            r.so = d3(make_float3(o.x, o.y, o.z));
-            r.sd = d3(make_float3(d.x, d.y, d.z));
-            r.sdist = double(o.w);
-            r.shadow_wb = a.sq_wb[s];
-            const bool unocc = shadow_unoccluded<kAnyHit, kStats>(fa, x, y, r, s_stack + threadIdx.x, kFinishBlock, pc);
-            p.rad = p.rad + (unocc ? d3(a.pwx[s], a.pwy[s], a.pwz[s]) : d3(a.pox[s], a.poy[s], a.poz[s]));
+    TraceStats ts;
+    Traversal<kAnyHit, kStats, kStackDepth> tr;
+    PathState p;
+    BounceOut r;  // the last shaded bounce: its radiance terms while its shadow ray is traced
+    r.cont = false;
+    HitInfo h;
+    bool found = false, exhausted = false;
+    Rng4 rng{0u, 0u, 0u, 0u};
+    float3 so = make_float3(0.f, 0.f, 0.f), sd = so;
+    uint32_t s = 0, x = 0, y = 0, f = 0, state = kFinIdle, after = kAfterNext;
+    auto start_ext = [&](const FrameArgs& fa) {
+        if (kStats) { if (p.bounce == 0u) pc.primary++; else pc.extension++; }
+        rng = Rng4{x, y, fa.seed_b, p.wb};
+        tr.begin(fa.tlas, to_f3(p.o), to_f3(p.d), float(kTMin), float(p.tfar),
+                 fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE, stack, kFinishBlock);
+        state = kFinExt;
+    };
+    auto start_shadow = [&](const FrameArgs& fa, float3 o, float3 d, float tmax, uint32_t wb, uint32_t then) {
+        // castShadowRay, rchit:35-59, as shadow_unoccluded traces it
+        if (kStats) pc.shadow++;
+        so = o;
+        sd = d;
+        rng = Rng4{x, y, fa.seed_b, wb};
+        tr.begin(fa.tlas, o, d, float(kShadowTMin), tmax,
+                 kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u), stack, kFinishBlock);
+        after = then;
+        state = kFinShadow;
+    };
+    for (;;) {
+        // lanes without a pixel take the next entry of q, then of qs (its retiring ones)
+        if (!exhausted) {
+            const uint32_t i = wave_fetch(state == kFinIdle, a.counts + kCursor, n, exhausted);
+            if (i != kEmpty) {
+                const uint32_t entry = i < n_e ? q[i] : qs[i - n_e];
+                if (i < n_e || (entry & kRetireBit)) {  // a shadow entry without the bit has a path entry too
+                    s = entry & ~kRetireBit;
+                    const FrameArgs& fa = fr.at(s, x, y);
+                    f = fr.frame_of(s);
+                    load_full(a, s, p);
+                    p.last_t = (fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE) ? a.last_t[s] : 0.f;
+                    const uint32_t then = i >= n_e ? kAfterDone : kAfterExt;
+                    if (a.flags[s] & kFlagShadow) {
+                        // the last shaded bounce's shadow ray, as the trace kernel would take it
+                        const float4 o = a.sq_o[s], d = a.sq_d[s];
+                        r.pend_with = d3(a.pwx[s], a.pwy[s], a.pwz[s]);
+                        r.pend_without = d3(a.pox[s], a.poy[s], a.poz[s]);
+                        start_shadow(fa, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, a.sq_wb[s], then);
+                    } else if (then == kAfterExt) {
+                        start_ext(fa);
+                    } else {
+                        write_accum(fa, x, y, p.rad, p.last_t);
+                    }
+                }
+            }
         }
-        const bool dist = fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE;
-        p.last_t = dist ? a.last_t[s] : 0.f;
-        if (!retire) run_samples<kAnyHit, kStats>(fa, x, y, p, true, s_stack + threadIdx.x, kFinishBlock, pc);
-        write_accum(fa, x, y, p.rad, p.last_t);
+        const uint64_t holding = __ballot(state != kFinIdle);
+        if (!holding && exhausted) break;
+        // the lanes that trace step while at least half of the lanes holding a pixel do
+        const uint32_t keep = (uint32_t(__popcll(holding)) + 1u) / 2u;
+        while (uint32_t(__popcll(__ballot(state == kFinExt || state == kFinShadow))) >= keep && keep) {
+            if (state == kFinExt || state == kFinShadow) {
+                const FrameArgs& fa = fr.frame(f);
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rng};
+                const bool shadow = state == kFinShadow;
+                if (tr.step(fa.tlas, FinishRay{p, so, sd, shadow}, ah, stack, kFinishBlock, ts)) {
+                    if (kStats) {
+                        TraceStats& to = shadow ? pc.trs : pc.tr;
+                        to.nodes += ts.nodes;
+                        to.tris += ts.tris;
+                        to.instances += ts.instances;
+                        to.anyhits += ts.anyhits;
+                        to.overflows += ts.overflows;
+                        ts = TraceStats{};
+                    }
+                    if (shadow) {
+                        p.rad = p.rad + (!tr.found ? r.pend_with : r.pend_without);
+                        if (after == kAfterExt) r.cont = true;  // the slot's current ray comes next
+                        state = after == kAfterDone ? kFinWrite : kFinNext;
+                    } else {
+                        p.wb = rng.w;
+                        found = tr.found;
+                        h = tr.hit;
+                        state = kFinShade;
+                    }
+                }
+            }
+        }
+        // the others: shading, then the next ray, sample or pixel
+        if (state == kFinShade) {
+            const FrameArgs& fa = fr.frame(f);
+            shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
+            if (r.shadow)
+                start_shadow(fa, to_f3(r.so), to_f3(r.sd), float(r.sdist), r.shadow_wb, kAfterNext);
+            else {
+                p.rad = p.rad + r.pend_without;
+                state = kFinNext;
+            }
+        }
+        if (state == kFinWrite) {
+            const FrameArgs& fa = fr.frame(f);
+            write_accum(fa, x, y, p.rad, p.last_t);
+            state = kFinIdle;
+        }
+        if (state == kFinNext) {
+            const FrameArgs& fa = fr.frame(f);
+            if (r.cont) {
+                start_ext(fa);
+            } else {
+                state = kFinIdle;
+                while (++p.sample < fa.samples) {
+                    begin_sample(fa, x, y, p);
+                    if (fa.bounces) {
+                        start_ext(fa);
+                        break;
+                    }
+                }
+                if (state == kFinIdle) write_accum(fa, x, y, p.rad, p.last_t);
+            }
+        }
     }
     if (kStats) flush_counters(fr.counters(), pc);
 }
@@ -552,7 +672,7 @@ struct WavefrontState {
     char* batch_mem = nullptr;  // frame arguments + slot offsets of a batch
     size_t batch_cap = 0;
     bool use_graph = false;     // replay captured chunks (RT_OPTION_GRAPH)
-    uint32_t refill_ext = 40, refill_sh = 1;
+    uint32_t refill_ext = 16, refill_sh = 1;
     int streams = 2;            // slot sets traced concurrently (RT_OPTION_WAVEFRONT_STREAMS)
     uint32_t finish_below = 0;  // live paths below which k_wf_finish ends a set (RT_OPTION_WAVEFRONT_FINISH)
     hipStream_t cap_stream = nullptr;

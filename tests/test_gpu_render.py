@@ -223,7 +223,7 @@ def test_traversal_variants_render_the_same_image(gpu_ctx, name):
             assert np.array_equal(ref.cpu().numpy(), img.cpu().numpy()), (re, rs)
     finally:
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
-        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
+        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 16)
         gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)
     with pytest.raises(rt.RtError):
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, rt.NUM_TRAVERSAL_VARIANTS)
@@ -268,7 +268,7 @@ def test_autotune_keeps_the_image(gpu_ctx):
     finally:
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
         gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
-        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
+        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 16)
         gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)
 
 
@@ -288,7 +288,7 @@ def test_autotune_on_a_deep_bvh_leaves_no_error(gpu_ctx):
     finally:
         gpu_ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, 0)
         gpu_ctx.set_option(rt.OPTION_GRAPH, 0)
-        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 40)
+        gpu_ctx.set_option(rt.OPTION_EXTEND_REFILL, 16)
         gpu_ctx.set_option(rt.OPTION_SHADOW_REFILL, 1)
 
 
