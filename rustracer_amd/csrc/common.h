@@ -190,6 +190,21 @@ __device__ inline float ord2f(uint32_t u) {
 
 __device__ inline uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
+// Work fetch of the persistent kernels: the lanes with `need` take the next indices
+// below n from a shared cursor, one atomic per wave.  Returns the lane's index, or
+// kEmpty (sets `exhausted`, uniformly over the wave, once the cursor reaches n).
+__device__ inline uint32_t wave_fetch(bool need, uint32_t* cursor, uint32_t n, bool& exhausted) {
+    uint64_t m = __ballot(need);
+    if (!m) return kEmpty;
+    uint32_t first = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == first) base = atomicAdd(cursor, uint32_t(__popcll(m)));
+    base = __shfl(base, first, 64);
+    if (base + uint32_t(__popcll(m)) >= n) exhausted = true;
+    uint32_t i = base + uint32_t(__popcll(m & lanemask_lt()));
+    return (need && i < n) ? i : kEmpty;
+}
+
 // Sum of a 64-lane wave, result in every lane.
 template <class T> __device__ inline T wave_sum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

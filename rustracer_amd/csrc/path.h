@@ -685,4 +685,137 @@ __device__ inline D3 render_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, u
     return p.rad;
 }
 
+// ------------------------------------------------------ pixels in stepped lanes
+// A lane of a persistent kernel that runs pixels' raygen loops (run_samples: the
+// same operations on the same values, in the same order) with its traversals
+// stepped rather than run to completion.  The kernel's waves alternate: step the
+// lanes that trace (path or shadow ray) while at least half of the lanes that hold
+// a pixel do, then the others advance together (shading, next ray, next sample), so
+// no lane waits each bounce for the slowest traversal of its wave.
+enum : uint32_t { kLaneIdle = 0, kLaneExt, kLaneShadow, kLaneShade, kLaneNext, kLaneDone };
+enum : uint32_t { kAfterExt = 0, kAfterDone, kAfterNext };  // what follows a shadow ray
+
+// The ray of a lane's traversal: the path's fp64 ray, or the fp32 shadow ray.
+struct LaneRay {
+    const PathState& p;
+    float3 so, sd;
+    bool shadow;
+    __device__ void get(D3& O, D3& Dd) const {
+        if (shadow) {
+            O = d3(so);
+            Dd = d3(sd);
+        } else {
+            O = p.o;
+            Dd = p.d;
+        }
+    }
+};
+
+template <bool kAnyHit, bool kStats>
+struct PathLane {
+    Traversal<kAnyHit, kStats, kStackDepth> tr;
+    TraceStats ts;
+    PathState p;
+    BounceOut r;  // the last shaded bounce: its radiance terms while its shadow ray is traced
+    HitInfo h;
+    Rng4 rng{0u, 0u, 0u, 0u};
+    float3 so{0.f, 0.f, 0.f}, sd{0.f, 0.f, 0.f};
+    uint32_t x = 0, y = 0, state = kLaneIdle, after = kAfterNext;
+    bool found = false;
+
+    __device__ bool tracing() const { return state == kLaneExt || state == kLaneShadow; }
+
+    // the path's ray (traceRayEXT of rgen:84-96)
+    __device__ void start_ext(const FrameArgs& fa, uint32_t* stack, uint32_t stride, PathCounters& pc) {
+        if (kStats) { if (p.bounce == 0u) pc.primary++; else pc.extension++; }
+        rng = Rng4{x, y, fa.seed_b, p.wb};
+        tr.begin(fa.tlas, to_f3(p.o), to_f3(p.d), float(kTMin), float(p.tfar),
+                 fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE, stack, stride);
+        state = kLaneExt;
+    }
+    // castShadowRay, rchit:35-59, as shadow_unoccluded traces it
+    __device__ void start_shadow(const FrameArgs& fa, float3 o, float3 d, float tmax, uint32_t wb, uint32_t then,
+                                 uint32_t* stack, uint32_t stride, PathCounters& pc) {
+        if (kStats) pc.shadow++;
+        so = o;
+        sd = d;
+        rng = Rng4{x, y, fa.seed_b, wb};
+        tr.begin(fa.tlas, o, d, float(kShadowTMin), tmax, kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u),
+                 stack, stride);
+        after = then;
+        state = kLaneShadow;
+    }
+    // A new pixel from its first sample (render_pixel); false when it has nothing to
+    // trace (state kLaneDone).
+    __device__ bool begin_pixel(const FrameArgs& fa, uint32_t px, uint32_t py, uint32_t* stack, uint32_t stride,
+                                PathCounters& pc) {
+        x = px;
+        y = py;
+        init_path(fa, x, y, p);
+        r.cont = false;
+        for (; p.sample < fa.samples; p.sample++) {
+            begin_sample(fa, x, y, p);
+            if (fa.bounces) {
+                start_ext(fa, stack, stride, pc);
+                return true;
+            }
+        }
+        state = kLaneDone;
+        return false;
+    }
+    // One traversal step of a tracing lane; at the traversal's end its result.
+    template <class AnyHitFn>
+    __device__ void step(const FrameArgs& fa, AnyHitFn& ah, uint32_t* stack, uint32_t stride, PathCounters& pc) {
+        const bool shadow = state == kLaneShadow;
+        if (!tr.step(fa.tlas, LaneRay{p, so, sd, shadow}, ah, stack, stride, ts)) return;
+        if (kStats) {
+            TraceStats& to = shadow ? pc.trs : pc.tr;
+            to.nodes += ts.nodes;
+            to.tris += ts.tris;
+            to.instances += ts.instances;
+            to.anyhits += ts.anyhits;
+            to.overflows += ts.overflows;
+            ts = TraceStats{};
+        }
+        if (shadow) {
+            p.rad = p.rad + (!tr.found ? r.pend_with : r.pend_without);
+            if (after == kAfterExt) r.cont = true;  // the path's current ray comes next
+            state = after == kAfterDone ? kLaneDone : kLaneNext;
+        } else {
+            p.wb = rng.w;
+            found = tr.found;
+            h = tr.hit;
+            state = kLaneShade;
+        }
+    }
+    // A lane that holds a pixel and does not trace: the shaders and the rest of the
+    // bounce, then the next ray or sample, until it traces again or its pixel is
+    // complete (state kLaneDone).
+    __device__ void advance(const FrameArgs& fa, uint32_t* stack, uint32_t stride, PathCounters& pc) {
+        if (state == kLaneShade) {
+            shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
+            if (r.shadow) {
+                start_shadow(fa, to_f3(r.so), to_f3(r.sd), float(r.sdist), r.shadow_wb, kAfterNext, stack, stride, pc);
+                return;
+            }
+            p.rad = p.rad + r.pend_without;
+            state = kLaneNext;
+        }
+        if (state == kLaneNext) {
+            if (r.cont) {
+                start_ext(fa, stack, stride, pc);
+                return;
+            }
+            state = kLaneDone;
+            while (++p.sample < fa.samples) {
+                begin_sample(fa, x, y, p);
+                if (fa.bounces) {
+                    start_ext(fa, stack, stride, pc);
+                    return;
+                }
+            }
+        }
+    }
+};
+
 }  // namespace rt

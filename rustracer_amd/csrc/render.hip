@@ -9,8 +9,9 @@
 //
 // Persistent form (default): a grid of as many workgroups as can be resident, each
 // lane taking its next pixel from one counter when its pixel is done (the pixel's
-// samples stay in order in one lane, so the image is the same).  A wave then no
-// longer lasts as long as its slowest pixel.  Pixels are numbered tile by tile
+// samples stay in order in one lane, so the image is the same), with its traversals
+// stepped so that a wave neither lasts as long as its slowest pixel nor waits each
+// bounce for its slowest traversal.  Pixels are numbered tile by tile
 // (8x8, tiles in image order), so the pixels in flight at any time form one band of
 // the image that every XCD works on: the caches the XCDs share hold the part of the
 // scene that band sees.  (One band per XCD, each XCD's workgroups taking their own
@@ -35,35 +36,72 @@ __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
     return (bid % nx) * per + bid / nx;
 }
 
-// One pixel of the megakernel: the raygen loop, then the accumulation and display
-// image writes (RayTracing.rgen:25-167).
+// The accumulation and display image writes of a finished pixel (RayTracing.rgen:
+// 132-167); `cycles` is the pixel's clock count for the HEAT mapping.
+__device__ inline void mega_write(const FrameArgs& fa, uint32_t x, uint32_t y, D3 radiance, float last_t,
+                                  uint64_t cycles) {
+    float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
+    float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
+    const float4 sum =
+        make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
+    *acc = sum;
+    if (fa.o.out) write_output(fa.o, fa.width, x, y, sum, last_t, cycles);
+}
+
+// One pixel of the megakernel: the raygen loop, then the writes (RayTracing.rgen:25-167).
 template <bool kAnyHit, bool kStats>
 __device__ inline void mega_pixel(const FrameArgs& fa, uint32_t x, uint32_t y, uint32_t* stack, PathCounters& pc) {
     const bool heat = fa.o.out && fa.o.mapping == RT_MAPPING_HEAT;
     const uint64_t t0 = heat ? uint64_t(clock64()) : 0u;  // clockARB(), rgen:27
     float last_t;
     D3 radiance = render_pixel<kAnyHit, kStats>(fa, x, y, stack, kMegaBlock, pc, last_t);
-    float4* acc = reinterpret_cast<float4*>(fa.accum) + size_t(y) * fa.width + x;
-    float4 prev = fa.reset ? make_float4(0.f, 0.f, 0.f, 0.f) : *acc;
-    const float4 sum =
-        make_float4(prev.x + float(radiance.x), prev.y + float(radiance.y), prev.z + float(radiance.z), 0.f);
-    *acc = sum;
-    if (fa.o.out) write_output(fa.o, fa.width, x, y, sum, last_t, heat ? uint64_t(clock64()) - t0 : 0u);
+    mega_write(fa, x, y, radiance, last_t, heat ? uint64_t(clock64()) - t0 : 0u);
 }
 
+// Persistent form: lanes take pixels from one counter (tile by tile, tiles in image
+// order) and run them as PathLanes, traversals stepped and the wave's shading done
+// together, so no lane waits each bounce for the slowest traversal of its wave; a
+// lane that finishes its pixel takes the next (one atomic per wave).
 template <bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kMegaBlock) void k_megakernel_persistent(FrameArgs fa, uint32_t* next_pixel) {
     __shared__ uint32_t s_stack[kStackDepth * kMegaBlock];
+    uint32_t* const stack = s_stack + threadIdx.x;
     const uint32_t tiles_x = (fa.width + 7) / 8;
     const uint32_t items = tiles_x * ((fa.height + 7) / 8) * 64u;
+    const bool heat = fa.o.out && fa.o.mapping == RT_MAPPING_HEAT;
     PathCounters pc;
+    PathLane<kAnyHit, kStats> ln;
+    uint64_t t0 = 0;  // clockARB() at the pixel's start, rgen:27
+    bool exhausted = false;
     for (;;) {
-        const uint32_t it = atomicAdd(next_pixel, 1u);
-        if (it >= items) break;
-        const uint32_t tile = it >> 6, within = it & 63u;
-        const uint32_t x = (tile % tiles_x) * 8 + (within & 7u);
-        const uint32_t y = (tile / tiles_x) * 8 + (within >> 3);
-        if (x < fa.width && y < fa.height) mega_pixel<kAnyHit, kStats>(fa, x, y, s_stack + threadIdx.x, pc);
+        if (!exhausted) {
+            const uint32_t it = wave_fetch(ln.state == kLaneIdle, next_pixel, items, exhausted);
+            if (it != kEmpty) {
+                const uint32_t tile = it >> 6, within = it & 63u;
+                const uint32_t x = (tile % tiles_x) * 8 + (within & 7u);
+                const uint32_t y = (tile / tiles_x) * 8 + (within >> 3);
+                if (x < fa.width && y < fa.height) {
+                    if (heat) t0 = uint64_t(clock64());
+                    ln.begin_pixel(fa, x, y, stack, kMegaBlock, pc);
+                }
+            }
+        }
+        const uint64_t holding = __ballot(ln.state != kLaneIdle);
+        if (!holding && exhausted) break;
+        const uint32_t keep = (uint32_t(__popcll(holding)) + 1u) / 2u;
+        while (keep && uint32_t(__popcll(__ballot(ln.tracing()))) >= keep) {
+            if (ln.tracing()) {
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &ln.rng};
+                ln.step(fa, ah, stack, kMegaBlock, pc);
+            }
+        }
+        if (ln.state != kLaneIdle && !ln.tracing()) {
+            ln.advance(fa, stack, kMegaBlock, pc);
+            if (ln.state == kLaneDone) {
+                mega_write(fa, ln.x, ln.y, ln.p.rad, ln.p.last_t, heat ? uint64_t(clock64()) - t0 : 0u);
+                ln.state = kLaneIdle;
+            }
+        }
     }
     if (kStats) flush_counters(fa.counters, pc);
 }

@@ -116,25 +116,6 @@ struct RayFromSlot {
     }
 };
 
-// Dynamic ray fetch for the traversal kernels: a lane whose ray has ended takes the
-// next queue entry (one atomic per wave); a wave keeps stepping its rays while at
-// least `refill` of its lanes are busy, then refills.  Traversal lengths vary by an
-// order of magnitude between rays, so a wave that only took new rays when all 64
-// had ended (refill 1) would idle many lanes; refilling often costs a ballot and an
-// atomic per refill.  Returns the queue index for the lane, or kEmpty (sets
-// `exhausted`, uniformly over the wave, once the queue has run out).
-__device__ inline uint32_t wave_fetch(bool need, uint32_t* cursor, uint32_t n, bool& exhausted) {
-    uint64_t m = __ballot(need);
-    if (!m) return kEmpty;
-    uint32_t first = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (__lane_id() == first) base = atomicAdd(cursor, uint32_t(__popcll(m)));
-    base = __shfl(base, first, 64);
-    if (base + uint32_t(__popcll(m)) >= n) exhausted = true;
-    uint32_t i = base + uint32_t(__popcll(m & lanemask_lt()));
-    return (need && i < n) ? i : kEmpty;
-}
-
 // Appends `item` to queue q (count c) for the lanes with want set: one atomic per wave.
 __device__ inline uint32_t wave_append(bool want, uint32_t* c) {
     uint64_t m = __ballot(want);
@@ -289,6 +270,9 @@ void k_wf_trace(Frames fr, WfArrays a, const uint32_t* q, const uint32_t* n_e_in
             }
         }
         if (!__ballot(active)) break;
+        // the wave steps its rays while at least refill_ext of its lanes are busy, then
+        // takes new rays: traversal lengths vary by an order of magnitude, so waiting
+        // for all 64 idles many lanes, and each refill costs a ballot and an atomic
         const uint32_t keep = exhausted ? 1u : a.refill_ext;
         do {
             if (active) {
@@ -416,33 +400,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(Frames fr, WfArrays a, co
 // long as their slowest ray while most of the GPU idles.  This kernel takes, after a
 // shade, the paths of path queue q and the retiring slots of shadow queue qs and runs
 // each one, in one lane, through its pending shadow ray and the rest of its pixel's
-// samples (the megakernel's loop from the slot's state), then writes the pixel.
-// A lane's traversals are stepped, not run to completion: the wave steps the lanes
-// that trace while at least half of its lanes that hold a pixel do, then the others
-// shade together; a lane that finishes its pixel takes the next queue entry (one
-// atomic per wave, the trace cursor).  No lane waits each bounce for the slowest
-// traversal of its wave, and each lane's operations are those of the loop, in order.
+// samples (the megakernel's loop from the slot's state), then writes the pixel.  Its
+// lanes are PathLanes: traversals stepped, the wave's shading done together, and a
+// lane that finishes its pixel takes the next queue entry (one atomic per wave, the
+// trace cursor, which the last shade cleared).
 constexpr int kFinishBlock = 128;
-// A finishing lane's state: no pixel, tracing the path's ray, tracing a shadow ray,
-// shading, taking the next ray / sample / pixel, writing a retiring pixel.
-enum : uint32_t { kFinIdle = 0, kFinExt, kFinShadow, kFinShade, kFinNext, kFinWrite };
-enum : uint32_t { kAfterExt = 0, kAfterDone, kAfterNext };  // what follows a shadow ray
-
-// The ray of a finishing lane's traversal: the path's fp64 ray, or the fp32 shadow ray.
-struct FinishRay {
-    const PathState& p;
-    float3 so, sd;
-    bool shadow;
-    __device__ void get(D3& O, D3& Dd) const {
-        if (shadow) {
-            O = d3(so);
-            Dd = d3(sd);
-        } else {
-            O = p.o;
-            Dd = p.d;
-        }
-    }
-};
 
 template <class Frames, bool kAnyHit, bool kStats>
 __global__ __launch_bounds__(kFinishBlock) void k_wf_finish(Frames fr, WfArrays a, const uint32_t* q,
@@ -452,123 +414,53 @@ __global__ __launch_bounds__(kFinishBlock) void k_wf_finish(Frames fr, WfArrays 
     uint32_t* const stack = s_stack + threadIdx.x;
     const uint32_t n_e = *n_e_in, n = n_e + *n_s_in;
     PathCounters pc;
-    TraceStats ts;
-    Traversal<kAnyHit, kStats, kStackDepth> tr;
-    PathState p;
-    BounceOut r;  // the last shaded bounce: its radiance terms while its shadow ray is traced
-    r.cont = false;
-    HitInfo h;
-    bool found = false, exhausted = false;
-    Rng4 rng{0u, 0u, 0u, 0u};
-    float3 so = make_float3(0.f, 0.f, 0.f), sd = so;
-    uint32_t s = 0, x = 0, y = 0, f = 0, state = kFinIdle, after = kAfterNext;
-    auto start_ext = [&](const FrameArgs& fa) {
-        if (kStats) { if (p.bounce == 0u) pc.primary++; else pc.extension++; }
-        rng = Rng4{x, y, fa.seed_b, p.wb};
-        tr.begin(fa.tlas, to_f3(p.o), to_f3(p.d), float(kTMin), float(p.tfar),
-                 fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : RT_RAY_FLAG_NONE, stack, kFinishBlock);
-        state = kFinExt;
-    };
-    auto start_shadow = [&](const FrameArgs& fa, float3 o, float3 d, float tmax, uint32_t wb, uint32_t then) {
-        // castShadowRay, rchit:35-59, as shadow_unoccluded traces it
-        if (kStats) pc.shadow++;
-        so = o;
-        sd = d;
-        rng = Rng4{x, y, fa.seed_b, wb};
-        tr.begin(fa.tlas, o, d, float(kShadowTMin), tmax,
-                 kShadowFlags | (fa.fully_opaque ? RT_RAY_FLAG_OPAQUE : 0u), stack, kFinishBlock);
-        after = then;
-        state = kFinShadow;
-    };
+    PathLane<kAnyHit, kStats> ln;
+    uint32_t f = 0;  // the lane's frame (a batch's frames)
+    bool exhausted = false;
     for (;;) {
         // lanes without a pixel take the next entry of q, then of qs (its retiring ones)
         if (!exhausted) {
-            const uint32_t i = wave_fetch(state == kFinIdle, a.counts + kCursor, n, exhausted);
+            const uint32_t i = wave_fetch(ln.state == kLaneIdle, a.counts + kCursor, n, exhausted);
             if (i != kEmpty) {
                 const uint32_t entry = i < n_e ? q[i] : qs[i - n_e];
                 if (i < n_e || (entry & kRetireBit)) {  // a shadow entry without the bit has a path entry too
-                    s = entry & ~kRetireBit;
-                    const FrameArgs& fa = fr.at(s, x, y);
+                    const uint32_t s = entry & ~kRetireBit;
+                    const FrameArgs& fa = fr.at(s, ln.x, ln.y);
                     f = fr.frame_of(s);
-                    load_full(a, s, p);
-                    p.last_t = (fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE) ? a.last_t[s] : 0.f;
+                    load_full(a, s, ln.p);
+                    ln.p.last_t = (fa.o.out && fa.o.mapping == RT_MAPPING_DISTANCE) ? a.last_t[s] : 0.f;
                     const uint32_t then = i >= n_e ? kAfterDone : kAfterExt;
                     if (a.flags[s] & kFlagShadow) {
                         // the last shaded bounce's shadow ray, as the trace kernel would take it
                         const float4 o = a.sq_o[s], d = a.sq_d[s];
-                        r.pend_with = d3(a.pwx[s], a.pwy[s], a.pwz[s]);
-                        r.pend_without = d3(a.pox[s], a.poy[s], a.poz[s]);
-                        start_shadow(fa, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, a.sq_wb[s], then);
+                        ln.r.pend_with = d3(a.pwx[s], a.pwy[s], a.pwz[s]);
+                        ln.r.pend_without = d3(a.pox[s], a.poy[s], a.poz[s]);
+                        ln.start_shadow(fa, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, a.sq_wb[s],
+                                        then, stack, kFinishBlock, pc);
                     } else if (then == kAfterExt) {
-                        start_ext(fa);
+                        ln.start_ext(fa, stack, kFinishBlock, pc);
                     } else {
-                        write_accum(fa, x, y, p.rad, p.last_t);
+                        ln.state = kLaneDone;
                     }
                 }
             }
         }
-        const uint64_t holding = __ballot(state != kFinIdle);
+        const uint64_t holding = __ballot(ln.state != kLaneIdle);
         if (!holding && exhausted) break;
-        // the lanes that trace step while at least half of the lanes holding a pixel do
         const uint32_t keep = (uint32_t(__popcll(holding)) + 1u) / 2u;
-        while (uint32_t(__popcll(__ballot(state == kFinExt || state == kFinShadow))) >= keep && keep) {
-            if (state == kFinExt || state == kFinShadow) {
+        while (keep && uint32_t(__popcll(__ballot(ln.tracing()))) >= keep) {
+            if (ln.tracing()) {
                 const FrameArgs& fa = fr.frame(f);
-                AlphaAnyHit ah{&fa.sv, &fa.tlas, &rng};
-                const bool shadow = state == kFinShadow;
-                if (tr.step(fa.tlas, FinishRay{p, so, sd, shadow}, ah, stack, kFinishBlock, ts)) {
-                    if (kStats) {
-                        TraceStats& to = shadow ? pc.trs : pc.tr;
-                        to.nodes += ts.nodes;
-                        to.tris += ts.tris;
-                        to.instances += ts.instances;
-                        to.anyhits += ts.anyhits;
-                        to.overflows += ts.overflows;
-                        ts = TraceStats{};
-                    }
-                    if (shadow) {
-                        p.rad = p.rad + (!tr.found ? r.pend_with : r.pend_without);
-                        if (after == kAfterExt) r.cont = true;  // the slot's current ray comes next
-                        state = after == kAfterDone ? kFinWrite : kFinNext;
-                    } else {
-                        p.wb = rng.w;
-                        found = tr.found;
-                        h = tr.hit;
-                        state = kFinShade;
-                    }
-                }
+                AlphaAnyHit ah{&fa.sv, &fa.tlas, &ln.rng};
+                ln.step(fa, ah, stack, kFinishBlock, pc);
             }
         }
-        // the others: shading, then the next ray, sample or pixel
-        if (state == kFinShade) {
+        if (ln.state != kLaneIdle && !ln.tracing()) {
             const FrameArgs& fa = fr.frame(f);
-            shade_bounce<kStats>(fa, x, y, p, found, h, r, pc);
-            if (r.shadow)
-                start_shadow(fa, to_f3(r.so), to_f3(r.sd), float(r.sdist), r.shadow_wb, kAfterNext);
-            else {
-                p.rad = p.rad + r.pend_without;
-                state = kFinNext;
-            }
-        }
-        if (state == kFinWrite) {
-            const FrameArgs& fa = fr.frame(f);
-            write_accum(fa, x, y, p.rad, p.last_t);
-            state = kFinIdle;
-        }
-        if (state == kFinNext) {
-            const FrameArgs& fa = fr.frame(f);
-            if (r.cont) {
-                start_ext(fa);
-            } else {
-                state = kFinIdle;
-                while (++p.sample < fa.samples) {
-                    begin_sample(fa, x, y, p);
-                    if (fa.bounces) {
-                        start_ext(fa);
-                        break;
-                    }
-                }
-                if (state == kFinIdle) write_accum(fa, x, y, p.rad, p.last_t);
+            ln.advance(fa, stack, kFinishBlock, pc);
+            if (ln.state == kLaneDone) {
+                write_accum(fa, ln.x, ln.y, ln.p.rad, ln.p.last_t);
+                ln.state = kLaneIdle;
             }
         }
     }
