@@ -303,10 +303,11 @@ enum {
      * Default 2. */
     RT_OPTION_WAVEFRONT_STREAMS = 8,
     /* Once fewer paths than this are alive in a slot set, the wavefront trace runs
-     * each of them to the end of its pixel's samples in one thread of one launch
-     * (the megakernel's loop from the path's state) instead of further queue
-     * iterations, whose launches would last as long as their slowest ray while most
-     * of the GPU idles.  0: never.  Same images.  Ignored while
+     * each of them to the end of its pixel's samples in one lane of one launch
+     * (the megakernel's loop from the path's state, traversals stepped as in the
+     * persistent megakernel) instead of further queue iterations, whose launches
+     * would last as long as their slowest ray while most of the GPU idles.
+     * 0: never.  Same images.  Ignored while
      * RT_OPTION_STAGE_TIMING is on.  Default 65536. */
     RT_OPTION_WAVEFRONT_FINISH = 9,
     /* Most triangles per BVH leaf (1 .. 16) of BLASes and flattened BVHs built
@@ -324,8 +325,10 @@ enum {
     /* The megakernel mode: 1 (default) a grid of as many workgroups as can be
      * resident, each lane taking the next pixel from one counter (pixels numbered
      * 8x8 tile by tile, tiles in image order) when its pixel's samples are done, so
-     * no wave waits on its slowest pixel; 0 one lane per pixel in 16x8-pixel
-     * workgroups.  Same images. */
+     * no wave waits on its slowest pixel, and stepping its traversals (the wave
+     * steps the lanes that trace, then shades the others together), so no lane
+     * waits each bounce for the slowest traversal of its wave; 0 one lane per pixel
+     * in 16x8-pixel workgroups, each traversal run to its end.  Same images. */
     RT_OPTION_MEGAKERNEL_PERSISTENT = 13
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
