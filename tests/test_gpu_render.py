@@ -317,3 +317,33 @@ def test_persistent_megakernel_equals_one_lane_per_pixel(gpu_ctx, name):
         out.append((acc.cpu().numpy(), disp.cpu().numpy(), st["rays"]))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("name", ["cornell_sphere", "textured_small"])
+@pytest.mark.parametrize("streams", [1, 3])
+def test_finishing_launch_over_most_of_the_frame_equals_one_lane_per_pixel(gpu_ctx, name, streams):
+    """A finishing threshold above the pixel count hands each slot set to k_wf_finish
+    after its first 8 iterations, so its stepped lanes take paths with and without a
+    pending shadow ray and retiring pixels, in opaque and alpha-tested scenes: the
+    image, its display image and the ray counts equal those of one lane per pixel
+    running each traversal to its end."""
+    if name == "textured_small":
+        ds = G.device_scene(gpu_ctx, "textured_small_mk", lambda: S.textured_instances(
+            width=100, height=60, spp=2, bounces=4, num_instances=60, tex_size=64, tri_range=(2000, 20000)))
+    else:
+        ds = G.device_scene(gpu_ctx, name)
+    sc = ds.scene
+    out = []
+    for mode in (rt.MODE_MEGAKERNEL, rt.MODE_WAVEFRONT):
+        gpu_ctx.set_option(rt.OPTION_MEGAKERNEL_PERSISTENT, 0)
+        gpu_ctx.set_option(rt.OPTION_WAVEFRONT_STREAMS, streams)
+        gpu_ctx.set_option(rt.OPTION_WAVEFRONT_FINISH, 1 << 30)
+        acc = rt.new_accumulation(sc.width, sc.height)
+        disp = torch.zeros((sc.height, sc.width, 4), dtype=torch.uint8, device="cuda")
+        st = rt.trace_rays(gpu_ctx, ds.tlas, ds.buffers, sc.ubo, acc, sc.width, sc.height, mode=mode, stats=True,
+                           out=disp)
+        gpu_ctx.sync()
+        out.append((acc.cpu().numpy(), disp.cpu().numpy(), st))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for k in ("rays", "primary_rays", "extension_rays", "shadow_rays", "shade_events"):
+        assert out[0][2][k] == out[1][2][k], k
