@@ -158,7 +158,8 @@ def adjoint_iteration(ctx, steps, warmup):
     metallic-roughness parameter textures and the point light's intensity: forward
     trace, L2 loss gradient, path-replay adjoint (every sample traced twice, nothing
     of the paths stored).  The forward runs with settings autotuned on this scene, in
-    a context of its own.  Returns milliseconds per iteration and its split."""
+    a context of its own, in the faster of the two modes (same image).  Returns
+    milliseconds per iteration and its split."""
     import numpy as np
     import torch
     from rustracer_amd import diff, rt, scenes
@@ -178,8 +179,24 @@ def adjoint_iteration(ctx, steps, warmup):
     tuned["frame_ms"] = round(tuned["frame_ms"], 2)
     ev = None
 
+    def forward_ms(mode):
+        rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=mode)
+        best = None
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=mode)
+            e1.record()
+            torch.cuda.synchronize()
+            best = e0.elapsed_time(e1) if best is None else min(best, e0.elapsed_time(e1))
+        return best
+
+    fmodes = {"wavefront": rt.MODE_WAVEFRONT, "megakernel": rt.MODE_MEGAKERNEL}
+    fwd_probe = {k: round(forward_ms(m), 3) for k, m in fmodes.items()}
+    fmode = min(fwd_probe, key=fwd_probe.get)
+
     def once():
-        rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=rt.MODE_WAVEFRONT)
+        rt.trace_rays(ctx, ds.tlas, ds.buffers, sc.ubo, acc, W, H, mode=fmodes[fmode])
         ev.record()
         g = (2.0 * (acc - target)).contiguous()
         rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads, grad_point_intensity=gi)
@@ -203,11 +220,12 @@ def adjoint_iteration(ctx, steps, warmup):
     st = rt.trace_rays_adjoint(ctx, ds.tlas, ds.buffers, sc.ubo, g, W, H, grads, stats=True, grad_point_intensity=gi)
     return {"ms_per_iter": round(t, 3), "forward_ms": round(f, 3), "adjoint_ms": round(t - f, 3),
             "adjoint_rays": int(st["rays"]), "grad_contributions": int(st["grad_taps"]),
-            "grad_atomics": int(st["grad_atomics"]), "autotune": tuned,
+            "grad_atomics": int(st["grad_atomics"]), "autotune": tuned, "forward_mode": fmode,
+            "forward_probe_ms": fwd_probe,
             "workload": "Cornell box 512x512, 16 spp, depth 4, 512^2 RGBA32F base-colour + metallic-roughness "
-                        "parameter textures and the point light's intensity; autotuned wavefront forward + L2 "
-                        "gradient + path-replay adjoint (each sample traced again, per-wave LDS sums before the "
-                        "float atomics)"}
+                        "parameter textures and the point light's intensity; forward in the faster mode (autotuned "
+                        "wavefront or persistent megakernel, same image) + L2 gradient + path-replay adjoint (each "
+                        "sample traced again, per-wave LDS sums before the float atomics)"}
 
 
 def dynamic_step(ctx, sc, buf, blases, steps):
