@@ -24,10 +24,11 @@ def _set_intensity(buffers, intensity):
 
 class TraceFunction(torch.autograd.Function):
     """accum = trace(intensity, *textures).  `intensity` (float32, one per point light)
-    or None; the parameter textures follow, in the order of `slots`."""
+    or None; the parameter textures follow, in the order of `slots`.  `mode`: the
+    forward's rt.MODE_* (both give the same image; the backward replays the paths)."""
 
     @staticmethod
-    def forward(fctx, rctx, tlas, buffers, ubo, width, height, slots, intensity, *params):
+    def forward(fctx, rctx, tlas, buffers, ubo, width, height, slots, mode, intensity, *params):
         for k, p in zip(slots, params):
             if buffers.textures[k].data_ptr() != p.data_ptr():
                 buffers.set_texture(k, p)
@@ -36,7 +37,7 @@ class TraceFunction(torch.autograd.Function):
             held = intensity.detach().clone()
             _set_intensity(buffers, held)
         accum = rt.new_accumulation(width, height)
-        rt.trace_rays(rctx, tlas, buffers, ubo, accum, width, height, mode=rt.MODE_WAVEFRONT)
+        rt.trace_rays(rctx, tlas, buffers, ubo, accum, width, height, mode=mode)
         fctx.state = (rctx, tlas, buffers, ubo.copy(), width, height, slots, held)
         fctx.save_for_backward(*params)
         return accum
@@ -56,7 +57,7 @@ class TraceFunction(torch.autograd.Function):
         grads = {k: torch.zeros_like(p) for k, p in zip(slots, params)}
         rt.trace_rays_adjoint(rctx, tlas, buffers, ubo, grad_accum.contiguous(), width, height, grads,
                               grad_point_intensity=gi)
-        return (None,) * 7 + (gi,) + tuple(grads[k] for k in slots)
+        return (None,) * 8 + (gi,) + tuple(grads[k] for k in slots)
 
 
 class DifferentiableScene:
@@ -65,10 +66,12 @@ class DifferentiableScene:
 
     slots: texture indices (RGBA32F) to optimise; their initial values come from
     the scene's textures.  lights=True adds `intensity`, a float32 tensor with one
-    entry per point light, initialised from the scene."""
+    entry per point light, initialised from the scene.  mode: the forward trace's
+    rt.MODE_* (the megakernel is the faster one on small frames)."""
 
-    def __init__(self, ctx, scene, slots, lights=False):
+    def __init__(self, ctx, scene, slots, lights=False, mode=rt.MODE_WAVEFRONT):
         self.ctx = ctx
+        self.mode = mode
         self.scene = scene
         self.buffers = rt.Buffers(scene)
         self.blases, self.tlas = rt.create_as(ctx, scene, self.buffers)
@@ -92,4 +95,4 @@ class DifferentiableScene:
         sc = self.scene
         ubo = sc.ubo if ubo is None else ubo
         return TraceFunction.apply(self.ctx, self.tlas, self.buffers, ubo, sc.width, sc.height, self.slots,
-                                   self.intensity, *self.params)
+                                   self.mode, self.intensity, *self.params)

@@ -216,6 +216,9 @@ def test_autograd_function_and_an_optimisation_step(gpu_ctx):
     gpu_ctx.sync()
     dscene = diff.DifferentiableScene(gpu_ctx, sc, [1, 2], lights=True)
     img = dscene.render()
+    # the megakernel forward gives the same image
+    mscene = diff.DifferentiableScene(gpu_ctx, sc, [1, 2], lights=True, mode=rt.MODE_MEGAKERNEL)
+    assert torch.equal(mscene.render().detach(), img.detach())
     loss = ((img - target) ** 2).sum()
     loss.backward()
     g = (2 * (img - target)).detach().contiguous()
