@@ -392,17 +392,18 @@ int rt_trace_rays(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
                   rt_trace_stats* stats);
 
 /* No counterpart in the reference: for callers that cannot allocate device memory
- * during a trace (tight serving loops, harnesses that forbid hipMalloc; it reads
- * queue lengths back and synchronises, so it cannot itself be stream-captured).  Same arguments and the same image as
- * rt_trace_rays(..., RT_MODE_WAVEFRONT, NULL), but calls no hipMalloc / hipFree:
- * the path state and the texture descriptors live in one 64 MiB device array per
+ * during a trace (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same image as rt_trace_rays(..., RT_MODE_PERSISTENT_MEGAKERNEL
+ * or RT_MODE_WAVEFRONT, NULL), but calls no hipMalloc / hipFree: it runs the
+ * persistent megakernel, whose only scratch is its pixel counter, and that counter
+ * and the frame's texture descriptors live in one device array of about 41 KB per
  * GPU, reserved when the library is loaded and kept until the process ends (its
- * address is looked up by rt_create).  Frames larger than the array's slots are
- * traced in passes of whole 8x8 tiles.  Limits: at most 1024 textures
+ * address is looked up by rt_create).  Limits: at most 1024 textures
  * (RT_ERR_INVALID_ARGUMENT beyond), no stats, no stage timing, no graph replay,
  * ubo->debug == 1 is RT_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's
- * array: calls that overlap on one GPU (several contexts or streams) race.
- * Synchronises the context's stream. */
+ * array: calls that overlap on one GPU (several contexts or streams) race.  Queued
+ * on the context's stream like rt_trace_rays (a copy of the descriptors and a clear
+ * of the counter, then the kernel); not meant for stream capture. */
 int rt_trace_rays_noalloc(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float* accum,
                           uint8_t* out_rgba8, uint32_t width, uint32_t height);
 

@@ -337,7 +337,7 @@ int rt_create(rt_ctx** out, int hip_device, void* hip_stream) {
     if (e == hipSuccess) ctx->d_next_pixel = dev_alloc<uint32_t>(1, &e);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     // the device address of the no-allocation scratch array (rt_trace_rays_noalloc)
-    if (e == hipSuccess) e = hipError_t(wavefront_noalloc_init(hip_device));
+    if (e == hipSuccess) e = hipError_t(noalloc_init(hip_device));
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) {
         float lut[256];
@@ -913,21 +913,9 @@ int rt_trace_rays_noalloc(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, float
     fa.sv.textures = nullptr;  // the descriptors go to the scratch array
     fa.o.out = reinterpret_cast<uint32_t*>(out_rgba8);
     fa.o.bgra = ctx->out_bgra ? 1u : 0u;
-    const bool any_hit = !fa.fully_opaque;
-    hipStream_t st = ctx->stream;
-    if (out_rgba8 && b->ubo->mapping == RT_MAPPING_HEAT) {
-        // one thread per pixel, no scratch (see rt_trace_rays); the descriptors still
-        // come from the scratch array
-        std::string err;
-        rc = launch_wavefront_noalloc(ctx->device, fa, b->textures, st, any_hit, ctx->variant, ctx->refill_ext,
-                                      ctx->refill_sh, ctx->finish, &err, /*megakernel=*/true);
-        if (rc) return fail(ctx, RT_ERR_HIP, "noalloc: " + err);
-        return RT_OK;
-    }
     std::string err;
-    rc = launch_wavefront_noalloc(ctx->device, fa, b->textures, st, any_hit, ctx->variant, ctx->refill_ext,
-                                  ctx->refill_sh, ctx->finish, &err, false);
-    if (rc == kWavefrontUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "noalloc: " + err);
+    rc = launch_noalloc(ctx->device, fa, b->textures, ctx->stream, !fa.fully_opaque, &err);
+    if (rc == kNoAllocUnsupported) return fail(ctx, RT_ERR_UNSUPPORTED, "noalloc: " + err);
     if (rc) return fail(ctx, RT_ERR_HIP, "noalloc: " + err + ": " + hipGetErrorString(hipError_t(rc)));
     return RT_OK;
 }

@@ -1,6 +1,6 @@
 // render.h — launch interface of the trace kernels (render.hip, wavefront.hip).
 #pragma once
-
+#include <string>
 #include <vector>
 
 #include "output.h"
@@ -63,5 +63,16 @@ int launch_megakernel(const FrameArgs& fa, hipStream_t st, bool any_hit, bool st
 int launch_ray_queries(const TlasView& tl, const SceneView& sv, const rt_ray* rays, uint32_t n, uint32_t flags,
                        rt_hit* hits, uint32_t seed_b, hipStream_t st, bool any_hit);
 int launch_primary(const FrameArgs& fa, rt_hit* hits, hipStream_t st, bool any_hit);
+
+// No-allocation form (rt_trace_rays_noalloc): the persistent megakernel with its pixel
+// counter and the frame's texture descriptors in a file-scope device array (about
+// 41 KB per device), no stats.  noalloc_init(device) must have run once with that
+// device current (it records the array's address).  Returns kNoAllocUnsupported (err
+// set) for a frame it cannot take.
+constexpr uint32_t kNoAllocMaxTextures = 1024;
+constexpr int kNoAllocUnsupported = -1000;
+int noalloc_init(int device);
+int launch_noalloc(int device, const FrameArgs& fa, const rt_texture* host_textures, hipStream_t st, bool any_hit,
+                   std::string* err);
 
 }  // namespace rt

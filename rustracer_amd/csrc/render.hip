@@ -266,4 +266,62 @@ int launch_primary(const FrameArgs& fa, rt_hit* hits, hipStream_t st, bool any_h
     return int(hipGetLastError());
 }
 
+// ------------------------------------------------------- no-allocation form
+// Callers that may not allocate (serving loops, harnesses that forbid it) trace
+// through launch_noalloc: the persistent megakernel, whose only scratch is its pixel
+// counter, with that counter and the frame's texture descriptors in one file-scope
+// device array instead of memory the library allocates.  Every call on one device
+// shares its array: calls that overlap on one device race.
+namespace {
+constexpr size_t kNoAllocTexBytes = size_t(kNoAllocMaxTextures) * sizeof(rt_texture);
+constexpr size_t kNoAllocCounter = (kNoAllocTexBytes + 255) & ~size_t(255);  // offset of the pixel counter
+constexpr size_t kNoAllocBytes = kNoAllocCounter + 256;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each device (the host-side name is not one).
+static char* g_scratch[kNoAllocMaxDevices] = {};
+
+int noalloc_init(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return int(hipErrorInvalidDevice);
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (g_scratch[device]) return 0;
+    char* p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));  // the current device's copy
+    if (e != hipSuccess) return int(e);
+    if (!p) return int(hipErrorInvalidSymbol);
+    g_scratch[device] = p;
+    return 0;
+}
+
+int launch_noalloc(int device, const FrameArgs& fa_in, const rt_texture* host_textures, hipStream_t st, bool any_hit,
+                   std::string* err) {
+    if (device < 0 || device >= kNoAllocMaxDevices || !g_scratch[device]) {
+        *err = "the device's scratch array was not set up";
+        return kNoAllocUnsupported;
+    }
+    if (fa_in.sv.num_textures > kNoAllocMaxTextures) {
+        *err = "more textures than the scratch array holds";
+        return kNoAllocUnsupported;
+    }
+    char* base = g_scratch[device];
+    FrameArgs fa = fa_in;
+    if (fa.sv.num_textures) {
+        // the descriptors of this frame (the caller's host array), ahead of its kernel
+        const hipError_t e = hipMemcpyAsync(base, host_textures, sizeof(rt_texture) * fa.sv.num_textures,
+                                            hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) {
+            *err = "copy of the texture descriptors";
+            return int(e);
+        }
+        fa.sv.textures = reinterpret_cast<rt_texture*>(base);
+    }
+    const int rc = launch_megakernel(fa, st, any_hit, false, nullptr, reinterpret_cast<uint32_t*>(base + kNoAllocCounter));
+    if (rc) *err = "megakernel launch";
+    return rc;
+}
+
 }  // namespace rt

@@ -39,16 +39,4 @@ void wavefront_set_finish(WavefrontState* s, uint32_t live_paths);
 int launch_wavefront_batch(WavefrontState* s, const FrameArgs* frames, uint32_t count, hipStream_t st, bool any_hit,
                            bool stats, int variant, StageTimer* tm, std::string* err);
 
-// No-allocation form (rt_trace_rays_noalloc): the path state in a file-scope device
-// array of 64 MiB per device, the frame traced in passes of wavefront_noalloc_slots()
-// pixels, one slot set on `st`, no graph, no stats (`megakernel`: the one-thread-per-
-// pixel megakernel instead, which needs only the descriptors).  wavefront_noalloc_init(device)
-// must have run once with that device current (it records the array's address).
-constexpr uint32_t kNoAllocMaxTextures = 1024;
-int wavefront_noalloc_init(int device);
-uint32_t wavefront_noalloc_slots();
-int launch_wavefront_noalloc(int device, const FrameArgs& fa, const rt_texture* host_textures, hipStream_t st,
-                             bool any_hit, int variant, uint32_t refill_ext, uint32_t refill_sh, uint32_t finish_below,
-                             std::string* err, bool megakernel);
-
 }  // namespace rt

@@ -975,143 +975,4 @@ int launch_wavefront_batch(WavefrontState* ws, const FrameArgs* frames, uint32_t
 }
 
 
-// ------------------------------------------------------- no-allocation form
-// The path state lives in one file-scope device array instead of memory the
-// library allocates: callers that may not allocate (serving loops, harnesses that
-// forbid it) trace through launch_wavefront_noalloc.
-// The array holds the texture descriptors of the frame, then the slot arrays of
-// kNoAllocSlots slots; a frame with more pixels is traced in passes of whole 8x8
-// tiles (pixels are independent, so the image is the same).  Every call on one
-// device shares its array: calls that overlap on one device race.
-namespace {
-constexpr size_t kNoAllocBytes = size_t(64) << 20;
-constexpr size_t kNoAllocTexBytes = size_t(kNoAllocMaxTextures) * sizeof(rt_texture);
-constexpr int kNoAllocMaxDevices = 64;
-}  // namespace
-
-static __device__ __align__(256) char scratch[kNoAllocBytes];
-
-// Device address of `scratch` on each device (the host-side name is not one).
-static char* g_scratch[kNoAllocMaxDevices] = {};
-
-int wavefront_noalloc_init(int device) {
-    if (device < 0 || device >= kNoAllocMaxDevices) return int(hipErrorInvalidDevice);
-    static std::mutex mu;
-    std::lock_guard<std::mutex> lk(mu);
-    if (g_scratch[device]) return 0;
-    char* p = nullptr;
-    hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));  // the current device's copy
-    if (e != hipSuccess) return int(e);
-    if (!p) return int(hipErrorInvalidSymbol);
-    g_scratch[device] = p;
-    return 0;
-}
-
-namespace {
-// Slots per pass: the largest multiple of 64 whose arrays fit after the descriptors.
-uint32_t noalloc_slots() {
-    const size_t room = kNoAllocBytes - ((kNoAllocTexBytes + 255) & ~size_t(255));
-    WfArrays probe{};
-    // bytes per slot of the layout (its arrays are padded to 256 B, so measured on a
-    // large count), then down to a count that fits with the padding
-    const size_t per_slot = layout(probe, nullptr, 1u << 20) >> 20;
-    uint32_t n = uint32_t(room / per_slot) & ~63u;
-    while (n > 64u && layout(probe, nullptr, n) > room) n -= 64u;
-    return n;
-}
-
-uint32_t resident_blocks(const void* f) {
-    int dev = 0, cus = 0, per_cu = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWfBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    return uint32_t(cus) * uint32_t(per_cu);
-}
-}  // namespace
-
-uint32_t wavefront_noalloc_slots() { return noalloc_slots(); }
-
-int launch_wavefront_noalloc(int device, const FrameArgs& fa_in, const rt_texture* host_textures, hipStream_t st,
-                             bool any_hit, int variant, uint32_t refill_ext, uint32_t refill_sh, uint32_t finish_below,
-                             std::string* err, bool megakernel) {
-    if (device < 0 || device >= kNoAllocMaxDevices || !g_scratch[device]) {
-        *err = "the device's scratch array was not set up";
-        return kWavefrontUnsupported;
-    }
-    if (variant < 0 || variant >= kNumTravVariants) {
-        *err = "unknown traversal variant";
-        return kWavefrontUnsupported;
-    }
-    if (fa_in.sv.num_textures > kNoAllocMaxTextures) {
-        *err = "more textures than the scratch array holds";
-        return kWavefrontUnsupported;
-    }
-    char* base = g_scratch[device];
-    FrameArgs fa = fa_in;
-    hipError_t e = hipSuccess;
-    if (fa.sv.num_textures) {
-        // the descriptors of this frame (the caller's host array), ahead of its kernels
-        e = hipMemcpyAsync(base, host_textures, sizeof(rt_texture) * fa.sv.num_textures, hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) {
-            *err = "copy of the texture descriptors";
-            return int(e);
-        }
-        fa.sv.textures = reinterpret_cast<rt_texture*>(base);
-    }
-    // the one-thread-per-pixel megakernel form needs no scratch beyond the descriptors
-    if (megakernel || fa.samples == 0u || fa.bounces == 0u) return launch_megakernel(fa, st, any_hit, false, nullptr);
-    const uint32_t cap = noalloc_slots();
-    WfArrays a{};
-    layout(a, base + ((kNoAllocTexBytes + 255) & ~size_t(255)), cap);
-    a.refill_ext = std::min(std::max(refill_ext, 1u), 64u);
-    a.refill_sh = std::min(std::max(refill_sh, 1u), 64u);
-    using K = Kernels<OneFrame>;
-    const typename K::TraceFn k_trace = K::trace(variant, any_hit, false);
-    const typename K::FinishFn k_finish = K::finish(any_hit, false);
-    const uint32_t g_trace = resident_blocks((const void*)k_trace);
-    const uint32_t g_shade = resident_blocks((const void*)k_wf_shade<OneFrame, false>);
-    const uint32_t g_init = resident_blocks((const void*)k_wf_init<OneFrame>);
-    const uint32_t tiles = frame_tiles(fa), per_pass = cap / 64u;
-    const uint64_t max_chunks = uint64_t(fa.samples) * fa.bounces / kCheckEvery + 2;
-    auto no_mark = [](int) {};
-    for (uint32_t t0 = 0; t0 < tiles; t0 += per_pass) {
-        const uint32_t nslots = std::min(per_pass, tiles - t0) * 64u;
-        const SlotSet<OneFrame> set{OneFrame{fa, 1u, t0}, a, nslots};
-        // the counts start at zero (the previous pass or call left its own values)
-        hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st, a.counts);
-        hipLaunchKernelGGL(k_wf_init<OneFrame>, dim3(std::min(g_init, (nslots + kWfBlock - 1) / kWfBlock)),
-                           dim3(kWfBlock), 0, st, set.fr, a, nslots);
-        e = hipGetLastError();
-        for (uint64_t c = 0; e == hipSuccess; c++) {
-            for (uint64_t k = 0; k < kCheckEvery; k++)
-                enqueue_iteration(set, k, st, k_trace, g_trace, g_shade, false, no_mark);
-            uint32_t live4[4] = {0u, 0u, 0u, 0u};
-            e = hipGetLastError();
-            if (e == hipSuccess) e = hipMemcpyAsync(live4, a.counts, sizeof(live4), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) break;
-            // after an even number of iterations: the path queue and the retiring
-            // entries of the last shade's shadow queue
-            const uint32_t live = live4[0] + live4[3];
-            if (live == 0u) break;
-            if (live < finish_below) {
-                const uint32_t blocks = std::min((live + kFinishBlock - 1) / kFinishBlock, 65535u);
-                hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(kFinishBlock), 0, st, set.fr, a, a.q0, a.counts, a.qs1,
-                                   shadow_count(a, 1));
-                e = hipGetLastError();
-                break;
-            }
-            if (c > max_chunks) {
-                *err = "paths did not terminate";
-                return int(hipErrorUnknown);
-            }
-        }
-        if (e != hipSuccess) {
-            *err = "wavefront pass";
-            return int(e);
-        }
-    }
-    return 0;
-}
-
 }  // namespace rt

@@ -426,9 +426,9 @@ def trace_rays(ctx, tlas, buffers, ubo, accum, width, height, mode=MODE_MEGAKERN
 
 
 def trace_rays_noalloc(ctx, tlas, buffers, ubo, accum, width, height, out=None):
-    """rt_trace_rays_noalloc: the wavefront trace of trace_rays with no device
-    allocation (path state in the library's per-GPU scratch array).  Same image as
-    trace_rays(..., mode=MODE_WAVEFRONT)."""
+    """rt_trace_rays_noalloc: the persistent-megakernel trace of trace_rays with no
+    device allocation (its pixel counter and the texture descriptors in the library's
+    per-GPU scratch array).  Same image as trace_rays in any mode."""
     b = buffers.bindings(ubo)
     if out is not None and (out.dtype != _torch().uint8 or not out.is_contiguous() or out.numel() != width * height * 4):
         raise ValueError("out must be a contiguous uint8 tensor of width * height * 4 elements")

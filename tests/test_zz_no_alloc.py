@@ -1,8 +1,8 @@
-"""rt_trace_rays_noalloc: the wavefront trace with no device allocation (path state
-in the library's per-GPU scratch array).  Its images must equal those of
-rt_trace_rays(RT_MODE_WAVEFRONT) bit for bit, on two calls in a row (the second
-smaller, so it runs over what the first left in the array), including a frame
-larger than one pass of the array's slots."""
+"""rt_trace_rays_noalloc: the persistent-megakernel trace with no device allocation
+(pixel counter and texture descriptors in the library's per-GPU scratch array).  Its
+images must equal those of rt_trace_rays(RT_MODE_WAVEFRONT) bit for bit, on two
+calls in a row, the second smaller and of another scene, so it runs over the counter
+and descriptors the first left in the array."""
 import numpy as np
 import pytest
 import torch
@@ -29,7 +29,7 @@ def _both(ctx, ds, ubo, w, h, out=False):
 
 
 def test_noalloc_equals_wavefront_on_two_calls(gpu_ctx):
-    # call 1: 512x512 textured Cornell box (262,144 pixels: more than one pass)
+    # call 1: 512x512 textured Cornell box
     ds = G.device_scene(gpu_ctx, "noalloc_params", factory=lambda: S.cornell_params(width=512, height=512, spp=4))
     sc = ds.scene
     print("noalloc call 1:", sc.width, "x", sc.height, "spp", sc.ubo["number_of_samples"], "textures",
