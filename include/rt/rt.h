@@ -329,7 +329,14 @@ enum {
      * steps the lanes that trace, then shades the others together), so no lane
      * waits each bounce for the slowest traversal of its wave; 0 one lane per pixel
      * in 16x8-pixel workgroups, each traversal run to its end.  Same images. */
-    RT_OPTION_MEGAKERNEL_PERSISTENT = 13
+    RT_OPTION_MEGAKERNEL_PERSISTENT = 13,
+    /* Treelet restructurings of the binary tree of BVHs built afterwards (0 .. 8),
+     * before the collapse to 4-wide nodes: bottom up, each node's treelet of up to
+     * five subtrees is rebuilt as the binary tree of least surface-area-heuristic
+     * cost over them (Karras & Aila, HPG 2013).  0: the Morton-order tree as
+     * emitted, built in less than half the time (for scenes rebuilt every frame).
+     * Same hits.  Default 2. */
+    RT_OPTION_BVH_TREELET_PASSES = 14
 };
 int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
 

@@ -36,6 +36,7 @@ constexpr double kAutotuneMargin = 0.015;  // relative gain a probe needs to rep
 // texels; deeper vertices scatter over the scene).
 constexpr uint32_t kAdjointTableBounces = 1;
 constexpr uint32_t kBlasLeafMax = 2;  // default triangles per leaf (RT_OPTION_BLAS_LEAF_SIZE)
+constexpr uint32_t kTreeletPasses = 2;  // default RT_OPTION_BVH_TREELET_PASSES
 constexpr uint32_t kTlasLeafMax = 1;
 
 std::string g_create_error;
@@ -126,6 +127,7 @@ struct rt_ctx {
     int streams = 2;         // RT_OPTION_WAVEFRONT_STREAMS
     uint32_t leaf_max = kBlasLeafMax;  // RT_OPTION_BLAS_LEAF_SIZE
     uint32_t collapse = kCollapseSah;  // RT_OPTION_BVH_COLLAPSE
+    uint32_t treelets = kTreeletPasses;  // RT_OPTION_BVH_TREELET_PASSES
     bool mega_persistent = true;       // RT_OPTION_MEGAKERNEL_PERSISTENT
     uint32_t* d_next_pixel = nullptr;  // pixel counter of the persistent megakernel and adjoint
     uint32_t finish = 65536; // RT_OPTION_WAVEFRONT_FINISH
@@ -432,7 +434,7 @@ int rt_build_blas_batch(rt_ctx* ctx, const rt_blas_input* inputs, uint32_t count
     LbvhResult res;
     std::string err;
     rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), n, (const uint32_t*)d_seg_start.get(),
-                    seg_start, ctx->leaf_max, ctx->collapse, &res, &err);
+                    seg_start, ctx->leaf_max, ctx->collapse, ctx->treelets, &res, &err);
     if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "BLAS build: " + err);
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
     auto pool = std::make_shared<BlasPool>();
@@ -662,7 +664,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
     LbvhResult res;
     std::string err;
     rc = lbvh_build(st, (const float4*)d_lo.get(), (const float4*)d_hi.get(), nt, (const uint32_t*)d_seg.get(),
-                    seg_start, kTlasLeafMax, ctx->collapse, &res, &err);
+                    seg_start, kTlasLeafMax, ctx->collapse, ctx->treelets, &res, &err);
     if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "TLAS build: " + err);
     DevPtr perm(res.perm), seg_bounds(res.seg_bounds);
     te.nodes.reset(res.nodes);
@@ -713,7 +715,7 @@ int rt_build_tlas(rt_ctx* ctx, const rt_instance* d_instances, uint32_t count, r
         if (rc) return hip_fail(ctx, hipError_t(rc), "gather_flat");
         LbvhResult fres;
         rc = lbvh_build(st, (const float4*)d_flo.get(), (const float4*)d_fhi.get(), n, (const uint32_t*)d_fseg.get(),
-                        fseg, ctx->leaf_max, ctx->collapse, &fres, &err);
+                        fseg, ctx->leaf_max, ctx->collapse, ctx->treelets, &fres, &err);
         if (rc) return fail(ctx, rc == int(hipErrorOutOfMemory) ? RT_ERR_OUT_OF_MEMORY : RT_ERR_HIP, "flattened BVH build: " + err);
         DevPtr fperm(fres.perm), fseg_bounds(fres.seg_bounds);
         te.flat_nodes.reset(fres.nodes);
@@ -1252,6 +1254,11 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value) {
     if (option == RT_OPTION_BVH_COLLAPSE) {
         if (value > 1) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_BVH_COLLAPSE takes 0 or 1");
         ctx->collapse = uint32_t(value);
+        return RT_OK;
+    }
+    if (option == RT_OPTION_BVH_TREELET_PASSES) {
+        if (value > 8) return fail(ctx, RT_ERR_INVALID_ARGUMENT, "RT_OPTION_BVH_TREELET_PASSES takes 0 .. 8");
+        ctx->treelets = uint32_t(value);
         return RT_OK;
     }
     if (option == RT_OPTION_BLAS_LEAF_SIZE) {

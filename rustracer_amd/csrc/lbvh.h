@@ -45,11 +45,12 @@ enum : uint32_t {
 
 // Builds the wide BVH of `n` primitives with boxes (lo[i], hi[i]) split into
 // `nseg` segments (d_seg_start has nseg + 1 entries, on device and host).
-// Segment s's root is node s.  Leaves hold at most leaf_max primitives.  Returns a
-// hipError_t-compatible code.
+// Segment s's root is node s.  Leaves hold at most leaf_max primitives.
+// treelet_passes: bottom-up treelet restructurings of the binary tree before the
+// collapse (0: none).  Returns a hipError_t-compatible code.
 int lbvh_build(hipStream_t stream, const float4* d_lo, const float4* d_hi, uint32_t n,
                const uint32_t* d_seg_start, const std::vector<uint32_t>& h_seg_start,
-               uint32_t leaf_max, uint32_t collapse, LbvhResult* out, std::string* err);
+               uint32_t leaf_max, uint32_t collapse, uint32_t treelet_passes, LbvhResult* out, std::string* err);
 
 // Triangle preparation for a BLAS batch: reads the caller's index and vertex
 // buffers, writes triangle records (input order) and boxes.  Sets *d_bad to 1 if an

@@ -101,12 +101,13 @@ OPTION_WAVEFRONT_FINISH = 9
 OPTION_BLAS_LEAF_SIZE = 10
 OPTION_BVH_COLLAPSE = 12
 OPTION_MEGAKERNEL_PERSISTENT = 13
+OPTION_BVH_TREELET_PASSES = 14
 
 # The value every option has on a new context (rt_create).
 DEFAULT_OPTIONS = {OPTION_STAGE_TIMING: 0, OPTION_FLATTEN_IDENTITY_INSTANCES: 1, OPTION_TRAVERSAL_VARIANT: 0,
                    OPTION_GRAPH: 0, OPTION_EXTEND_REFILL: 16, OPTION_SHADOW_REFILL: 1, OPTION_OUTPUT_BGRA: 0,
                    OPTION_WAVEFRONT_STREAMS: 2, OPTION_WAVEFRONT_FINISH: 65536, OPTION_BLAS_LEAF_SIZE: 2,
-                   OPTION_BVH_COLLAPSE: 1, OPTION_MEGAKERNEL_PERSISTENT: 1}
+                   OPTION_BVH_COLLAPSE: 1, OPTION_MEGAKERNEL_PERSISTENT: 1, OPTION_BVH_TREELET_PASSES: 2}
 TONE_MAP_DEFAULT, TONE_MAP_UNCHARTED, TONE_MAP_HEJL_RICHARD, TONE_MAP_ACES, TONE_MAP_SRGB = 0, 1, 2, 3, 4
 NUM_TRAVERSAL_VARIANTS = 4
 
