@@ -89,6 +89,8 @@ def test_refit_and_option_argument_checks(gpu_ctx):
     lib, h = gpu_ctx.lib, gpu_ctx.handle
     assert lib.rt_refit_tlas(h, 987654) == -7  # unknown TLAS handle
     assert lib.rt_set_option(h, 11, 1) == -1  # no such option
+    assert lib.rt_set_option(h, rt.OPTION_BVH_TREELET_PASSES, 9) == -1  # 0 .. 8
+    assert b"0 .. 8" in lib.rt_last_error(h)
     ds = G.device_scene(gpu_ctx, "cornell_sphere")
     sc = ds.scene
     deep = sc.ubo.copy()
