@@ -310,7 +310,32 @@ def batch_benchmark(ctx, steps):
     ctx.sync()
     st = rt.trace_rays_batch(ctx, items, stats=True)
     rays = st["rays"]
-    # traversal variant for the batch (the context's was tuned on the single frame)
+
+    def timed():
+        rt.trace_rays_batch(ctx, items)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rt.trace_rays_batch(ctx, items)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # the batch's own refill threshold, slot sets and finishing threshold (the
+    # context's were tuned on another frame: the last autotune was the adjoint's)
+    ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, rt.DEFAULT_OPTIONS[rt.OPTION_TRAVERSAL_VARIANT])
+    ctx.set_option(rt.OPTION_GRAPH, 0)
+    t_set = {}
+    for refill in (1, 16):
+        for streams in (2, 3, 4):
+            for finish in (1 << 16, 1 << 18):
+                ctx.set_option(rt.OPTION_EXTEND_REFILL, refill)
+                ctx.set_option(rt.OPTION_WAVEFRONT_STREAMS, streams)
+                ctx.set_option(rt.OPTION_WAVEFRONT_FINISH, finish)
+                t_set[(refill, streams, finish)] = timed()
+    refill, streams, finish = min(t_set, key=t_set.get)
+    ctx.set_option(rt.OPTION_EXTEND_REFILL, refill)
+    ctx.set_option(rt.OPTION_WAVEFRONT_STREAMS, streams)
+    ctx.set_option(rt.OPTION_WAVEFRONT_FINISH, finish)
+    # traversal variant for the batch
     t_var = {}
     for v in range(rt.NUM_TRAVERSAL_VARIANTS):
         ctx.set_option(rt.OPTION_TRAVERSAL_VARIANT, v)
@@ -354,6 +379,7 @@ def batch_benchmark(ctx, steps):
             "batch_ms": round(tb * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
             "batch_ms_eager": round(t_graph[0] * 1e3, 2), "batch_ms_graph": round(t_graph[1] * 1e3, 2),
             "batch_mrays_per_s": round(rays / tb / 1e6, 1), "sequential_mrays_per_s": round(rays / t_seq / 1e6, 1),
+            "extend_refill": refill, "wavefront_streams": streams, "wavefront_finish": finish,
             "traversal_variant": variant, "scene_gen_s": round(gen_s, 1)}
 
 

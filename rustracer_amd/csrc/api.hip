@@ -6,6 +6,7 @@
 // record_raytracing_commands -> trace_rays (crates/examples/gltf_viewer/src/main.rs:244-270,
 // crates/libs/vulkan/src/command.rs:374-392) for frames.  Every entry point returns
 // a status code; nothing aborts.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1090,13 +1091,15 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     const uint32_t keep_re = ctx->refill_ext, keep_rs = ctx->refill_sh;
     const int keep_streams = ctx->streams;
     const uint32_t keep_finish = ctx->finish;
-    // one probe: the fastest of three timed frames of the current settings (after a
-    // warm-up frame), or < 0 when a traversal gave up
+    // one probe: the median of three timed frames of the current settings (after a
+    // warm-up frame), or < 0 when a traversal gave up.  The median, not the fastest:
+    // with several slot sets some settings alternate between fast and slow frames,
+    // and the fastest frame would credit them with their lucky ones.
     auto probe = [&](double& ms_out) -> int {
         int r = rt_trace_rays(ctx, tlas, &bb, (float*)scratch.get(), nullptr, width, height, RT_MODE_WAVEFRONT, nullptr);
         if (r) return r;
         hipError_t er = hipSuccess;
-        float best_ms = -1.f;
+        float rep_ms[3] = {0.f, 0.f, 0.f};
         for (int rep = 0; rep < 3 && er == hipSuccess; rep++) {
             er = hipEventRecord(ctx->ev0, ctx->stream);
             if (er == hipSuccess)
@@ -1107,13 +1110,14 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
             if (er == hipSuccess) er = hipEventSynchronize(ctx->ev1);
             float ms = 0.f;
             if (er == hipSuccess) er = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-            if (er == hipSuccess && (best_ms < 0.f || ms < best_ms)) best_ms = ms;
+            rep_ms[rep] = ms;
         }
+        std::sort(rep_ms, rep_ms + 3);
         uint32_t flags = 0;
         if (er == hipSuccess) er = hipMemcpy(&flags, ctx->d_err, sizeof(flags), hipMemcpyDeviceToHost);
         if (er == hipSuccess && flags) er = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
         if (er != hipSuccess) return hip_fail(ctx, er, "autotune probe");
-        ms_out = flags ? -1.0 : double(best_ms);
+        ms_out = flags ? -1.0 : double(rep_ms[1]);
         return RT_OK;
     };
     // a setting replaces the incumbent only when it is faster by more than the
@@ -1151,31 +1155,27 @@ int rt_autotune(rt_ctx* ctx, rt_tlas tlas, const rt_bindings* b, uint32_t width,
     }
     if (rc == RT_OK && best_v < 0) rc = fail(ctx, RT_ERR_STACK_OVERFLOW, "no traversal variant completed this frame");
     if (rc == RT_OK) ctx->variant = best_v;
-    // 3. concurrent slot sets
+    // 3-4. concurrent slot sets and the live-path count (per set) that hands the tail
+    //    to the finishing launch, together: the best threshold depends on the number
+    //    of sets (on the 1080p benchmark frame, 2 sets finishing below 262,144 paths
+    //    beat 3 sets at every threshold, while at 65,536 paths 3 sets beat 2)
     if (rc == RT_OK) {
-        int chosen = ctx->streams;
+        const uint32_t finish[] = {0u, 1u << 16, 1u << 17, 1u << 18};
+        int chosen_n = ctx->streams;
+        uint32_t chosen_f = ctx->finish;
         for (int n = 1; n <= wavefront_max_streams() && rc == RT_OK; n++) {
-            if (n == chosen) continue;
-            ctx->streams = n;
-            double ms = -1.0;
-            rc = probe(ms);
-            if (rc == RT_OK && consider(ms)) { best = ms; chosen = n; }
+            for (uint32_t f : finish) {
+                if (n == chosen_n && f == chosen_f) continue;
+                ctx->streams = n;
+                ctx->finish = f;
+                double ms = -1.0;
+                rc = probe(ms);
+                if (rc) break;
+                if (consider(ms)) { best = ms; chosen_n = n; chosen_f = f; }
+            }
         }
-        ctx->streams = chosen;
-    }
-    // 4. live-path count that hands the tail to the finishing launch
-    if (rc == RT_OK) {
-        const uint32_t finish[] = {0u, 1u << 14, 1u << 16, 1u << 17, 1u << 18};
-        uint32_t chosen = ctx->finish;
-        for (uint32_t f : finish) {
-            if (f == chosen) continue;
-            ctx->finish = f;
-            double ms = -1.0;
-            rc = probe(ms);
-            if (rc) break;
-            if (consider(ms)) { best = ms; chosen = f; }
-        }
-        ctx->finish = chosen;
+        ctx->streams = chosen_n;
+        ctx->finish = chosen_f;
     }
     // 5. graph replay
     if (rc == RT_OK) {
