@@ -311,13 +311,15 @@ def batch_benchmark(ctx, steps):
     st = rt.trace_rays_batch(ctx, items, stats=True)
     rays = st["rays"]
 
-    def timed():
+    def timed(reps=2):
+        """mean time of `reps` batches after a warm-up batch"""
         rt.trace_rays_batch(ctx, items)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        rt.trace_rays_batch(ctx, items)
+        for _ in range(reps):
+            rt.trace_rays_batch(ctx, items)
         torch.cuda.synchronize()
-        return time.perf_counter() - t0
+        return (time.perf_counter() - t0) / reps
 
     # the batch's own refill threshold, slot sets and finishing threshold (the
     # context's were tuned on another frame: the last autotune was the adjoint's)

@@ -351,7 +351,8 @@ int rt_set_option(rt_ctx* ctx, uint32_t option, uint64_t value);
  * launch (every pair of 1 .. 4 sets and 0, 65,536, 131,072, 262,144 paths), graph
  * replay, and the traversal variant once more under those settings, keeping the
  * fastest settings on this context (the RT_OPTION_* values).  Each probe takes the
- * median of three timed frames after a warm-up frame, and a setting
+ * mean of three timed frames after a warm-up frame (of six when the three differ by
+ * more than 3 %), and a setting
  * replaces the one it is compared with only when it is more than 1.5 % faster.
  * A variant whose stack ring gives up on a ray of this frame (restart budget) is
  * not a candidate.  Fills *out (may be NULL).  Synchronises the stream. */
